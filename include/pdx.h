@@ -1,0 +1,167 @@
+/* pdx - MI355X Projective-Dynamics substep engine, C ABI.
+ *
+ * Drop-in for the reference's substep and global solve (12and94/pd_ipc):
+ *   int pd::stepOnce(SimContext&)                       core/sim/Integrator.h:14
+ *   IGlobalSolver::analyze/factorize/solve/solveComponent/stats
+ *                                                       core/solver/IGlobalSolver.h:45-127
+ *   EigenDirectSolver (AMD + LDL^T, compact shared factor)
+ *                                                       core/solver/EigenDirectSolver.h:33-54
+ *
+ * Conventions: every entry point returns an int status (PDX_OK or a negative
+ * code) where the reference aborts; host buffers are caller-owned and copied;
+ * device memory is owned by the context; there is no global mutable state, so
+ * distinct contexts may be driven from distinct host threads.  Calls on one
+ * context must not overlap.
+ *
+ * A context holds one or more cloth instances that step together in one
+ * persistent launch (one workgroup per instance).  A context with one instance
+ * in PDX_PREC_FP64 is the plain drop-in for one SimContext.
+ */
+#ifndef PDX_H
+#define PDX_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  PDX_OK = 0,
+  PDX_EINVAL = -1,      /* bad argument (sizes, indices, NULL) */
+  PDX_ENOTREADY = -2,   /* solve before factor, or no instance */
+  PDX_ENOTSPD = -3,     /* LDL^T hit a zero pivot (EigenDirectSolver.cpp:284-291) */
+  PDX_EHIP = -4,        /* HIP runtime error */
+  PDX_EUNSUPPORTED = -5,/* component factors differ / size beyond the build's limits */
+  PDX_ENOMEM = -6
+};
+
+enum {
+  PDX_PREC_FP64 = 0,    /* reference arithmetic in FP64 (the drop-in) */
+  PDX_PREC_MIXED32 = 1  /* FP32 factor/solve on the update, FP64 state, gather, residual */
+};
+
+enum {
+  PDX_SOLVE_EXACT = 0,  /* level-scheduled gather sweeps in the reference's sum order */
+  PDX_SOLVE_FAST = 1    /* chain-blocked sweeps (forward order kept, backward reordered) */
+};
+
+typedef struct pdx_ctx pdx_ctx;
+
+/* Mirrors SceneConfig (core/sim/Scene.h:14-137): the fields the substep reads. */
+typedef struct {
+  double dt;               /* substep h */
+  double gravity[3];
+  double damping;          /* velocityDamping k_d */
+  int max_iterations;      /* PD iterations per substep */
+  double rel_tol;          /* relTolerance (<=0 disables) */
+  double abs_tol;          /* absTolerance (<=0 disables) */
+  double residual_tol;     /* residualTolerance (<=0 disables; then the displacement test decides) */
+} pdx_config;
+
+/* One cloth: mirrors Mesh (core/mesh/Mesh.h:96-217).  velocities/pin_positions may
+ * be NULL (zero / current positions); bends may be absent (n_bends = 0). */
+typedef struct {
+  int n_vertices;
+  const double* positions;      /* 3n, xyz interleaved (dof 3v+d) */
+  const double* velocities;     /* 3n or NULL */
+  const double* masses;         /* n */
+  const uint8_t* pinned;        /* n, 1 = pinned */
+  const double* pin_positions;  /* 3n or NULL */
+  int n_edges;                  /* distance + shear constraints (Edge, Mesh.h:21-26) */
+  const int32_t* edge_a;
+  const int32_t* edge_b;
+  const double* edge_rest;
+  const double* edge_kappa;
+  int n_bends;                  /* linear midpoint bend stencils (BendStencil, Mesh.h:53-58) */
+  const int32_t* bend_a;
+  const int32_t* bend_b;
+  const int32_t* bend_c;
+  const double* bend_k;
+} pdx_scene;
+
+/* Per-instance statistics (Scene.h:161-169 + IGlobalSolver::Stats). */
+typedef struct {
+  int iterations_used;     /* last substep */
+  int converged;           /* last substep reached the criterion */
+  int early_exit_count;    /* substeps that converged before max_iterations */
+  int factorize_count;     /* numeric factorisations so far */
+  int analyze_count;       /* symbolic analyses so far */
+  int color_count;         /* edge colours */
+  long long total_iterations;
+  long long substeps;
+  double last_residual;    /* m/s^2; negative when the residual test is off */
+  long long factor_nnz;    /* strictly-lower nnz of one component (reference factorNnz/3) */
+  int fwd_levels;          /* level count of the forward sweep */
+  int bwd_levels;
+  int fwd_steps;           /* barrier-separated steps actually launched per sweep */
+  int bwd_steps;
+  double analyze_seconds;
+  double factorize_seconds;
+} pdx_stats;
+
+/* --- context ---------------------------------------------------------------- */
+int pdx_create(pdx_ctx** out, int device, int precision, int solve_mode);
+int pdx_destroy(pdx_ctx* ctx);
+const char* pdx_last_error(const pdx_ctx* ctx);
+
+/* Adds an instance (topology upload: Scene.cpp makeScene's products).  Returns
+ * its index through *instance.  Must precede the first step. */
+int pdx_add_instance(pdx_ctx* ctx, const pdx_scene* scene, const pdx_config* cfg, int* instance);
+int pdx_instance_count(const pdx_ctx* ctx);
+
+/* Analyse + factor every instance whose stamp changed (host FP64 AMD + LDL^T,
+ * Assembler.cpp:29-92 stamps), build the sweep schedules, upload.  stepping
+ * calls this implicitly, as stepOnce does on its first call. */
+int pdx_factor(pdx_ctx* ctx);
+
+/* Runs nsub substeps on every instance (Integrator.cpp:312-797 per substep) in one
+ * persistent launch.  Blocks until done. */
+int pdx_step(pdx_ctx* ctx, int nsub);
+/* Same, asynchronous on the context's stream (no host synchronisation). */
+int pdx_step_async(pdx_ctx* ctx, int nsub);
+int pdx_sync(pdx_ctx* ctx);
+/* Captures `nsub` substeps as a HIP graph and replays it `replays` times. */
+int pdx_step_graph(pdx_ctx* ctx, int nsub, int replays);
+
+/* --- state ------------------------------------------------------------------ */
+int pdx_get_state(pdx_ctx* ctx, int instance, double* positions, double* velocities);
+int pdx_set_state(pdx_ctx* ctx, int instance, const double* positions, const double* velocities);
+/* Moves pins without refactoring (pin positions are not in the stamp). */
+int pdx_set_pin_positions(pdx_ctx* ctx, int instance, const double* pin_positions);
+/* Changes edge stiffness; the next step refactors once (stamp change). */
+int pdx_set_edge_kappa(pdx_ctx* ctx, int instance, const double* kappa);
+int pdx_set_config(pdx_ctx* ctx, int instance, const pdx_config* cfg);
+int pdx_get_stats(pdx_ctx* ctx, int instance, pdx_stats* out);
+
+/* --- global solve (EigenDirectSolver::solve, EigenDirectSolver.cpp:299-357) ---- */
+/* x = L^{-1} b for one instance, all three components, on the GPU. */
+int pdx_solve3(pdx_ctx* ctx, int instance, const double* b, double* x);
+/* Host copy of the factor (rank space): ptr[m+1], rows[nnz], vals[nnz], dinv[m],
+ * rank_vertex[m], perm3[3m]; any pointer may be NULL.  *m / *nnz receive sizes. */
+int pdx_get_factor(pdx_ctx* ctx, int instance, int* m, long long* nnz, int32_t* ptr,
+                   int32_t* rows, double* vals, double* dinv, int32_t* rank_vertex,
+                   int32_t* perm3);
+
+/* --- timing ---------------------------------------------------------------- */
+/* Average duration (ms) of the substep kernel over the last `pdx_step*` call,
+ * measured with HIP events on the stream the kernel runs on. */
+double pdx_last_kernel_ms(const pdx_ctx* ctx);
+/* Bytes of factor + schedule + scene arrays the kernel streams per PD iteration,
+ * summed over instances (algorithmic, each array once per phase). */
+double pdx_bytes_per_iteration(const pdx_ctx* ctx);
+
+/* --- scene builders (grid cloth, Mesh.cpp:12-54, :183-207, :219-285) ---------- */
+/* Fills caller buffers for an nx*ny grid in the xz-plane, spacing sx, sz.
+ * Edges: row/column neighbours in the reference order, then (shear>0) one
+ * diagonal per quad; bends (bend>0): rows then columns.  Sizes:
+ *   n = nx*ny, E = 2nxny-nx-ny (+(nx-1)(ny-1)), B = (nx-2)ny+(ny-2)nx.
+ * Pass NULL arrays to query sizes only. */
+int pdx_grid_counts(int nx, int ny, int shear, int bend, int* n, int* E, int* B);
+int pdx_grid_fill(int nx, int ny, double sx, double sz, double kappa, double shear_kappa,
+                  double bend_k, double* positions, int32_t* ea, int32_t* eb, double* rest,
+                  double* kap, int32_t* sa, int32_t* sb, int32_t* sc, double* bk);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

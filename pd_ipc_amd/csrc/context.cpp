@@ -1,0 +1,791 @@
+// C ABI of the substep engine: contexts, instances, the factor-once stamp logic
+// (core/sim/Integrator.cpp:341-372), packing of the sweep schedules into the
+// device arena, and kernel launches.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/pdx.h"
+#include "devtypes.h"
+#include "host.h"
+
+namespace pdx {
+hipError_t launchSubstep(const InstDev* d, const int* order, int ninst, int nsub, int block,
+                         size_t lds, bool fp32, hipStream_t s);
+hipError_t launchSolve(const InstDev* d, int inst, const double* b, double* x, int block,
+                       size_t lds, bool fp32, hipStream_t s);
+hipError_t setLdsLimit(size_t bytes);
+}  // namespace pdx
+
+using namespace pdx;
+
+namespace {
+
+constexpr size_t kLdsLimit = 160 * 1024;
+constexpr size_t kRedBytes = 32 * sizeof(double);
+
+struct HostSweep {
+  std::vector<int> levChunk, chunkRows, chunkRowBase, chunkEntBase, slotRank, slotLen, col;
+  std::vector<double> val;
+  int nlev = 0;
+};
+
+// Lays one sweep out level by level, <=64-row chunks sorted by row length
+// (descending), entries jagged-diagonal inside a chunk.
+void packSweep(const Sweep& sw, bool fwd, const Factor& f, HostSweep& out) {
+  out = HostSweep{};
+  out.nlev = sw.nlev;
+  out.levChunk.push_back(0);
+  std::vector<int> rows;
+  for (int l = 0; l < sw.nlev; ++l) {
+    rows.assign(sw.order.begin() + sw.levPtr[(size_t)l], sw.order.begin() + sw.levPtr[(size_t)l + 1]);
+    auto lenOf = [&](int k) {
+      return fwd ? f.rp[(size_t)k + 1] - f.rp[(size_t)k] : f.cp[(size_t)k + 1] - f.cp[(size_t)k];
+    };
+    std::stable_sort(rows.begin(), rows.end(), [&](int a, int b) { return lenOf(a) > lenOf(b); });
+    for (size_t c0 = 0; c0 < rows.size(); c0 += 64) {
+      const size_t c1 = std::min(rows.size(), c0 + 64);
+      out.chunkRows.push_back((int)(c1 - c0));
+      out.chunkRowBase.push_back((int)out.slotRank.size());
+      out.chunkEntBase.push_back((int)out.col.size());
+      int maxlen = 0;
+      for (size_t t = c0; t < c1; ++t) {
+        out.slotRank.push_back(rows[t]);
+        out.slotLen.push_back(lenOf(rows[t]));
+        maxlen = std::max(maxlen, lenOf(rows[t]));
+      }
+      for (int j = 0; j < maxlen; ++j)
+        for (size_t t = c0; t < c1; ++t) {
+          const int k = rows[t];
+          if (j >= lenOf(k)) continue;
+          if (fwd) {
+            out.col.push_back(f.rc[(size_t)f.rp[(size_t)k] + j]);
+            out.val.push_back(f.rx[(size_t)f.rp[(size_t)k] + j]);
+          } else {
+            out.col.push_back(f.ci[(size_t)f.cp[(size_t)k] + j]);
+            out.val.push_back(f.cx[(size_t)f.cp[(size_t)k] + j]);
+          }
+        }
+    }
+    out.levChunk.push_back((int)out.chunkRows.size());
+  }
+}
+
+struct Instance {
+  Scene scene;
+  Config cfg;
+  Topology topo;
+  Matrix A;
+  Factor fac;
+  std::vector<double> brhs;
+  bool topoBuilt = false, analyzed = false, factored = false;
+  uint64_t numStamp = 0, strStamp = 0;
+  int factorizeCount = 0, analyzeCount = 0;
+  double analyzeSec = 0, factorSec = 0;
+  HostSweep hf, hb;
+  bool packed = false;
+  // device addresses of the writable state (inside the arena)
+  double *dx = nullptr, *dv = nullptr, *dpin = nullptr, *dbrhs = nullptr;
+  int* distats = nullptr;
+  long long* dlstats = nullptr;
+  double* ddstats = nullptr;
+  double bytesPerIter = 0;
+};
+
+// Byte arena: one host staging buffer mirrored by one device allocation.
+struct Arena {
+  std::vector<char> host;
+  size_t add(const void* p, size_t bytes, size_t align = 256) {
+    size_t off = (host.size() + align - 1) / align * align;
+    host.resize(off + std::max<size_t>(bytes, 1));
+    if (p && bytes) std::memcpy(host.data() + off, p, bytes);
+    return off;
+  }
+  template <typename T>
+  size_t vec(const std::vector<T>& v) {
+    return add(v.data(), v.size() * sizeof(T));
+  }
+};
+
+}  // namespace
+
+struct pdx_ctx {
+  int device = 0;
+  int prec = PDX_PREC_FP64;
+  int mode = PDX_SOLVE_EXACT;
+  hipStream_t stream = nullptr;
+  std::vector<Instance> inst;
+  char* dbuf = nullptr;
+  size_t dbytes = 0;
+  InstDev* dInst = nullptr;
+  int* dOrder = nullptr;
+  std::vector<InstDev> hInst;
+  bool uploaded = false;
+  bool dirty = true;
+  bool ldsW = true;
+  size_t lds = 0;
+  int block = 256;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double lastKernelMs = 0.0;
+  int lastLaunches = 0;
+  double bytesIter = 0.0;
+  std::string err;
+  double* dScratchB = nullptr;
+  double* dScratchX = nullptr;
+  int scratchN = 0;
+};
+
+namespace {
+
+int fail(pdx_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HIPCHECK(ctx, expr)                                                          \
+  do {                                                                               \
+    hipError_t e_ = (expr);                                                          \
+    if (e_ != hipSuccess) return fail(ctx, PDX_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+double now() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+void fillConfig(Config& c, const pdx_config* p) {
+  c.dt = p->dt;
+  for (int d = 0; d < 3; ++d) c.g[d] = p->gravity[d];
+  c.damping = p->damping;
+  c.maxIter = p->max_iterations;
+  c.relTol = p->rel_tol;
+  c.absTol = p->abs_tol;
+  c.resTol = p->residual_tol;
+}
+
+int validateScene(const pdx_scene* s) {
+  if (!s || s->n_vertices <= 0 || !s->positions || !s->masses || !s->pinned) return PDX_EINVAL;
+  if (s->n_edges < 0 || s->n_bends < 0) return PDX_EINVAL;
+  if (s->n_edges > 0 && (!s->edge_a || !s->edge_b || !s->edge_rest || !s->edge_kappa)) return PDX_EINVAL;
+  if (s->n_bends > 0 && (!s->bend_a || !s->bend_b || !s->bend_c || !s->bend_k)) return PDX_EINVAL;
+  for (int e = 0; e < s->n_edges; ++e) {
+    const int a = s->edge_a[e], b = s->edge_b[e];
+    if (a < 0 || b < 0 || a >= s->n_vertices || b >= s->n_vertices || a == b) return PDX_EINVAL;
+  }
+  for (int q = 0; q < s->n_bends; ++q) {
+    const int a = s->bend_a[q], b = s->bend_b[q], c = s->bend_c[q];
+    if (a < 0 || b < 0 || c < 0 || a >= s->n_vertices || b >= s->n_vertices || c >= s->n_vertices) return PDX_EINVAL;
+  }
+  return PDX_OK;
+}
+
+// Downloads the device state of every instance into the host scenes (before a
+// re-upload) so that stepping continues from where it was.
+int pullState(pdx_ctx* c) {
+  if (!c->uploaded) return PDX_OK;
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  for (auto& I : c->inst) {
+    HIPCHECK(c, hipMemcpy(I.scene.x.data(), I.dx, I.scene.x.size() * 8, hipMemcpyDeviceToHost));
+    HIPCHECK(c, hipMemcpy(I.scene.v.data(), I.dv, I.scene.v.size() * 8, hipMemcpyDeviceToHost));
+  }
+  return PDX_OK;
+}
+
+double sweepBytes(const HostSweep& s, size_t w) {
+  return (double)s.col.size() * (4 + w) + (double)s.slotRank.size() * 8 + (double)s.chunkRows.size() * 12 +
+         (double)s.levChunk.size() * 4;
+}
+
+// Factor every instance whose stamps changed (parallel over instances), repack.
+int refreshFactors(pdx_ctx* c, bool& changed) {
+  changed = false;
+  const int ni = (int)c->inst.size();
+  std::vector<int> todo;
+  for (int i = 0; i < ni; ++i) {
+    Instance& I = c->inst[(size_t)i];
+    const uint64_t ns = numericStamp(I.scene, I.cfg.dt);
+    if (!I.factored || ns != I.numStamp) todo.push_back(i);
+  }
+  if (todo.empty()) return PDX_OK;
+  changed = true;
+  std::vector<int> rcs(todo.size(), 0);
+  std::atomic<size_t> next{0};
+  auto worker = [&]() {
+    for (;;) {
+      const size_t t = next.fetch_add(1);
+      if (t >= todo.size()) return;
+      Instance& I = c->inst[(size_t)todo[t]];
+      const double t0 = now();
+      if (!I.topoBuilt) {
+        buildTopology(I.scene, I.topo);
+        I.topoBuilt = true;
+      }
+      assembleMatrix(I.scene, I.cfg.dt, I.A);
+      bendingRhs(I.scene, I.brhs);
+      const uint64_t ss = structureStamp(I.scene);
+      int rc = 0;
+      if (!I.analyzed || ss != I.strStamp) {
+        rc = analyzeAndFactor(I.scene, I.A, I.fac);
+        if (rc == 0) {
+          I.analyzed = true;
+          I.strStamp = ss;
+          I.analyzeCount += 1;
+        }
+        I.analyzeSec = now() - t0;
+      } else {
+        rc = refactorNumeric(I.A, I.fac);
+      }
+      if (rc == 0) {
+        I.factorizeCount += 1;
+        I.factored = true;
+        I.numStamp = numericStamp(I.scene, I.cfg.dt);
+        packSweep(I.fac.fwd, true, I.fac, I.hf);
+        packSweep(I.fac.bwd, false, I.fac, I.hb);
+        I.packed = true;
+      }
+      I.factorSec = now() - t0;
+      rcs[t] = rc;
+    }
+  };
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  const int nth = (int)std::min<size_t>(todo.size(), std::min(16u, hw));
+  std::vector<std::thread> pool;
+  for (int t = 1; t < nth; ++t) pool.emplace_back(worker);
+  worker();
+  for (auto& th : pool) th.join();
+  for (size_t t = 0; t < todo.size(); ++t)
+    if (rcs[t] != 0)
+      return fail(c, rcs[t], "factorisation failed for instance " + std::to_string(todo[t]) +
+                                 (rcs[t] == PDX_ENOTSPD ? " (zero pivot: not positive definite)"
+                                                        : " (component factors differ)"));
+  return PDX_OK;
+}
+
+template <typename T>
+std::vector<T> castVec(const std::vector<double>& v) {
+  return std::vector<T>(v.begin(), v.end());
+}
+
+// Builds the device arena of all instances and uploads it.
+int upload(pdx_ctx* c) {
+  const bool fp32 = c->prec == PDX_PREC_MIXED32;
+  const size_t w = fp32 ? 4 : 8;
+  size_t maxW = 0;
+  for (auto& I : c->inst) maxW = std::max(maxW, (size_t)3 * I.fac.m * w);
+  c->ldsW = maxW + kRedBytes <= kLdsLimit;
+  c->lds = kRedBytes + (c->ldsW ? maxW : 0);
+  Arena ar;
+  struct Off {
+    size_t x, v, xbs, xhat, prev, mass, pinned, pinpos, brhs, ea, eb, rest, kappa, vstart, vedges, vedgesC,
+        bstart, bst, bw, sa, sb, sc, bk, vrank, rankv, dinv, W, ist, lst, dst;
+    size_t f[9], b[9];
+  };
+  std::vector<Off> offs(c->inst.size());
+  c->bytesIter = 0;
+  for (size_t i = 0; i < c->inst.size(); ++i) {
+    Instance& I = c->inst[i];
+    const Scene& s = I.scene;
+    Off& o = offs[i];
+    const size_t n3 = (size_t)3 * s.n;
+    o.x = ar.vec(s.x);
+    o.v = ar.vec(s.v);
+    o.xbs = ar.add(nullptr, n3 * 8);
+    o.xhat = ar.add(nullptr, n3 * 8);
+    o.prev = ar.add(nullptr, n3 * 8);
+    o.mass = ar.vec(s.mass);
+    o.pinned = ar.vec(s.pinned);
+    o.pinpos = ar.vec(s.pinpos);
+    bool anyBrhs = false;
+    for (double b : I.brhs) anyBrhs = anyBrhs || b != 0.0;
+    o.brhs = (s.B > 0) ? ar.vec(I.brhs) : 0;
+    (void)anyBrhs;
+    o.ea = ar.vec(s.ea);
+    o.eb = ar.vec(s.eb);
+    o.rest = ar.vec(s.rest);
+    o.kappa = ar.vec(s.kappa);
+    std::vector<int> vs(I.topo.vstart.begin(), I.topo.vstart.end());
+    std::vector<int> ve(I.topo.vedges.begin(), I.topo.vedges.end());
+    std::vector<int> vc(I.topo.vedgesColor.begin(), I.topo.vedgesColor.end());
+    std::vector<int> bs(I.topo.bstart.begin(), I.topo.bstart.end());
+    std::vector<int> bst(I.topo.bstencil.begin(), I.topo.bstencil.end());
+    o.vstart = ar.vec(vs);
+    o.vedges = ar.vec(ve);
+    o.vedgesC = ar.vec(vc);
+    o.bstart = ar.vec(bs);
+    o.bst = ar.vec(bst);
+    o.bw = ar.vec(I.topo.bweight);
+    o.sa = ar.vec(s.sa);
+    o.sb = ar.vec(s.sb);
+    o.sc = ar.vec(s.sc);
+    o.bk = ar.vec(s.bk);
+    o.vrank = ar.vec(I.fac.vertexRank[0]);
+    o.rankv = ar.vec(I.fac.rankVertex[0]);
+    o.dinv = fp32 ? ar.vec(castVec<float>(I.fac.dinv)) : ar.vec(I.fac.dinv);
+    HostSweep* hs[2] = {&I.hf, &I.hb};
+    size_t* fo[2] = {o.f, o.b};
+    for (int q = 0; q < 2; ++q) {
+      HostSweep& h = *hs[q];
+      fo[q][0] = ar.vec(h.levChunk);
+      fo[q][1] = ar.vec(h.chunkRows);
+      fo[q][2] = ar.vec(h.chunkRowBase);
+      fo[q][3] = ar.vec(h.chunkEntBase);
+      fo[q][4] = ar.vec(h.slotRank);
+      fo[q][5] = ar.vec(h.slotLen);
+      fo[q][6] = ar.vec(h.col);
+      fo[q][7] = fp32 ? ar.vec(castVec<float>(h.val)) : ar.vec(h.val);
+    }
+    o.W = c->ldsW ? 0 : ar.add(nullptr, (size_t)3 * I.fac.m * w);
+    std::vector<int> ist(4, 0);
+    std::vector<long long> lst(2, 0);
+    std::vector<double> dst(2, -1.0);
+    o.ist = ar.vec(ist);
+    o.lst = ar.vec(lst);
+    o.dst = ar.vec(dst);
+    // algorithmic bytes per PD iteration: both sweeps + dinv + maps, the gather's
+    // scene arrays twice (rhs + residual), the per-vertex state passes
+    const double nb = (double)s.n, Eb = (double)s.E, Bb = (double)s.B;
+    double bi = sweepBytes(I.hf, w) + sweepBytes(I.hb, w) + nb * (w + 8);
+    bi += 2.0 * (Eb * (8 + 16) + (nb + 1) * 4 + 2 * Eb * 4);
+    bi += nb * 24 * 4 + nb * 8 + nb;
+    if (Bb > 0) bi += Bb * (12 + 8) + (nb + 1) * 4 + 3 * Bb * 5 + nb * 24;
+    I.bytesPerIter = bi;
+    c->bytesIter += bi;
+  }
+  const size_t instOff = ar.add(nullptr, sizeof(InstDev) * c->inst.size());
+  // LPT order: heaviest instance (factor entries) first
+  std::vector<int> order(c->inst.size());
+  for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    return c->inst[(size_t)a].fac.nnz > c->inst[(size_t)b].fac.nnz;
+  });
+  const size_t orderOff = ar.vec(order);
+
+  HIPCHECK(c, hipSetDevice(c->device));
+  if (c->dbuf) {
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    HIPCHECK(c, hipFree(c->dbuf));
+    c->dbuf = nullptr;
+  }
+  HIPCHECK(c, hipMalloc(&c->dbuf, ar.host.size()));
+  c->dbytes = ar.host.size();
+  char* B = c->dbuf;
+  c->hInst.assign(c->inst.size(), InstDev{});
+  for (size_t i = 0; i < c->inst.size(); ++i) {
+    Instance& I = c->inst[i];
+    const Off& o = offs[i];
+    InstDev& d = c->hInst[i];
+    const Scene& s = I.scene;
+    d.n = s.n;
+    d.E = s.E;
+    d.B = s.B;
+    d.m = I.fac.m;
+    d.maxIter = I.cfg.maxIter;
+    d.wantResidual = I.cfg.resTol > 0.0 ? 1 : 0;
+    d.hasBendRhs = s.B > 0;
+    d.ldsW = c->ldsW ? 1 : 0;
+    const double h = I.cfg.dt;
+    d.h = h;
+    d.invH = 1.0 / h;
+    d.invH2 = 1.0 / (h * h);
+    d.dampInvH = (1.0 - I.cfg.damping) * (1.0 / h);
+    for (int q = 0; q < 3; ++q) d.gh2[q] = I.cfg.g[q] * (h * h);
+    const double gl = std::sqrt(I.cfg.g[0] * I.cfg.g[0] + I.cfg.g[1] * I.cfg.g[1] + I.cfg.g[2] * I.cfg.g[2]);
+    d.gScaleH2 = gl * h * h;
+    d.relTol = I.cfg.relTol;
+    d.absTol = I.cfg.absTol;
+    d.resTol = I.cfg.resTol;
+    d.resThresh = I.cfg.resTol * std::max(gl, 1.0e-12);
+    d.x = (double*)(B + o.x);
+    d.vel = (double*)(B + o.v);
+    d.xbs = (double*)(B + o.xbs);
+    d.xhat = (double*)(B + o.xhat);
+    d.prev = (double*)(B + o.prev);
+    d.mass = (const double*)(B + o.mass);
+    d.pinned = (const uint8_t*)(B + o.pinned);
+    d.pinpos = (const double*)(B + o.pinpos);
+    d.brhs = s.B > 0 ? (const double*)(B + o.brhs) : nullptr;
+    d.ea = (const int*)(B + o.ea);
+    d.eb = (const int*)(B + o.eb);
+    d.rest = (const double*)(B + o.rest);
+    d.kappa = (const double*)(B + o.kappa);
+    d.vstart = (const int*)(B + o.vstart);
+    d.vedges = (const int*)(B + o.vedges);
+    d.vedgesColor = (const int*)(B + o.vedgesC);
+    d.bstart = (const int*)(B + o.bstart);
+    d.bstencil = (const int*)(B + o.bst);
+    d.bweight = (const int8_t*)(B + o.bw);
+    d.sa = (const int*)(B + o.sa);
+    d.sb = (const int*)(B + o.sb);
+    d.sc = (const int*)(B + o.sc);
+    d.bk = (const double*)(B + o.bk);
+    d.vrank = (const int*)(B + o.vrank);
+    d.rankv = (const int*)(B + o.rankv);
+    d.dinv = B + o.dinv;
+    SweepDev* sd[2] = {&d.fwd, &d.bwd};
+    const size_t* fo[2] = {o.f, o.b};
+    const HostSweep* hs[2] = {&I.hf, &I.hb};
+    for (int q = 0; q < 2; ++q) {
+      sd[q]->nlev = hs[q]->nlev;
+      sd[q]->levChunk = (const int*)(B + fo[q][0]);
+      sd[q]->chunkRows = (const int*)(B + fo[q][1]);
+      sd[q]->chunkRowBase = (const int*)(B + fo[q][2]);
+      sd[q]->chunkEntBase = (const int*)(B + fo[q][3]);
+      sd[q]->slotRank = (const int*)(B + fo[q][4]);
+      sd[q]->slotLen = (const int*)(B + fo[q][5]);
+      sd[q]->col = (const int*)(B + fo[q][6]);
+      sd[q]->val = B + fo[q][7];
+    }
+    d.Wg = c->ldsW ? nullptr : (void*)(B + o.W);
+    d.istats = (int*)(B + o.ist);
+    d.lstats = (long long*)(B + o.lst);
+    d.dstats = (double*)(B + o.dst);
+    I.dx = d.x;
+    I.dv = d.vel;
+    I.dpin = (double*)d.pinpos;
+    I.dbrhs = (double*)d.brhs;
+    I.distats = d.istats;
+    I.dlstats = d.lstats;
+    I.ddstats = d.dstats;
+  }
+  std::memcpy(ar.host.data() + instOff, c->hInst.data(), sizeof(InstDev) * c->hInst.size());
+  HIPCHECK(c, hipMemcpy(c->dbuf, ar.host.data(), ar.host.size(), hipMemcpyHostToDevice));
+  c->dInst = (InstDev*)(B + instOff);
+  c->dOrder = (int*)(B + orderOff);
+  HIPCHECK(c, setLdsLimit(kLdsLimit));
+  c->uploaded = true;
+  c->dirty = false;
+  return PDX_OK;
+}
+
+// The HIP stream and events are created on first device use, so that host-only
+// entry points (factor inspection, scene checks) work without a GPU.
+int ensureDevice(pdx_ctx* c) {
+  if (c->stream) return PDX_OK;
+  HIPCHECK(c, hipSetDevice(c->device));
+  HIPCHECK(c, hipStreamCreate(&c->stream));
+  HIPCHECK(c, hipEventCreate(&c->ev0));
+  HIPCHECK(c, hipEventCreate(&c->ev1));
+  return PDX_OK;
+}
+
+int ensureReady(pdx_ctx* c) {
+  if (c->inst.empty()) return fail(c, PDX_ENOTREADY, "no instance");
+  bool changed = false;
+  if (c->uploaded && !c->dirty) {
+    // cheap stamp re-check (kappa / config changes mark the context dirty)
+    return PDX_OK;
+  }
+  int rc = ensureDevice(c);
+  if (rc) return rc;
+  rc = pullState(c);
+  if (rc) return rc;
+  rc = refreshFactors(c, changed);
+  if (rc) return rc;
+  // carry statistics across a re-upload
+  std::vector<std::vector<int>> ist;
+  std::vector<std::vector<long long>> lst;
+  if (c->uploaded) {
+    for (auto& I : c->inst) {
+      std::vector<int> a(4);
+      std::vector<long long> b(2);
+      HIPCHECK(c, hipMemcpy(a.data(), I.distats, 16, hipMemcpyDeviceToHost));
+      HIPCHECK(c, hipMemcpy(b.data(), I.dlstats, 16, hipMemcpyDeviceToHost));
+      ist.push_back(a);
+      lst.push_back(b);
+    }
+  }
+  rc = upload(c);
+  if (rc) return rc;
+  for (size_t i = 0; i < ist.size(); ++i) {
+    HIPCHECK(c, hipMemcpy(c->inst[i].distats, ist[i].data(), 16, hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(c->inst[i].dlstats, lst[i].data(), 16, hipMemcpyHostToDevice));
+  }
+  return PDX_OK;
+}
+
+int launch(pdx_ctx* c, int nsub) {
+  HIPCHECK(c, hipSetDevice(c->device));
+  HIPCHECK(c, hipEventRecord(c->ev0, c->stream));
+  HIPCHECK(c, launchSubstep(c->dInst, c->dOrder, (int)c->inst.size(), nsub, c->block, c->lds,
+                            c->prec == PDX_PREC_MIXED32, c->stream));
+  HIPCHECK(c, hipEventRecord(c->ev1, c->stream));
+  c->lastLaunches = 1;
+  return PDX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pdx_create(pdx_ctx** out, int device, int precision, int solve_mode) {
+  if (!out) return PDX_EINVAL;
+  *out = nullptr;
+  if (precision != PDX_PREC_FP64 && precision != PDX_PREC_MIXED32) return PDX_EINVAL;
+  if (solve_mode != PDX_SOLVE_EXACT && solve_mode != PDX_SOLVE_FAST) return PDX_EINVAL;
+  pdx_ctx* c = new pdx_ctx();
+  c->device = device;
+  c->prec = precision;
+  c->mode = solve_mode;
+  *out = c;
+  return PDX_OK;
+}
+
+int pdx_destroy(pdx_ctx* c) {
+  if (!c) return PDX_EINVAL;
+  if (c->stream) {
+    hipSetDevice(c->device);
+    hipStreamSynchronize(c->stream);
+  }
+  if (c->dbuf) hipFree(c->dbuf);
+  if (c->dScratchB) hipFree(c->dScratchB);
+  if (c->dScratchX) hipFree(c->dScratchX);
+  if (c->ev0) hipEventDestroy(c->ev0);
+  if (c->ev1) hipEventDestroy(c->ev1);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return PDX_OK;
+}
+
+const char* pdx_last_error(const pdx_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int pdx_add_instance(pdx_ctx* c, const pdx_scene* s, const pdx_config* cfg, int* instance) {
+  if (!c || !cfg) return PDX_EINVAL;
+  if (validateScene(s) != PDX_OK) return fail(c, PDX_EINVAL, "invalid scene");
+  if (cfg->dt <= 0.0 || cfg->max_iterations < 0) return fail(c, PDX_EINVAL, "invalid config");
+  if (c->uploaded) {
+    const int rc = pullState(c);
+    if (rc) return rc;
+  }
+  Instance I;
+  const int n = s->n_vertices;
+  Scene& S = I.scene;
+  S.n = n;
+  S.E = s->n_edges;
+  S.B = s->n_bends;
+  S.x.assign(s->positions, s->positions + 3 * (size_t)n);
+  S.v = s->velocities ? std::vector<double>(s->velocities, s->velocities + 3 * (size_t)n)
+                      : std::vector<double>(3 * (size_t)n, 0.0);
+  S.mass.assign(s->masses, s->masses + n);
+  S.pinned.assign(s->pinned, s->pinned + n);
+  for (auto& p : S.pinned) p = p ? 1 : 0;
+  S.pinpos = s->pin_positions ? std::vector<double>(s->pin_positions, s->pin_positions + 3 * (size_t)n) : S.x;
+  S.ea.assign(s->edge_a, s->edge_a + S.E);
+  S.eb.assign(s->edge_b, s->edge_b + S.E);
+  S.rest.assign(s->edge_rest, s->edge_rest + S.E);
+  S.kappa.assign(s->edge_kappa, s->edge_kappa + S.E);
+  if (S.B > 0) {
+    S.sa.assign(s->bend_a, s->bend_a + S.B);
+    S.sb.assign(s->bend_b, s->bend_b + S.B);
+    S.sc.assign(s->bend_c, s->bend_c + S.B);
+    S.bk.assign(s->bend_k, s->bend_k + S.B);
+  }
+  fillConfig(I.cfg, cfg);
+  c->inst.push_back(std::move(I));
+  c->dirty = true;
+  if (instance) *instance = (int)c->inst.size() - 1;
+  return PDX_OK;
+}
+
+int pdx_instance_count(const pdx_ctx* c) { return c ? (int)c->inst.size() : 0; }
+
+int pdx_factor(pdx_ctx* c) {
+  if (!c) return PDX_EINVAL;
+  return ensureReady(c);
+}
+
+int pdx_step_async(pdx_ctx* c, int nsub) {
+  if (!c || nsub < 0) return PDX_EINVAL;
+  int rc = ensureReady(c);
+  if (rc) return rc;
+  if (nsub == 0) return PDX_OK;
+  return launch(c, nsub);
+}
+
+int pdx_sync(pdx_ctx* c) {
+  if (!c) return PDX_EINVAL;
+  if (!c->stream) return PDX_OK;
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  float ms = 0.f;
+  if (c->lastLaunches > 0 && hipEventElapsedTime(&ms, c->ev0, c->ev1) == hipSuccess)
+    c->lastKernelMs = ms / c->lastLaunches;
+  return PDX_OK;
+}
+
+int pdx_step(pdx_ctx* c, int nsub) {
+  int rc = pdx_step_async(c, nsub);
+  if (rc) return rc;
+  return pdx_sync(c);
+}
+
+int pdx_step_graph(pdx_ctx* c, int nsub, int replays) {
+  if (!c || nsub <= 0 || replays < 0) return PDX_EINVAL;
+  int rc = ensureReady(c);
+  if (rc) return rc;
+  hipGraph_t g = nullptr;
+  hipGraphExec_t ge = nullptr;
+  HIPCHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  const hipError_t le = launchSubstep(c->dInst, c->dOrder, (int)c->inst.size(), nsub, c->block, c->lds,
+                                      c->prec == PDX_PREC_MIXED32, c->stream);
+  HIPCHECK(c, hipStreamEndCapture(c->stream, &g));
+  if (le != hipSuccess) {
+    hipGraphDestroy(g);
+    return fail(c, PDX_EHIP, std::string("capture: ") + hipGetErrorString(le));
+  }
+  HIPCHECK(c, hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+  HIPCHECK(c, hipEventRecord(c->ev0, c->stream));
+  for (int r = 0; r < replays; ++r) HIPCHECK(c, hipGraphLaunch(ge, c->stream));
+  HIPCHECK(c, hipEventRecord(c->ev1, c->stream));
+  c->lastLaunches = std::max(1, replays);
+  rc = pdx_sync(c);
+  hipGraphExecDestroy(ge);
+  hipGraphDestroy(g);
+  return rc;
+}
+
+int pdx_get_state(pdx_ctx* c, int i, double* x, double* v) {
+  if (!c || i < 0 || i >= (int)c->inst.size()) return PDX_EINVAL;
+  Instance& I = c->inst[(size_t)i];
+  if (!c->uploaded) {
+    if (x) std::memcpy(x, I.scene.x.data(), I.scene.x.size() * 8);
+    if (v) std::memcpy(v, I.scene.v.data(), I.scene.v.size() * 8);
+    return PDX_OK;
+  }
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  if (x) HIPCHECK(c, hipMemcpy(x, I.dx, I.scene.x.size() * 8, hipMemcpyDeviceToHost));
+  if (v) HIPCHECK(c, hipMemcpy(v, I.dv, I.scene.v.size() * 8, hipMemcpyDeviceToHost));
+  return PDX_OK;
+}
+
+int pdx_set_state(pdx_ctx* c, int i, const double* x, const double* v) {
+  if (!c || i < 0 || i >= (int)c->inst.size()) return PDX_EINVAL;
+  Instance& I = c->inst[(size_t)i];
+  if (x) std::memcpy(I.scene.x.data(), x, I.scene.x.size() * 8);
+  if (v) std::memcpy(I.scene.v.data(), v, I.scene.v.size() * 8);
+  if (c->uploaded) {
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    if (x) HIPCHECK(c, hipMemcpy(I.dx, x, I.scene.x.size() * 8, hipMemcpyHostToDevice));
+    if (v) HIPCHECK(c, hipMemcpy(I.dv, v, I.scene.v.size() * 8, hipMemcpyHostToDevice));
+  }
+  return PDX_OK;
+}
+
+int pdx_set_pin_positions(pdx_ctx* c, int i, const double* q) {
+  if (!c || !q || i < 0 || i >= (int)c->inst.size()) return PDX_EINVAL;
+  Instance& I = c->inst[(size_t)i];
+  std::memcpy(I.scene.pinpos.data(), q, I.scene.pinpos.size() * 8);
+  if (I.scene.B > 0) bendingRhs(I.scene, I.brhs);
+  if (c->uploaded) {
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    HIPCHECK(c, hipMemcpy(I.dpin, q, I.scene.pinpos.size() * 8, hipMemcpyHostToDevice));
+    if (I.scene.B > 0)
+      HIPCHECK(c, hipMemcpy(I.dbrhs, I.brhs.data(), I.brhs.size() * 8, hipMemcpyHostToDevice));
+  }
+  return PDX_OK;
+}
+
+int pdx_set_edge_kappa(pdx_ctx* c, int i, const double* kappa) {
+  if (!c || !kappa || i < 0 || i >= (int)c->inst.size()) return PDX_EINVAL;
+  Instance& I = c->inst[(size_t)i];
+  const int rc = pullState(c);
+  if (rc) return rc;
+  std::memcpy(I.scene.kappa.data(), kappa, I.scene.kappa.size() * 8);
+  if (numericStamp(I.scene, I.cfg.dt) != I.numStamp) c->dirty = true;
+  return PDX_OK;
+}
+
+int pdx_set_config(pdx_ctx* c, int i, const pdx_config* cfg) {
+  if (!c || !cfg || i < 0 || i >= (int)c->inst.size()) return PDX_EINVAL;
+  if (cfg->dt <= 0.0 || cfg->max_iterations < 0) return fail(c, PDX_EINVAL, "invalid config");
+  const int rc = pullState(c);
+  if (rc) return rc;
+  fillConfig(c->inst[(size_t)i].cfg, cfg);
+  c->dirty = true;  // damping/tolerances re-upload; only a dt change refactors (stamp)
+  return PDX_OK;
+}
+
+int pdx_get_stats(pdx_ctx* c, int i, pdx_stats* s) {
+  if (!c || !s || i < 0 || i >= (int)c->inst.size()) return PDX_EINVAL;
+  Instance& I = c->inst[(size_t)i];
+  std::memset(s, 0, sizeof(*s));
+  s->factorize_count = I.factorizeCount;
+  s->analyze_count = I.analyzeCount;
+  s->color_count = I.topo.colorCount;
+  s->factor_nnz = I.fac.nnz;
+  s->fwd_levels = I.fac.fwd.nlev;
+  s->bwd_levels = I.fac.bwd.nlev;
+  s->fwd_steps = I.hf.nlev;
+  s->bwd_steps = I.hb.nlev;
+  s->analyze_seconds = I.analyzeSec;
+  s->factorize_seconds = I.factorSec;
+  s->last_residual = -1.0;
+  if (c->uploaded) {
+    int ist[4];
+    long long lst[2];
+    double dst[2];
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    HIPCHECK(c, hipMemcpy(ist, I.distats, 16, hipMemcpyDeviceToHost));
+    HIPCHECK(c, hipMemcpy(lst, I.dlstats, 16, hipMemcpyDeviceToHost));
+    HIPCHECK(c, hipMemcpy(dst, I.ddstats, 16, hipMemcpyDeviceToHost));
+    s->iterations_used = ist[0];
+    s->converged = ist[1];
+    s->early_exit_count = ist[2];
+    s->total_iterations = lst[0];
+    s->substeps = lst[1];
+    s->last_residual = dst[0];
+  }
+  return PDX_OK;
+}
+
+int pdx_solve3(pdx_ctx* c, int i, const double* b, double* x) {
+  if (!c || !b || !x || i < 0 || i >= (int)c->inst.size()) return PDX_EINVAL;
+  int rc = ensureReady(c);
+  if (rc) return rc;
+  Instance& I = c->inst[(size_t)i];
+  const int n3 = 3 * I.scene.n;
+  if (c->scratchN < n3) {
+    if (c->dScratchB) hipFree(c->dScratchB);
+    if (c->dScratchX) hipFree(c->dScratchX);
+    HIPCHECK(c, hipMalloc(&c->dScratchB, (size_t)n3 * 8));
+    HIPCHECK(c, hipMalloc(&c->dScratchX, (size_t)n3 * 8));
+    c->scratchN = n3;
+  }
+  HIPCHECK(c, hipMemcpyAsync(c->dScratchB, b, (size_t)n3 * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(c, launchSolve(c->dInst, i, c->dScratchB, c->dScratchX, c->block, c->lds,
+                          c->prec == PDX_PREC_MIXED32, c->stream));
+  HIPCHECK(c, hipMemcpyAsync(x, c->dScratchX, (size_t)n3 * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  return PDX_OK;
+}
+
+int pdx_get_factor(pdx_ctx* c, int i, int* m, long long* nnz, int32_t* ptr, int32_t* rows, double* vals,
+                   double* dinv, int32_t* rank_vertex, int32_t* perm3) {
+  if (!c || i < 0 || i >= (int)c->inst.size()) return PDX_EINVAL;
+  Instance& I = c->inst[(size_t)i];
+  if (!I.factored) {
+    bool changed = false;
+    const int rc = refreshFactors(c, changed);
+    if (rc) return rc;
+  }
+  const Factor& f = I.fac;
+  if (m) *m = f.m;
+  if (nnz) *nnz = f.nnz;
+  if (ptr) std::copy(f.cp.begin(), f.cp.end(), ptr);
+  if (rows) std::copy(f.ci.begin(), f.ci.end(), rows);
+  if (vals) std::copy(f.cx.begin(), f.cx.end(), vals);
+  if (dinv) std::copy(f.dinv.begin(), f.dinv.end(), dinv);
+  if (rank_vertex) std::copy(f.rankVertex[0].begin(), f.rankVertex[0].end(), rank_vertex);
+  if (perm3) std::copy(f.perm3.begin(), f.perm3.end(), perm3);
+  return PDX_OK;
+}
+
+double pdx_last_kernel_ms(const pdx_ctx* c) { return c ? c->lastKernelMs : 0.0; }
+double pdx_bytes_per_iteration(const pdx_ctx* c) { return c ? c->bytesIter : 0.0; }
+
+}  // extern "C"

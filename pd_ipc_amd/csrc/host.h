@@ -1,0 +1,85 @@
+// Host-side data structures of the PD substep engine: scene topology, the
+// reference's edge colouring and incidence lists, the assembled global matrix,
+// its LDL^T factor, and the level schedules the GPU solve kernels walk.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace pdx {
+
+// One cloth instance as handed over the C ABI (all arrays copied).
+struct Scene {
+  int n = 0, E = 0, B = 0;
+  std::vector<double> x, v, mass, pinpos;  // 3n, 3n, n, 3n
+  std::vector<uint8_t> pinned;             // n
+  std::vector<int> ea, eb, sa, sb, sc;     // E, E, B, B, B
+  std::vector<double> rest, kappa, bk;     // E, E, B
+};
+
+struct Config {
+  double dt = 1.0 / 120.0;
+  double g[3] = {0.0, -9.81, 0.0};
+  double damping = 0.02;
+  int maxIter = 10;
+  double relTol = 1e-3, absTol = 1e-12, resTol = 1e-3;
+};
+
+// Topology-only data (core/mesh/ConstraintColoring.h:34-81).
+struct Topology {
+  int colorCount = 0;
+  std::vector<uint32_t> colorOf, cstart, corder;
+  std::vector<uint32_t> vstart, vedges;        // incidence, edge-index order
+  std::vector<uint32_t> vedgesColor;           // incidence, colour order (one edge per colour)
+  std::vector<uint32_t> bstart, bstencil;      // vertex -> bend stencils (stencil order)
+  std::vector<int8_t> bweight;                 // 1, -2, 1
+};
+
+void buildTopology(const Scene& s, Topology& t);
+
+// Scalar component matrix A (L = A (x) I3), CSC over vertices with rows ascending.
+struct Matrix {
+  int n = 0;
+  std::vector<int> colptr, rows;
+  std::vector<double> vals;
+};
+void assembleMatrix(const Scene& s, double dt, Matrix& A);
+void bendingRhs(const Scene& s, std::vector<double>& out);
+
+void amdOrder(int n, const std::vector<int>& colptr, const std::vector<int>& rowidx,
+              std::vector<int>& perm);
+
+// Level-scheduled sweep of a unit triangular factor (rank space).
+struct Sweep {
+  int nlev = 0;
+  std::vector<int> levelOf;                    // per rank
+  std::vector<int> levPtr;                     // nlev+1 ranges into `order`
+  std::vector<int> order;                      // ranks in level order
+};
+
+// Factor of the scalar component in the AMD order of the 3n pattern
+// (core/solver/EigenDirectSolver.cpp:230-297 + compact factor :157-225).
+struct Factor {
+  int m = 0;
+  std::vector<int> perm3;                      // AMD pivot order of the 3n pattern
+  std::vector<int> rankVertex[3];              // rank -> vertex, per component
+  std::vector<int> vertexRank[3];              // vertex -> rank, per component
+  bool sharedOrder = true;                     // the three component orders coincide
+  // strictly lower factor, CSC by rank (rows ascending) and CSR by rank (cols ascending)
+  std::vector<int> cp, ci;
+  std::vector<double> cx;
+  std::vector<int> rp, rc;
+  std::vector<double> rx;
+  std::vector<double> dinv;
+  int64_t nnz = 0;
+  Sweep fwd, bwd;
+};
+
+// Returns 0 or a negative status (-3 = not positive definite).
+int analyzeAndFactor(const Scene& s, const Matrix& A, Factor& f);
+int refactorNumeric(const Matrix& A, Factor& f);
+
+uint64_t numericStamp(const Scene& s, double dt);
+uint64_t structureStamp(const Scene& s);
+
+}  // namespace pdx

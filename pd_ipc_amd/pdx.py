@@ -1,0 +1,308 @@
+"""ctypes host mirror of the C ABI in include/pdx.h.
+
+The names follow the reference's interface: ``Engine.step_once`` is
+``pd::stepOnce`` (core/sim/Integrator.h:14), ``Engine.solve`` is
+``EigenDirectSolver::solve`` (core/solver/EigenDirectSolver.cpp:299-357), and
+``Engine.stats`` carries the reference's counters (core/sim/Scene.h:161-169,
+core/solver/IGlobalSolver.h:114-125).  Errors raise ``PdxError`` where the
+reference aborts.  The library must be built (``python -m pd_ipc_amd.build``);
+there is no fallback.
+"""
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpdx.so")
+
+PDX_PREC_FP64 = 0
+PDX_PREC_MIXED32 = 1
+PDX_SOLVE_EXACT = 0
+PDX_SOLVE_FAST = 1
+
+_STATUS = {0: "ok", -1: "invalid argument", -2: "not ready", -3: "not positive definite",
+           -4: "HIP runtime error", -5: "unsupported", -6: "out of memory"}
+
+
+class PdxError(RuntimeError):
+    def __init__(self, code, msg=""):
+        super().__init__(f"pdx error {code} ({_STATUS.get(code, '?')}): {msg}")
+        self.code = code
+
+
+class _Config(C.Structure):
+    _fields_ = [("dt", C.c_double), ("gravity", C.c_double * 3), ("damping", C.c_double),
+                ("max_iterations", C.c_int), ("rel_tol", C.c_double), ("abs_tol", C.c_double),
+                ("residual_tol", C.c_double)]
+
+
+class _Scene(C.Structure):
+    _fields_ = [("n_vertices", C.c_int), ("positions", C.c_void_p), ("velocities", C.c_void_p),
+                ("masses", C.c_void_p), ("pinned", C.c_void_p), ("pin_positions", C.c_void_p),
+                ("n_edges", C.c_int), ("edge_a", C.c_void_p), ("edge_b", C.c_void_p),
+                ("edge_rest", C.c_void_p), ("edge_kappa", C.c_void_p), ("n_bends", C.c_int),
+                ("bend_a", C.c_void_p), ("bend_b", C.c_void_p), ("bend_c", C.c_void_p),
+                ("bend_k", C.c_void_p)]
+
+
+class _Stats(C.Structure):
+    _fields_ = [("iterations_used", C.c_int), ("converged", C.c_int), ("early_exit_count", C.c_int),
+                ("factorize_count", C.c_int), ("analyze_count", C.c_int), ("color_count", C.c_int),
+                ("total_iterations", C.c_longlong), ("substeps", C.c_longlong),
+                ("last_residual", C.c_double), ("factor_nnz", C.c_longlong),
+                ("fwd_levels", C.c_int), ("bwd_levels", C.c_int), ("fwd_steps", C.c_int),
+                ("bwd_steps", C.c_int), ("analyze_seconds", C.c_double),
+                ("factorize_seconds", C.c_double)]
+
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Loads libpdx.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise PdxError(-2, f"{path} is missing: run `python -m pd_ipc_amd.build`")
+    lib = C.CDLL(path)
+    P, I, D = C.c_void_p, C.c_int, C.c_double
+    sig = {
+        "pdx_create": (I, [C.POINTER(P), I, I, I]),
+        "pdx_destroy": (I, [P]),
+        "pdx_last_error": (C.c_char_p, [P]),
+        "pdx_add_instance": (I, [P, C.POINTER(_Scene), C.POINTER(_Config), C.POINTER(I)]),
+        "pdx_instance_count": (I, [P]),
+        "pdx_factor": (I, [P]),
+        "pdx_step": (I, [P, I]),
+        "pdx_step_async": (I, [P, I]),
+        "pdx_sync": (I, [P]),
+        "pdx_step_graph": (I, [P, I, I]),
+        "pdx_get_state": (I, [P, I, P, P]),
+        "pdx_set_state": (I, [P, I, P, P]),
+        "pdx_set_pin_positions": (I, [P, I, P]),
+        "pdx_set_edge_kappa": (I, [P, I, P]),
+        "pdx_set_config": (I, [P, I, C.POINTER(_Config)]),
+        "pdx_get_stats": (I, [P, I, C.POINTER(_Stats)]),
+        "pdx_solve3": (I, [P, I, P, P]),
+        "pdx_get_factor": (I, [P, I, C.POINTER(I), C.POINTER(C.c_longlong), P, P, P, P, P, P]),
+        "pdx_last_kernel_ms": (D, [P]),
+        "pdx_bytes_per_iteration": (D, [P]),
+        "pdx_grid_counts": (I, [I, I, I, I, C.POINTER(I), C.POINTER(I), C.POINTER(I)]),
+        "pdx_grid_fill": (I, [I, I, D, D, D, D, D, P, P, P, P, P, P, P, P, P]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+@dataclass
+class SimConfig:
+    """The SceneConfig fields the substep reads (core/sim/Scene.h:14-137)."""
+    dt: float = 1.0 / 120.0
+    gravity: tuple = (0.0, -9.81, 0.0)
+    damping: float = 0.02
+    max_iterations: int = 10
+    rel_tol: float = 1e-3
+    abs_tol: float = 1e-12
+    residual_tol: float = 1e-3
+
+    @staticmethod
+    def fixed_iterations(iters, dt, damping=0.02, gravity=(0.0, -9.81, 0.0)):
+        """Fixed PD iteration count with the residual still evaluated every
+        iteration (the reference's own setting, _verify/iteration_spectrum.cpp:106-108)."""
+        return SimConfig(dt=dt, gravity=gravity, damping=damping, max_iterations=iters,
+                         rel_tol=0.0, abs_tol=0.0, residual_tol=1e-30)
+
+    def _c(self):
+        c = _Config()
+        c.dt = self.dt
+        for i in range(3):
+            c.gravity[i] = self.gravity[i]
+        c.damping = self.damping
+        c.max_iterations = self.max_iterations
+        c.rel_tol = self.rel_tol
+        c.abs_tol = self.abs_tol
+        c.residual_tol = self.residual_tol
+        return c
+
+
+@dataclass
+class Cloth:
+    """One cloth instance as flat numpy arrays (the Mesh fields of core/mesh/Mesh.h)."""
+    positions: np.ndarray
+    masses: np.ndarray
+    pinned: np.ndarray
+    edge_a: np.ndarray
+    edge_b: np.ndarray
+    edge_rest: np.ndarray
+    edge_kappa: np.ndarray
+    velocities: np.ndarray = None
+    pin_positions: np.ndarray = None
+    bend_a: np.ndarray = None
+    bend_b: np.ndarray = None
+    bend_c: np.ndarray = None
+    bend_k: np.ndarray = None
+    meta: dict = field(default_factory=dict)
+
+    def __post_init__(self):
+        f8 = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+        i4 = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.int32).reshape(-1)
+        self.positions = f8(self.positions)
+        self.masses = f8(self.masses)
+        self.pinned = np.ascontiguousarray(self.pinned, dtype=np.uint8).reshape(-1)
+        self.edge_a, self.edge_b = i4(self.edge_a), i4(self.edge_b)
+        self.edge_rest, self.edge_kappa = f8(self.edge_rest), f8(self.edge_kappa)
+        self.velocities = f8(self.velocities) if self.velocities is not None else np.zeros_like(self.positions)
+        self.pin_positions = f8(self.pin_positions) if self.pin_positions is not None else self.positions.copy()
+        if self.bend_a is None or len(self.bend_a) == 0:
+            self.bend_a = self.bend_b = self.bend_c = np.zeros(0, np.int32)
+            self.bend_k = np.zeros(0, np.float64)
+        else:
+            self.bend_a, self.bend_b, self.bend_c = i4(self.bend_a), i4(self.bend_b), i4(self.bend_c)
+            self.bend_k = f8(self.bend_k)
+
+    @property
+    def n(self):
+        return len(self.masses)
+
+    @property
+    def n_edges(self):
+        return len(self.edge_a)
+
+    @property
+    def n_bends(self):
+        return len(self.bend_a)
+
+    def _c(self):
+        s = _Scene()
+        s.n_vertices = self.n
+        s.positions, s.velocities = _ptr(self.positions), _ptr(self.velocities)
+        s.masses, s.pinned, s.pin_positions = _ptr(self.masses), _ptr(self.pinned), _ptr(self.pin_positions)
+        s.n_edges = self.n_edges
+        s.edge_a, s.edge_b = _ptr(self.edge_a), _ptr(self.edge_b)
+        s.edge_rest, s.edge_kappa = _ptr(self.edge_rest), _ptr(self.edge_kappa)
+        s.n_bends = self.n_bends
+        s.bend_a, s.bend_b, s.bend_c, s.bend_k = (_ptr(self.bend_a), _ptr(self.bend_b),
+                                                  _ptr(self.bend_c), _ptr(self.bend_k))
+        return s
+
+
+class Engine:
+    """A context of one or more cloth instances stepped together on one GPU."""
+
+    def __init__(self, precision="fp64", solve="exact", device=0):
+        self.lib = load_library()
+        prec = {"fp64": PDX_PREC_FP64, "mixed32": PDX_PREC_MIXED32}[precision]
+        mode = {"exact": PDX_SOLVE_EXACT, "fast": PDX_SOLVE_FAST}[solve]
+        h = C.c_void_p()
+        rc = self.lib.pdx_create(C.byref(h), device, prec, mode)
+        if rc != 0:
+            raise PdxError(rc, "pdx_create")
+        self.h = h
+        self.precision = precision
+        self.cloths = []
+
+    def close(self):
+        if self.h:
+            self.lib.pdx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise PdxError(rc, f"{what}: {self.lib.pdx_last_error(self.h).decode(errors='replace')}")
+
+    def add(self, cloth, cfg):
+        idx = C.c_int()
+        s, c = cloth._c(), cfg._c()
+        self._check(self.lib.pdx_add_instance(self.h, C.byref(s), C.byref(c), C.byref(idx)), "add_instance")
+        self.cloths.append(cloth)
+        return idx.value
+
+    def factor(self):
+        self._check(self.lib.pdx_factor(self.h), "factor")
+
+    def step(self, nsub=1):
+        self._check(self.lib.pdx_step(self.h, nsub), "step")
+
+    def step_once(self):
+        """pd::stepOnce on every instance of the context."""
+        self.step(1)
+
+    def step_async(self, nsub):
+        self._check(self.lib.pdx_step_async(self.h, nsub), "step_async")
+
+    def sync(self):
+        self._check(self.lib.pdx_sync(self.h), "sync")
+
+    def step_graph(self, nsub, replays):
+        self._check(self.lib.pdx_step_graph(self.h, nsub, replays), "step_graph")
+
+    def state(self, i=0):
+        n = self.cloths[i].n
+        x = np.empty(3 * n)
+        v = np.empty(3 * n)
+        self._check(self.lib.pdx_get_state(self.h, i, _ptr(x), _ptr(v)), "get_state")
+        return x, v
+
+    def set_state(self, i, x=None, v=None):
+        x = None if x is None else np.ascontiguousarray(x, np.float64)
+        v = None if v is None else np.ascontiguousarray(v, np.float64)
+        self._check(self.lib.pdx_set_state(self.h, i, _ptr(x), _ptr(v)), "set_state")
+
+    def set_pin_positions(self, i, q):
+        q = np.ascontiguousarray(q, np.float64)
+        self._check(self.lib.pdx_set_pin_positions(self.h, i, _ptr(q)), "set_pin_positions")
+
+    def set_edge_kappa(self, i, kappa):
+        k = np.ascontiguousarray(kappa, np.float64)
+        self._check(self.lib.pdx_set_edge_kappa(self.h, i, _ptr(k)), "set_edge_kappa")
+
+    def set_config(self, i, cfg):
+        c = cfg._c()
+        self._check(self.lib.pdx_set_config(self.h, i, C.byref(c)), "set_config")
+
+    def stats(self, i=0):
+        s = _Stats()
+        self._check(self.lib.pdx_get_stats(self.h, i, C.byref(s)), "get_stats")
+        return {k: getattr(s, k) for k, _ in _Stats._fields_}
+
+    def solve(self, i, b):
+        b = np.ascontiguousarray(b, np.float64)
+        x = np.empty_like(b)
+        self._check(self.lib.pdx_solve3(self.h, i, _ptr(b), _ptr(x)), "solve3")
+        return x
+
+    def factor_arrays(self, i=0):
+        m, nnz = C.c_int(), C.c_longlong()
+        self._check(self.lib.pdx_get_factor(self.h, i, C.byref(m), C.byref(nnz), None, None, None,
+                                            None, None, None), "get_factor")
+        m, nnz = m.value, nnz.value
+        out = dict(ptr=np.empty(m + 1, np.int32), rows=np.empty(nnz, np.int32), vals=np.empty(nnz),
+                   dinv=np.empty(m), rank_vertex=np.empty(m, np.int32), perm3=np.empty(3 * m, np.int32))
+        self._check(self.lib.pdx_get_factor(self.h, i, C.byref(C.c_int()), C.byref(C.c_longlong()),
+                                            _ptr(out["ptr"]), _ptr(out["rows"]), _ptr(out["vals"]),
+                                            _ptr(out["dinv"]), _ptr(out["rank_vertex"]),
+                                            _ptr(out["perm3"])), "get_factor")
+        return out
+
+    def last_kernel_ms(self):
+        return self.lib.pdx_last_kernel_ms(self.h)
+
+    def bytes_per_iteration(self):
+        return self.lib.pdx_bytes_per_iteration(self.h)
