@@ -40,16 +40,28 @@ class SplitMix64:
         """Uniform integer in [lo, hi]."""
         return lo + int(self.uniform() * (hi - lo + 1))
 
+    def uniforms(self, count):
+        """`count` consecutive uniforms of the stream (vectorised splitmix64)."""
+        k = np.arange(1, count + 1, dtype=np.uint64)
+        with np.errstate(over="ignore"):
+            z = np.uint64(self.s) + k * np.uint64(0x9E3779B97F4A7C15)
+            z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+            z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+            z = z ^ (z >> np.uint64(31))
+        self.s = (self.s + count * 0x9E3779B97F4A7C15) & _MASK
+        return (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
     def normals(self, count):
-        out = np.empty(count)
-        for i in range(0, count, 2):
-            u1 = max(self.uniform(), 1e-300)
-            u2 = self.uniform()
-            r = math.sqrt(-2.0 * math.log(u1))
-            out[i] = r * math.cos(2.0 * math.pi * u2)
-            if i + 1 < count:
-                out[i + 1] = r * math.sin(2.0 * math.pi * u2)
-        return out
+        """Box-Muller over pairs of consecutive uniforms."""
+        m = (count + 1) // 2
+        u = self.uniforms(2 * m)
+        u1 = np.maximum(u[0::2], 1e-300)
+        r = np.sqrt(-2.0 * np.log(u1))
+        t = 2.0 * np.pi * u[1::2]
+        out = np.empty(2 * m)
+        out[0::2] = r * np.cos(t)
+        out[1::2] = r * np.sin(t)
+        return out[:count]
 
 
 def grid_cloth(nx, ny, kappa, shear=0.0, bend=0.0, sx=None, sz=None, pins="top", mass="uniform"):

@@ -1,0 +1,88 @@
+"""GPU parity of the HIP substep engine against the FP64 CPU oracle.
+
+Exact mode must reproduce the oracle bit for bit (same factor, same sum orders);
+the mixed FP32 form must stay within 1e-5 m of it.  Every GPU step prints the
+case it runs before it runs.
+"""
+import numpy as np
+import pytest
+
+from pd_ipc_amd import Engine, SimConfig, scenes
+from oracle_ffi import Oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print("[case]", *a, flush=True)
+
+
+def _pair(cloth, cfg, precision="fp64", solve="exact"):
+    e = Engine(precision, solve)
+    e.add(cloth, cfg)
+    e.factor()
+    o = Oracle(cloth, cfg, e.factor_arrays(0)["perm3"])
+    return e, o
+
+
+@pytest.mark.parametrize("nx,ny,shear,bend", [(16, 16, 0.0, 0.0), (20, 13, 500.0, 50.0), (40, 40, 0.0, 0.0)])
+def test_solve3_bitwise(nx, ny, shear, bend):
+    _say("solve3", nx, ny, shear, bend)
+    c = scenes.grid_cloth(nx, ny, 2300.0, shear=shear, bend=bend)
+    cfg = SimConfig.fixed_iterations(10, scenes.H600)
+    e, o = _pair(c, cfg)
+    rng = np.random.default_rng(7)
+    b = rng.standard_normal(3 * c.n)
+    xg = e.solve(0, b)
+    xo = o.solve(b)
+    assert np.array_equal(xg.view(np.uint64), xo.view(np.uint64))
+
+
+@pytest.mark.parametrize("nx,ny,shear,bend,nsub", [(40, 40, 0.0, 0.0, 30), (24, 17, 500.0, 50.0, 20)])
+def test_step_exact_bitwise(nx, ny, shear, bend, nsub):
+    _say("step exact", nx, ny, shear, bend, nsub)
+    c = scenes.grid_cloth(nx, ny, 2300.0, shear=shear, bend=bend)
+    cfg = SimConfig.fixed_iterations(10, scenes.H600)
+    e, o = _pair(c, cfg)
+    e.step(nsub)
+    o.step(nsub)
+    xg, vg = e.state(0)
+    xo, vo = o.positions(), o.velocities()
+    assert np.array_equal(xg.view(np.uint64), xo.view(np.uint64)), np.abs(xg - xo).max()
+    assert np.array_equal(vg.view(np.uint64), vo.view(np.uint64))
+    s = e.stats(0)
+    assert s["factorize_count"] == 1
+    assert s["substeps"] == nsub and s["iterations_used"] == 10
+    assert s["last_residual"] == o.last_residual()
+
+
+def test_bench_scene_published_outputs():
+    """The reference's pd_bench realtime run (docs/perf.md:1423): 40x40, kappa 1e4,
+    40 iterations, residual tol 0.3, 300 substeps."""
+    _say("bench scene 300 substeps")
+    c, cfg = scenes.bench_scene_40()
+    e = Engine("fp64")
+    e.add(c, cfg)
+    e.step(300)
+    x, _ = e.state(0)
+    s = e.stats(0)
+    assert s["converged"] == 0 and s["early_exit_count"] == 0
+    assert f"{s['last_residual']:.6g}" == "8.63991"
+    a, b = c.edge_a, c.edge_b
+    P = x.reshape(-1, 3)
+    ln = np.sqrt(((P[a] - P[b]) ** 2).sum(1))
+    strain = np.abs(ln - c.edge_rest) / c.edge_rest
+    assert f"{strain.max():.6g}" == "0.0585735"
+
+
+def test_mixed32_error_bound():
+    _say("mixed32 vs oracle 40x40 shear+bend")
+    c = scenes.grid_cloth(40, 40, 2300.0, shear=500.0, bend=50.0)
+    cfg = SimConfig.fixed_iterations(20, scenes.H600)
+    e, o = _pair(c, cfg, precision="mixed32")
+    e.step(50)
+    o.step(50)
+    xg, _ = e.state(0)
+    err = np.abs(xg - o.positions()).max()
+    print("max position error", err)
+    assert err <= 1e-5
