@@ -19,9 +19,9 @@
 
 namespace pdx {
 hipError_t launchSubstep(const InstDev* d, const int* order, int ninst, int nsub, int block,
-                         size_t lds, bool fp32, hipStream_t s);
+                         size_t lds, bool fp32, bool blocked, hipStream_t s);
 hipError_t launchSolve(const InstDev* d, int inst, const double* b, double* x, int block,
-                       size_t lds, bool fp32, hipStream_t s);
+                       size_t lds, bool fp32, bool blocked, hipStream_t s);
 hipError_t setLdsLimit(size_t bytes);
 }  // namespace pdx
 
@@ -79,6 +79,168 @@ void packSweep(const Sweep& sw, bool fwd, const Factor& f, HostSweep& out) {
   }
 }
 
+struct HostBlockSweep {
+  int nstep = 0;
+  std::vector<int> stepTask, taskInfo, taskMaxLen, taskRowBase, taskEntBase, taskDenseBase, slotRank, slotLen,
+      col;
+  std::vector<double> val, dense;
+};
+
+// Block schedule of one sweep.  Ranks are cut into contiguous blocks: a block
+// continues while rank k-1 has parent k in the elimination tree and k has no
+// other child (a chain), up to `cap` rows.  Block levels follow the dependencies
+// between blocks; every level becomes one barrier-separated step whose tasks are
+// chain blocks (one wavefront each) and batches of single-row blocks.
+void packBlocked(const Factor& f, bool fwd, int cap, HostBlockSweep& out) {
+  out = HostBlockSweep{};
+  const int m = f.m;
+  std::vector<int> parent((size_t)m, -1), nchild((size_t)m, 0);
+  for (int j = 0; j < m; ++j)
+    if (f.cp[(size_t)j + 1] > f.cp[(size_t)j]) {
+      parent[(size_t)j] = f.ci[(size_t)f.cp[(size_t)j]];
+      nchild[(size_t)parent[(size_t)j]]++;
+    }
+  std::vector<int> bs{0};
+  for (int k = 1; k < m; ++k) {
+    const bool chain = parent[(size_t)k - 1] == k && nchild[(size_t)k] == 1;
+    if (!chain || k - bs.back() >= cap) bs.push_back(k);
+  }
+  bs.push_back(m);
+  const int nb = (int)bs.size() - 1;
+  std::vector<int> blk((size_t)m);
+  for (int b = 0; b < nb; ++b)
+    for (int k = bs[(size_t)b]; k < bs[(size_t)b + 1]; ++k) blk[(size_t)k] = b;
+  // external entries of row k of block b: forward = row k's columns < start,
+  // backward = column k's rows >= end (both ascending)
+  auto extRange = [&](int k, int b, int& p0, int& p1) {
+    if (fwd) {
+      p0 = f.rp[(size_t)k];
+      p1 = p0;
+      while (p1 < f.rp[(size_t)k + 1] && f.rc[(size_t)p1] < bs[(size_t)b]) ++p1;
+    } else {
+      p1 = f.cp[(size_t)k + 1];
+      p0 = p1;
+      while (p0 > f.cp[(size_t)k] && f.ci[(size_t)p0 - 1] >= bs[(size_t)b + 1]) --p0;
+    }
+  };
+  std::vector<int> blev((size_t)nb, 0);
+  int nstep = 0;
+  for (int q = 0; q < nb; ++q) {
+    const int b = fwd ? q : nb - 1 - q;
+    int lv = 0;
+    for (int k = bs[(size_t)b]; k < bs[(size_t)b + 1]; ++k) {
+      int p0, p1;
+      extRange(k, b, p0, p1);
+      for (int p = p0; p < p1; ++p) {
+        const int o = fwd ? f.rc[(size_t)p] : f.ci[(size_t)p];
+        lv = std::max(lv, blev[(size_t)blk[(size_t)o]] + 1);
+      }
+    }
+    blev[(size_t)b] = lv;
+    nstep = std::max(nstep, lv + 1);
+  }
+  out.nstep = m ? nstep : 0;
+  std::vector<std::vector<int>> byLevel((size_t)out.nstep);
+  for (int b = 0; b < nb; ++b) byLevel[(size_t)blev[(size_t)b]].push_back(b);
+  out.stepTask.push_back(0);
+  auto extLen = [&](int k, int b) {
+    int p0, p1;
+    extRange(k, b, p0, p1);
+    return p1 - p0;
+  };
+  struct Task {
+    std::vector<int> ranks;
+    std::vector<int> blocks;
+    bool chain;
+    double cost;
+  };
+  for (int l = 0; l < out.nstep; ++l) {
+    std::vector<Task> tasks;
+    std::vector<std::pair<int, int>> singles;  // (extLen, block)
+    for (int b : byLevel[(size_t)l]) {
+      const int len = bs[(size_t)b + 1] - bs[(size_t)b];
+      if (len == 1) {
+        singles.push_back({extLen(bs[(size_t)b], b), b});
+      } else {
+        Task t;
+        t.chain = true;
+        int mx = 0;
+        for (int k = bs[(size_t)b]; k < bs[(size_t)b + 1]; ++k) {
+          t.ranks.push_back(k);
+          mx = std::max(mx, extLen(k, b));
+        }
+        t.blocks.assign((size_t)len, b);
+        t.cost = mx + 2.0 * len;
+        tasks.push_back(t);
+      }
+    }
+    std::stable_sort(singles.begin(), singles.end(), [](auto& a, auto& b) { return a.first > b.first; });
+    for (size_t c0 = 0; c0 < singles.size(); c0 += 64) {
+      Task t;
+      t.chain = false;
+      for (size_t q = c0; q < std::min(singles.size(), c0 + 64); ++q) {
+        t.ranks.push_back(bs[(size_t)singles[q].second]);
+        t.blocks.push_back(singles[q].second);
+      }
+      t.cost = singles[c0].first;
+      tasks.push_back(t);
+    }
+    std::stable_sort(tasks.begin(), tasks.end(), [](const Task& a, const Task& b) { return a.cost > b.cost; });
+    for (const Task& t : tasks) {
+      const int nr = (int)t.ranks.size();
+      int maxlen = 0;
+      std::vector<int> p0s((size_t)nr), lens((size_t)nr);
+      for (int r = 0; r < nr; ++r) {
+        int p0, p1;
+        extRange(t.ranks[(size_t)r], t.blocks[(size_t)r], p0, p1);
+        p0s[(size_t)r] = p0;
+        lens[(size_t)r] = p1 - p0;
+        maxlen = std::max(maxlen, p1 - p0);
+      }
+      out.taskInfo.push_back(nr | (t.chain ? (1 << 16) : 0));
+      out.taskMaxLen.push_back(maxlen);
+      out.taskRowBase.push_back((int)out.slotRank.size());
+      out.taskEntBase.push_back((int)out.col.size());
+      out.taskDenseBase.push_back(t.chain ? (int)out.dense.size() : 0);
+      for (int r = 0; r < nr; ++r) {
+        out.slotRank.push_back(t.ranks[(size_t)r]);
+        out.slotLen.push_back(lens[(size_t)r]);
+      }
+      for (int j = 0; j < maxlen; ++j)
+        for (int r = 0; r < nr; ++r) {
+          if (j < lens[(size_t)r]) {
+            const int p = p0s[(size_t)r] + j;
+            out.col.push_back(fwd ? f.rc[(size_t)p] : f.ci[(size_t)p]);
+            out.val.push_back(fwd ? f.rx[(size_t)p] : f.cx[(size_t)p]);
+          } else {
+            out.col.push_back(0);
+            out.val.push_back(0.0);
+          }
+        }
+      if (t.chain) {
+        const int s0 = t.ranks[0];
+        // L(s0+r, s0+c) for r > c from row s0+r of the CSR copy
+        std::vector<double> blkL((size_t)nr * nr, 0.0);
+        for (int r = 1; r < nr; ++r) {
+          const int k = s0 + r;
+          for (int p = f.rp[(size_t)k]; p < f.rp[(size_t)k + 1]; ++p) {
+            const int c = f.rc[(size_t)p] - s0;
+            if (c >= 0 && c < r) blkL[(size_t)r * nr + c] = f.rx[(size_t)p];
+          }
+        }
+        if (fwd) {
+          for (int sc = 0; sc < nr; ++sc)
+            for (int tt = 0; tt < nr; ++tt) out.dense.push_back(blkL[(size_t)tt * nr + sc]);
+        } else {
+          for (int r = 0; r < nr; ++r)
+            for (int tt = 0; tt < nr; ++tt) out.dense.push_back(blkL[(size_t)r * nr + tt]);
+        }
+      }
+    }
+    out.stepTask.push_back((int)out.taskInfo.size());
+  }
+}
+
 struct Instance {
   Scene scene;
   Config cfg;
@@ -91,6 +253,7 @@ struct Instance {
   int factorizeCount = 0, analyzeCount = 0;
   double analyzeSec = 0, factorSec = 0;
   HostSweep hf, hb;
+  HostBlockSweep bf, bb;
   bool packed = false;
   // device addresses of the writable state (inside the arena)
   double *dx = nullptr, *dv = nullptr, *dpin = nullptr, *dbrhs = nullptr;
@@ -133,6 +296,7 @@ struct pdx_ctx {
   bool ldsW = true;
   size_t lds = 0;
   int block = 256;
+  int chainCap = 32;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double lastKernelMs = 0.0;
   int lastLaunches = 0;
@@ -203,6 +367,11 @@ double sweepBytes(const HostSweep& s, size_t w) {
          (double)s.levChunk.size() * 4;
 }
 
+double blockBytes(const HostBlockSweep& s, size_t w) {
+  return (double)s.col.size() * (4 + w) + (double)s.dense.size() * w + (double)s.slotRank.size() * 8 +
+         (double)s.taskInfo.size() * 24 + (double)s.stepTask.size() * 4;
+}
+
 // Factor every instance whose stamps changed (parallel over instances), repack.
 int refreshFactors(pdx_ctx* c, bool& changed) {
   changed = false;
@@ -246,8 +415,13 @@ int refreshFactors(pdx_ctx* c, bool& changed) {
         I.factorizeCount += 1;
         I.factored = true;
         I.numStamp = numericStamp(I.scene, I.cfg.dt);
-        packSweep(I.fac.fwd, true, I.fac, I.hf);
-        packSweep(I.fac.bwd, false, I.fac, I.hb);
+        if (c->mode == PDX_SOLVE_FAST) {
+          packBlocked(I.fac, true, c->chainCap, I.bf);
+          packBlocked(I.fac, false, c->chainCap, I.bb);
+        } else {
+          packSweep(I.fac.fwd, true, I.fac, I.hf);
+          packSweep(I.fac.bwd, false, I.fac, I.hb);
+        }
         I.packed = true;
       }
       I.factorSec = now() - t0;
@@ -286,6 +460,7 @@ int upload(pdx_ctx* c) {
     size_t x, v, xbs, xhat, prev, mass, pinned, pinpos, brhs, ea, eb, rest, kappa, vstart, vedges, vedgesC,
         bstart, bst, bw, sa, sb, sc, bk, vrank, rankv, dinv, W, ist, lst, dst;
     size_t f[9], b[9];
+    size_t bf[13], bb[13];
   };
   std::vector<Off> offs(c->inst.size());
   c->bytesIter = 0;
@@ -341,6 +516,22 @@ int upload(pdx_ctx* c) {
       fo[q][6] = ar.vec(h.col);
       fo[q][7] = fp32 ? ar.vec(castVec<float>(h.val)) : ar.vec(h.val);
     }
+    HostBlockSweep* bsw[2] = {&I.bf, &I.bb};
+    size_t* bo[2] = {o.bf, o.bb};
+    for (int q = 0; q < 2; ++q) {
+      HostBlockSweep& h = *bsw[q];
+      bo[q][0] = ar.vec(h.stepTask);
+      bo[q][1] = ar.vec(h.taskInfo);
+      bo[q][2] = ar.vec(h.taskMaxLen);
+      bo[q][3] = ar.vec(h.taskRowBase);
+      bo[q][4] = ar.vec(h.taskEntBase);
+      bo[q][5] = ar.vec(h.taskDenseBase);
+      bo[q][6] = ar.vec(h.slotRank);
+      bo[q][7] = ar.vec(h.slotLen);
+      bo[q][8] = ar.vec(h.col);
+      bo[q][9] = fp32 ? ar.vec(castVec<float>(h.val)) : ar.vec(h.val);
+      bo[q][10] = fp32 ? ar.vec(castVec<float>(h.dense)) : ar.vec(h.dense);
+    }
     o.W = c->ldsW ? 0 : ar.add(nullptr, (size_t)3 * I.fac.m * w);
     std::vector<int> ist(4, 0);
     std::vector<long long> lst(2, 0);
@@ -351,7 +542,8 @@ int upload(pdx_ctx* c) {
     // algorithmic bytes per PD iteration: both sweeps + dinv + maps, the gather's
     // scene arrays twice (rhs + residual), the per-vertex state passes
     const double nb = (double)s.n, Eb = (double)s.E, Bb = (double)s.B;
-    double bi = sweepBytes(I.hf, w) + sweepBytes(I.hb, w) + nb * (w + 8);
+    double bi = sweepBytes(I.hf, w) + sweepBytes(I.hb, w) + blockBytes(I.bf, w) + blockBytes(I.bb, w) +
+                nb * (w + 8);
     bi += 2.0 * (Eb * (8 + 16) + (nb + 1) * 4 + 2 * Eb * 4);
     bi += nb * 24 * 4 + nb * 8 + nb;
     if (Bb > 0) bi += Bb * (12 + 8) + (nb + 1) * 4 + 3 * Bb * 5 + nb * 24;
@@ -442,6 +634,24 @@ int upload(pdx_ctx* c) {
       sd[q]->col = (const int*)(B + fo[q][6]);
       sd[q]->val = B + fo[q][7];
     }
+    BlockSweepDev* bd[2] = {&d.bfwd, &d.bbwd};
+    const size_t* bo[2] = {o.bf, o.bb};
+    const HostBlockSweep* bsw[2] = {&I.bf, &I.bb};
+    for (int q = 0; q < 2; ++q) {
+      bd[q]->nstep = bsw[q]->nstep;
+      bd[q]->cap = c->chainCap;
+      bd[q]->stepTask = (const int*)(B + bo[q][0]);
+      bd[q]->taskInfo = (const int*)(B + bo[q][1]);
+      bd[q]->taskMaxLen = (const int*)(B + bo[q][2]);
+      bd[q]->taskRowBase = (const int*)(B + bo[q][3]);
+      bd[q]->taskEntBase = (const int*)(B + bo[q][4]);
+      bd[q]->taskDenseBase = (const int*)(B + bo[q][5]);
+      bd[q]->slotRank = (const int*)(B + bo[q][6]);
+      bd[q]->slotLen = (const int*)(B + bo[q][7]);
+      bd[q]->col = (const int*)(B + bo[q][8]);
+      bd[q]->val = B + bo[q][9];
+      bd[q]->dense = B + bo[q][10];
+    }
     d.Wg = c->ldsW ? nullptr : (void*)(B + o.W);
     d.istats = (int*)(B + o.ist);
     d.lstats = (long long*)(B + o.lst);
@@ -514,7 +724,7 @@ int launch(pdx_ctx* c, int nsub) {
   HIPCHECK(c, hipSetDevice(c->device));
   HIPCHECK(c, hipEventRecord(c->ev0, c->stream));
   HIPCHECK(c, launchSubstep(c->dInst, c->dOrder, (int)c->inst.size(), nsub, c->block, c->lds,
-                            c->prec == PDX_PREC_MIXED32, c->stream));
+                            c->prec == PDX_PREC_MIXED32, c->mode == PDX_SOLVE_FAST, c->stream));
   HIPCHECK(c, hipEventRecord(c->ev1, c->stream));
   c->lastLaunches = 1;
   return PDX_OK;
@@ -632,7 +842,7 @@ int pdx_step_graph(pdx_ctx* c, int nsub, int replays) {
   hipGraphExec_t ge = nullptr;
   HIPCHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   const hipError_t le = launchSubstep(c->dInst, c->dOrder, (int)c->inst.size(), nsub, c->block, c->lds,
-                                      c->prec == PDX_PREC_MIXED32, c->stream);
+                                      c->prec == PDX_PREC_MIXED32, c->mode == PDX_SOLVE_FAST, c->stream);
   HIPCHECK(c, hipStreamEndCapture(c->stream, &g));
   if (le != hipSuccess) {
     hipGraphDestroy(g);
@@ -720,8 +930,8 @@ int pdx_get_stats(pdx_ctx* c, int i, pdx_stats* s) {
   s->factor_nnz = I.fac.nnz;
   s->fwd_levels = I.fac.fwd.nlev;
   s->bwd_levels = I.fac.bwd.nlev;
-  s->fwd_steps = I.hf.nlev;
-  s->bwd_steps = I.hb.nlev;
+  s->fwd_steps = c->mode == PDX_SOLVE_FAST ? I.bf.nstep : I.hf.nlev;
+  s->bwd_steps = c->mode == PDX_SOLVE_FAST ? I.bb.nstep : I.hb.nlev;
   s->analyze_seconds = I.analyzeSec;
   s->factorize_seconds = I.factorSec;
   s->last_residual = -1.0;
@@ -758,7 +968,7 @@ int pdx_solve3(pdx_ctx* c, int i, const double* b, double* x) {
   }
   HIPCHECK(c, hipMemcpyAsync(c->dScratchB, b, (size_t)n3 * 8, hipMemcpyHostToDevice, c->stream));
   HIPCHECK(c, launchSolve(c->dInst, i, c->dScratchB, c->dScratchX, c->block, c->lds,
-                          c->prec == PDX_PREC_MIXED32, c->stream));
+                          c->prec == PDX_PREC_MIXED32, c->mode == PDX_SOLVE_FAST, c->stream));
   HIPCHECK(c, hipMemcpyAsync(x, c->dScratchX, (size_t)n3 * 8, hipMemcpyDeviceToHost, c->stream));
   HIPCHECK(c, hipStreamSynchronize(c->stream));
   return PDX_OK;

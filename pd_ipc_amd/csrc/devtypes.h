@@ -23,6 +23,31 @@ struct SweepDev {
   const void* val;  // double or float by precision
 };
 
+// Block-scheduled sweep (fast mode).  The ranks are cut into contiguous blocks:
+// elimination-tree chains of at most `cap` rows, and single rows.  Step st owns
+// tasks [stepTask[st], stepTask[st+1]); a task is one wavefront's work:
+//  * a row batch (<= 64 single-row blocks, one per lane, gather form), or
+//  * a chain block (lane t = rank start+t): the external part of every row is
+//    gathered in parallel, then the block's own dense lower triangle is applied
+//    column by column, each finished value broadcast to the other lanes.
+// External entries are ELL: entry j of lane t at entBase + j*nrows + t.
+struct BlockSweepDev {
+  int nstep;
+  int cap;
+  const int* stepTask;
+  const int* taskInfo;      // nrows | (chain << 16)
+  const int* taskMaxLen;
+  const int* taskRowBase;
+  const int* taskEntBase;
+  const int* taskDenseBase;
+  const int* slotRank;
+  const int* slotLen;
+  const int* col;
+  const void* val;
+  const void* dense;        // chain blocks: cap x cap, [s * nrows + t] = L(start+t, start+s) (forward)
+                            // or [r * nrows + t] = L(start+r, start+t) (backward)
+};
+
 struct InstDev {
   int n, E, B, m;
   int maxIter, wantResidual, hasBendRhs, ldsW;
@@ -60,6 +85,7 @@ struct InstDev {
   const int* rankv;   // rank -> vertex
   const void* dinv;   // Real[m]
   SweepDev fwd, bwd;
+  BlockSweepDev bfwd, bbwd;
   void* Wg;           // global W scratch (3m Real) when W is not LDS-resident
   // statistics
   int* istats;        // [0] iterations used, [1] converged, [2] early exits

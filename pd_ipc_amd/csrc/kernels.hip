@@ -115,6 +115,122 @@ __device__ void sweep(const SweepDev& sw, Real* W, const Real* dinv, Store store
   }
 }
 
+__device__ inline double readlaneT(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ inline float readlaneT(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+constexpr int kCap = 32;
+
+// Block-scheduled sweep: per step, each wavefront takes whole tasks (a chain block
+// or a batch of single rows).  Chain blocks keep their dense triangle in
+// registers (lane t holds row/column t) and broadcast each finished value with
+// readlane, so a chain of c rows costs c short ALU steps instead of c barriers.
+// Forward rows keep the reference's ascending column order (external columns
+// precede the block's own); backward rows apply the block's own rows last.
+template <typename Real, bool kFwd, typename Store>
+__device__ void blockSweep(const BlockSweepDev& sw, Real* W, const Real* dinv, Store store) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = threadIdx.x / kWave;
+  const int nw = blockDim.x / kWave;
+  const Real* val = static_cast<const Real*>(sw.val);
+  const Real* dense = static_cast<const Real*>(sw.dense);
+  for (int st = 0; st < sw.nstep; ++st) {
+    const int t1 = sw.stepTask[st + 1];
+    for (int t = sw.stepTask[st] + wave; t < t1; t += nw) {
+      const int info = sw.taskInfo[t];
+      const int nrows = info & 0xffff;
+      const bool chain = (info >> 16) != 0;
+      const int maxlen = sw.taskMaxLen[t];
+      const int rbase = sw.taskRowBase[t];
+      const int ebase = sw.taskEntBase[t];
+      const bool act = lane < nrows;
+      int k = 0, len = 0;
+      if (act) {
+        k = sw.slotRank[rbase + lane];
+        len = sw.slotLen[rbase + lane];
+      }
+      Real dreg[kCap];
+      if (chain) {
+        const Real* D = dense + sw.taskDenseBase[t];
+#pragma unroll
+        for (int q = 0; q < kCap; ++q) dreg[q] = (act && q < nrows) ? D[q * nrows + lane] : Real(0);
+      }
+      Real a0 = 0, a1 = 0, a2 = 0;
+      if (act) {
+        a0 = W[3 * k + 0];
+        a1 = W[3 * k + 1];
+        a2 = W[3 * k + 2];
+        if (!kFwd) {
+          const Real dv = dinv[k];
+          a0 = a0 * dv;
+          a1 = a1 * dv;
+          a2 = a2 * dv;
+        }
+      }
+      for (int j = 0; j < maxlen; j += 4) {
+        int cc[4];
+        Real lv[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const bool ok = act && (j + q < maxlen);
+          const int e = ebase + (j + q) * nrows + lane;
+          cc[q] = ok ? sw.col[e] : 0;
+          lv[q] = ok ? val[e] : Real(0);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          if (j + q < len) {
+            const int c = cc[q];
+            a0 = subMul<Real>(a0, lv[q], W[3 * c + 0]);
+            a1 = subMul<Real>(a1, lv[q], W[3 * c + 1]);
+            a2 = subMul<Real>(a2, lv[q], W[3 * c + 2]);
+          }
+        }
+      }
+      if (chain) {
+        if (kFwd) {
+#pragma unroll
+          for (int q = 0; q < kCap; ++q) {
+            if (q < nrows) {
+              const Real y0 = readlaneT(a0, q), y1 = readlaneT(a1, q), y2 = readlaneT(a2, q);
+              if (lane > q && dreg[q] != Real(0)) {
+                a0 = subMul<Real>(a0, dreg[q], y0);
+                a1 = subMul<Real>(a1, dreg[q], y1);
+                a2 = subMul<Real>(a2, dreg[q], y2);
+              }
+            }
+          }
+        } else {
+#pragma unroll
+          for (int q = kCap - 1; q >= 0; --q) {
+            if (q < nrows) {
+              const Real y0 = readlaneT(a0, q), y1 = readlaneT(a1, q), y2 = readlaneT(a2, q);
+              if (lane < q && dreg[q] != Real(0)) {
+                a0 = subMul<Real>(a0, dreg[q], y0);
+                a1 = subMul<Real>(a1, dreg[q], y1);
+                a2 = subMul<Real>(a2, dreg[q], y2);
+              }
+            }
+          }
+        }
+      }
+      if (act) {
+        W[3 * k + 0] = a0;
+        W[3 * k + 1] = a1;
+        W[3 * k + 2] = a2;
+        store(k, a0, a1, a2);
+      }
+    }
+    __syncthreads();
+  }
+}
+
 struct Acc3 {
   double x, y, z;
 };
@@ -291,15 +407,19 @@ __device__ void gatherFirst(const InstDev& I, Real* W) {
   }
 }
 
-template <typename Real, bool kExact>
+template <typename Real, bool kExact, bool kBlocked>
 __device__ void runSweeps(const InstDev& I, Real* W) {
   const Real* dinv = static_cast<const Real*>(I.dinv);
-  sweep<Real, true>(I.fwd, W, dinv, [](int, Real, Real, Real) {});
+  auto none = [](int, Real, Real, Real) {};
+  if (kBlocked)
+    blockSweep<Real, true>(I.bfwd, W, dinv, none);
+  else
+    sweep<Real, true>(I.fwd, W, dinv, none);
   double* x = I.x;
   const int* rankv = I.rankv;
   const uint8_t* pinned = I.pinned;
   const double* pinpos = I.pinpos;
-  sweep<Real, false>(I.bwd, W, dinv, [=](int k, Real a0, Real a1, Real a2) {
+  auto put = [=](int k, Real a0, Real a1, Real a2) {
     const int v = rankv[k];
     if (kExact) {
       x[3 * v] = (double)a0;
@@ -314,10 +434,14 @@ __device__ void runSweeps(const InstDev& I, Real* W) {
       x[3 * v + 1] += (double)a1;
       x[3 * v + 2] += (double)a2;
     }
-  });
+  };
+  if (kBlocked)
+    blockSweep<Real, false>(I.bbwd, W, dinv, put);
+  else
+    sweep<Real, false>(I.bwd, W, dinv, put);
 }
 
-template <typename Real, bool kExact>
+template <typename Real, bool kExact, bool kBlocked>
 __global__ void __launch_bounds__(1024) substepKernel(const InstDev* __restrict__ insts,
                                                       const int* __restrict__ order, int nsub) {
   extern __shared__ double smem[];
@@ -348,7 +472,7 @@ __global__ void __launch_bounds__(1024) substepKernel(const InstDev* __restrict_
     int used = 0, conv = 0;
     double lastRes = -1.0;
     for (int k = 0; k < I.maxIter; ++k) {
-      runSweeps<Real, kExact>(I, W);
+      runSweeps<Real, kExact, kBlocked>(I, W);
       const bool more = (k + 1 < I.maxIter);
       double ldiff = 0.0, lres = 0.0;
       for (int v = tid; v < n; v += bs) {
@@ -423,7 +547,7 @@ __global__ void __launch_bounds__(1024) substepKernel(const InstDev* __restrict_
 }
 
 // Stand-alone global solve x = L^{-1} b (EigenDirectSolver::solve) on one instance.
-template <typename Real>
+template <typename Real, bool kBlocked>
 __global__ void __launch_bounds__(1024) solveKernel(const InstDev* __restrict__ insts, int inst,
                                                     const double* __restrict__ b,
                                                     double* __restrict__ xout) {
@@ -436,48 +560,64 @@ __global__ void __launch_bounds__(1024) solveKernel(const InstDev* __restrict__ 
   }
   __syncthreads();
   const Real* dinv = static_cast<const Real*>(I.dinv);
-  sweep<Real, true>(I.fwd, W, dinv, [](int, Real, Real, Real) {});
+  auto none = [](int, Real, Real, Real) {};
+  if (kBlocked)
+    blockSweep<Real, true>(I.bfwd, W, dinv, none);
+  else
+    sweep<Real, true>(I.fwd, W, dinv, none);
   const int* rankv = I.rankv;
-  sweep<Real, false>(I.bwd, W, dinv, [=](int k, Real a0, Real a1, Real a2) {
+  auto put = [=](int k, Real a0, Real a1, Real a2) {
     const int v = rankv[k];
     xout[3 * v] = (double)a0;
     xout[3 * v + 1] = (double)a1;
     xout[3 * v + 2] = (double)a2;
-  });
+  };
+  if (kBlocked)
+    blockSweep<Real, false>(I.bbwd, W, dinv, put);
+  else
+    sweep<Real, false>(I.bwd, W, dinv, put);
 }
 
 }  // namespace
 
+#define PDX_KERNEL_TABLE(X) X(float, false, true) X(float, false, false) X(double, true, true) X(double, true, false)
+
 hipError_t launchSubstep(const InstDev* d, const int* order, int ninst, int nsub, int block,
-                         size_t lds, bool fp32, hipStream_t s) {
-  if (fp32)
-    hipLaunchKernelGGL((substepKernel<float, false>), dim3(ninst), dim3(block), lds, s, d, order, nsub);
-  else
-    hipLaunchKernelGGL((substepKernel<double, true>), dim3(ninst), dim3(block), lds, s, d, order, nsub);
-  return hipGetLastError();
+                         size_t lds, bool fp32, bool blocked, hipStream_t s) {
+#define PDX_L(R, E, B)                                                                            \
+  if (fp32 == (sizeof(R) == 4) && blocked == B) {                                                 \
+    hipLaunchKernelGGL((substepKernel<R, E, B>), dim3(ninst), dim3(block), lds, s, d, order, nsub); \
+    return hipGetLastError();                                                                     \
+  }
+  PDX_KERNEL_TABLE(PDX_L)
+#undef PDX_L
+  return hipErrorInvalidValue;
 }
 
 hipError_t launchSolve(const InstDev* d, int inst, const double* b, double* x, int block,
-                       size_t lds, bool fp32, hipStream_t s) {
-  if (fp32)
-    hipLaunchKernelGGL((solveKernel<float>), dim3(1), dim3(block), lds, s, d, inst, b, x);
-  else
-    hipLaunchKernelGGL((solveKernel<double>), dim3(1), dim3(block), lds, s, d, inst, b, x);
-  return hipGetLastError();
+                       size_t lds, bool fp32, bool blocked, hipStream_t s) {
+#define PDX_L(R, E, B)                                                                  \
+  if (fp32 == (sizeof(R) == 4) && blocked == B) {                                       \
+    hipLaunchKernelGGL((solveKernel<R, B>), dim3(1), dim3(block), lds, s, d, inst, b, x); \
+    return hipGetLastError();                                                           \
+  }
+  PDX_KERNEL_TABLE(PDX_L)
+#undef PDX_L
+  return hipErrorInvalidValue;
 }
 
 hipError_t setLdsLimit(size_t bytes) {
-  hipError_t e = hipFuncSetAttribute((const void*)substepKernel<float, false>,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  hipError_t e = hipSuccess;
+#define PDX_L(R, E, B)                                                                                \
+  e = hipFuncSetAttribute((const void*)substepKernel<R, E, B>, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                          (int)bytes);                                                                \
+  if (e != hipSuccess) return e;                                                                      \
+  e = hipFuncSetAttribute((const void*)solveKernel<R, B>, hipFuncAttributeMaxDynamicSharedMemorySize,     \
+                          (int)bytes);                                                                \
   if (e != hipSuccess) return e;
-  e = hipFuncSetAttribute((const void*)substepKernel<double, true>,
-                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-  if (e != hipSuccess) return e;
-  e = hipFuncSetAttribute((const void*)solveKernel<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                          (int)bytes);
-  if (e != hipSuccess) return e;
-  return hipFuncSetAttribute((const void*)solveKernel<double>,
-                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  PDX_KERNEL_TABLE(PDX_L)
+#undef PDX_L
+  return e;
 }
 
 }  // namespace pdx

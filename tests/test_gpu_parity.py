@@ -86,3 +86,46 @@ def test_mixed32_error_bound():
     err = np.abs(xg - o.positions()).max()
     print("max position error", err)
     assert err <= 1e-5
+
+
+@pytest.mark.parametrize("nx,ny,shear,bend", [(16, 16, 0.0, 0.0), (33, 21, 500.0, 50.0), (60, 60, 500.0, 50.0)])
+def test_solve3_fast_tolerance(nx, ny, shear, bend):
+    """Chain-blocked sweeps reorder the backward sums: equal to the oracle's solve
+    to rounding."""
+    _say("solve3 fast", nx, ny, shear, bend)
+    c = scenes.grid_cloth(nx, ny, 2300.0, shear=shear, bend=bend)
+    cfg = SimConfig.fixed_iterations(10, scenes.H600)
+    e, o = _pair(c, cfg, solve="fast")
+    b = np.random.default_rng(11).standard_normal(3 * c.n)
+    xg = e.solve(0, b)
+    xo = o.solve(b)
+    assert np.abs(xg - xo).max() <= 1e-12 * np.abs(xo).max()
+
+
+@pytest.mark.parametrize("nx,ny,shear,bend,iters,nsub", [(40, 40, 0.0, 0.0, 10, 60), (30, 22, 500.0, 50.0, 40, 30)])
+def test_step_fast_within_1e10(nx, ny, shear, bend, iters, nsub):
+    """The fast FP64 drop-in stays within 1e-10 relative of the reference path."""
+    _say("step fast", nx, ny, shear, bend, iters, nsub)
+    c = scenes.grid_cloth(nx, ny, 2300.0, shear=shear, bend=bend)
+    cfg = SimConfig.fixed_iterations(iters, scenes.H600)
+    e, o = _pair(c, cfg, solve="fast")
+    e.step(nsub)
+    o.step(nsub)
+    xg, _ = e.state(0)
+    xo = o.positions()
+    rel = np.abs(xg - xo).max() / np.abs(xo).max()
+    print("relative position difference", rel)
+    assert rel <= 1e-10
+
+
+def test_mixed32_fast_error_bound():
+    _say("mixed32 fast vs oracle 48x40 shear+bend")
+    c = scenes.grid_cloth(48, 40, 1000.0, shear=200.0, bend=20.0, pins=[0, 47])
+    cfg = SimConfig.fixed_iterations(40, scenes.H600)
+    e, o = _pair(c, cfg, precision="mixed32", solve="fast")
+    e.step(50)
+    o.step(50)
+    xg, _ = e.state(0)
+    err = np.abs(xg - o.positions()).max()
+    print("max position error", err)
+    assert err <= 1e-5
