@@ -97,7 +97,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--instances", type=int, default=512)
     ap.add_argument("--seed", type=int, default=4)
-    ap.add_argument("--solve", default="exact", choices=["exact", "fast"])
+    ap.add_argument("--solve", default="fast", choices=["exact", "fast"])
     ap.add_argument("--dropin-seconds", type=float, default=20.0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")

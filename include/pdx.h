@@ -97,6 +97,11 @@ typedef struct {
   int bwd_steps;
   double analyze_seconds;
   double factorize_seconds;
+  /* device time of workgroup 0 of the instance per phase, summed over substeps:
+   * gather (rhs + residual), forward sweep, backward sweep, predict + velocity,
+   * criterion reductions; fast mode, both sweeps: run streaming of thread 0, its wait at
+   * the step barrier, finalisation of wavefront 0, its wait at the second barrier */
+  double phase_seconds[9];
 } pdx_stats;
 
 /* --- context ---------------------------------------------------------------- */
@@ -141,6 +146,11 @@ int pdx_solve3(pdx_ctx* ctx, int instance, const double* b, double* x);
 int pdx_get_factor(pdx_ctx* ctx, int instance, int* m, long long* nnz, int32_t* ptr,
                    int32_t* rows, double* vals, double* dinv, int32_t* rank_vertex,
                    int32_t* perm3);
+
+/* Launch shape: threads per workgroup (64..512, multiple of 64) and the longest
+ * elimination-tree chain one wavefront takes as a block (1..32, fast mode).
+ * Changing the chain cap re-packs the schedules (no new factorisation is counted). */
+int pdx_set_schedule(pdx_ctx* ctx, int block_threads, int chain_cap);
 
 /* --- timing ---------------------------------------------------------------- */
 /* Average duration (ms) of the substep kernel over the last `pdx_step*` call,

@@ -19,9 +19,9 @@
 
 namespace pdx {
 hipError_t launchSubstep(const InstDev* d, const int* order, int ninst, int nsub, int block,
-                         size_t lds, bool fp32, bool blocked, hipStream_t s);
+                         size_t lds, bool fp32, bool blocked, bool idx16, hipStream_t s);
 hipError_t launchSolve(const InstDev* d, int inst, const double* b, double* x, int block,
-                       size_t lds, bool fp32, bool blocked, hipStream_t s);
+                       size_t lds, bool fp32, bool blocked, bool idx16, hipStream_t s);
 hipError_t setLdsLimit(size_t bytes);
 }  // namespace pdx
 
@@ -79,18 +79,31 @@ void packSweep(const Sweep& sw, bool fwd, const Factor& f, HostSweep& out) {
   }
 }
 
+constexpr int kRun = 16;  // factor entries per streamed run (matches the kernel)
+
 struct HostBlockSweep {
   int nstep = 0;
-  std::vector<int> stepTask, taskInfo, taskMaxLen, taskRowBase, taskEntBase, taskDenseBase, slotRank, slotLen,
-      col;
-  std::vector<double> val, dense;
+  int maxRuns = 0;        // most runs in one step (partial-sum slots in LDS)
+  int maxDense = 0;       // most dense-block values in one step (LDS staging)
+  std::vector<int> stepTask, stepRun, stepDense;  // nstep+1 each
+  std::vector<int> taskInfo, taskSlot, taskDense; // per task
+  std::vector<int> slotRank, slotRunBeg, slotRunEnd;
+  std::vector<int> laneMeta;                      // 64 lanes x {rank, run begin, run end, info} per task
+  std::vector<double> laneDinv;                   // 64 lanes x 1/D of the lane's rank per task
+  std::vector<int> runCol;                        // kRun per run
+  std::vector<double> runVal, dense;
 };
 
-// Block schedule of one sweep.  Ranks are cut into contiguous blocks: a block
-// continues while rank k-1 has parent k in the elimination tree and k has no
-// other child (a chain), up to `cap` rows.  Block levels follow the dependencies
-// between blocks; every level becomes one barrier-separated step whose tasks are
-// chain blocks (one wavefront each) and batches of single-row blocks.
+// Block schedule of one sweep (fast mode).  Ranks are cut into contiguous
+// blocks: a block continues while rank k-1 has parent k in the elimination tree
+// and k has no other child (a chain), up to `cap` rows.  Block levels follow the
+// dependencies between blocks; every level is one step.  Within a step, the
+// external entries of all its rows (forward: columns before the row's block,
+// backward: rows after it) are cut into runs of kRun entries, padded, laid out
+// contiguously, so that the whole workgroup streams a step with one wide load
+// per run; each run yields one partial sum.  The tasks of a step (chain blocks,
+// one wavefront each, and batches of up to 64 single rows) then combine their
+// rows' partials in run order and finish the chains.
 void packBlocked(const Factor& f, bool fwd, int cap, HostBlockSweep& out) {
   out = HostBlockSweep{};
   const int m = f.m;
@@ -110,8 +123,6 @@ void packBlocked(const Factor& f, bool fwd, int cap, HostBlockSweep& out) {
   std::vector<int> blk((size_t)m);
   for (int b = 0; b < nb; ++b)
     for (int k = bs[(size_t)b]; k < bs[(size_t)b + 1]; ++k) blk[(size_t)k] = b;
-  // external entries of row k of block b: forward = row k's columns < start,
-  // backward = column k's rows >= end (both ascending)
   auto extRange = [&](int k, int b, int& p0, int& p1) {
     if (fwd) {
       p0 = f.rp[(size_t)k];
@@ -142,21 +153,22 @@ void packBlocked(const Factor& f, bool fwd, int cap, HostBlockSweep& out) {
   out.nstep = m ? nstep : 0;
   std::vector<std::vector<int>> byLevel((size_t)out.nstep);
   for (int b = 0; b < nb; ++b) byLevel[(size_t)blev[(size_t)b]].push_back(b);
-  out.stepTask.push_back(0);
   auto extLen = [&](int k, int b) {
     int p0, p1;
     extRange(k, b, p0, p1);
     return p1 - p0;
   };
   struct Task {
-    std::vector<int> ranks;
-    std::vector<int> blocks;
+    std::vector<int> ranks, blocks;
     bool chain;
     double cost;
   };
+  out.stepTask.push_back(0);
+  out.stepRun.push_back(0);
+  out.stepDense.push_back(0);
   for (int l = 0; l < out.nstep; ++l) {
     std::vector<Task> tasks;
-    std::vector<std::pair<int, int>> singles;  // (extLen, block)
+    std::vector<std::pair<int, int>> singles;
     for (int b : byLevel[(size_t)l]) {
       const int len = bs[(size_t)b + 1] - bs[(size_t)b];
       if (len == 1) {
@@ -164,13 +176,9 @@ void packBlocked(const Factor& f, bool fwd, int cap, HostBlockSweep& out) {
       } else {
         Task t;
         t.chain = true;
-        int mx = 0;
-        for (int k = bs[(size_t)b]; k < bs[(size_t)b + 1]; ++k) {
-          t.ranks.push_back(k);
-          mx = std::max(mx, extLen(k, b));
-        }
+        for (int k = bs[(size_t)b]; k < bs[(size_t)b + 1]; ++k) t.ranks.push_back(k);
         t.blocks.assign((size_t)len, b);
-        t.cost = mx + 2.0 * len;
+        t.cost = 4.0 * len;
         tasks.push_back(t);
       }
     }
@@ -182,44 +190,53 @@ void packBlocked(const Factor& f, bool fwd, int cap, HostBlockSweep& out) {
         t.ranks.push_back(bs[(size_t)singles[q].second]);
         t.blocks.push_back(singles[q].second);
       }
-      t.cost = singles[c0].first;
+      t.cost = 1.0;
       tasks.push_back(t);
     }
     std::stable_sort(tasks.begin(), tasks.end(), [](const Task& a, const Task& b) { return a.cost > b.cost; });
+    const int runBase = out.stepRun.back();
+    const int denseBase = out.stepDense.back();
+    int runs = 0;
     for (const Task& t : tasks) {
       const int nr = (int)t.ranks.size();
-      int maxlen = 0;
-      std::vector<int> p0s((size_t)nr), lens((size_t)nr);
-      for (int r = 0; r < nr; ++r) {
-        int p0, p1;
-        extRange(t.ranks[(size_t)r], t.blocks[(size_t)r], p0, p1);
-        p0s[(size_t)r] = p0;
-        lens[(size_t)r] = p1 - p0;
-        maxlen = std::max(maxlen, p1 - p0);
-      }
       out.taskInfo.push_back(nr | (t.chain ? (1 << 16) : 0));
-      out.taskMaxLen.push_back(maxlen);
-      out.taskRowBase.push_back((int)out.slotRank.size());
-      out.taskEntBase.push_back((int)out.col.size());
-      out.taskDenseBase.push_back(t.chain ? (int)out.dense.size() : 0);
-      for (int r = 0; r < nr; ++r) {
-        out.slotRank.push_back(t.ranks[(size_t)r]);
-        out.slotLen.push_back(lens[(size_t)r]);
+      out.taskSlot.push_back((int)out.slotRank.size());
+      out.taskDense.push_back(t.chain ? (int)out.dense.size() - denseBase : 0);
+      const size_t meta0 = out.laneMeta.size();
+      const int info = nr | (t.chain ? (1 << 7) : 0) | ((t.chain ? (int)out.dense.size() - denseBase : 0) << 8);
+      for (int ln = 0; ln < 64; ++ln) {
+        out.laneMeta.push_back(-1);
+        out.laneMeta.push_back(0);
+        out.laneMeta.push_back(0);
+        out.laneMeta.push_back(info);
+        out.laneDinv.push_back(0.0);
       }
-      for (int j = 0; j < maxlen; ++j)
-        for (int r = 0; r < nr; ++r) {
-          if (j < lens[(size_t)r]) {
-            const int p = p0s[(size_t)r] + j;
-            out.col.push_back(fwd ? f.rc[(size_t)p] : f.ci[(size_t)p]);
-            out.val.push_back(fwd ? f.rx[(size_t)p] : f.cx[(size_t)p]);
-          } else {
-            out.col.push_back(0);
-            out.val.push_back(0.0);
+      for (int r = 0; r < nr; ++r) {
+        const int k = t.ranks[(size_t)r];
+        int p0, p1;
+        extRange(k, t.blocks[(size_t)r], p0, p1);
+        out.slotRank.push_back(k);
+        out.slotRunBeg.push_back(runs);
+        for (int p = p0; p < p1; p += kRun) {
+          for (int q = 0; q < kRun; ++q) {
+            if (p + q < p1) {
+              out.runCol.push_back(fwd ? f.rc[(size_t)(p + q)] : f.ci[(size_t)(p + q)]);
+              out.runVal.push_back(fwd ? f.rx[(size_t)(p + q)] : f.cx[(size_t)(p + q)]);
+            } else {
+              out.runCol.push_back(0);
+              out.runVal.push_back(0.0);
+            }
           }
+          ++runs;
         }
+        out.slotRunEnd.push_back(runs);
+        out.laneMeta[meta0 + 4 * (size_t)r + 0] = k;
+        out.laneMeta[meta0 + 4 * (size_t)r + 1] = out.slotRunBeg.back();
+        out.laneMeta[meta0 + 4 * (size_t)r + 2] = runs;
+        out.laneDinv[meta0 / 4 + (size_t)r] = f.dinv[(size_t)k];
+      }
       if (t.chain) {
         const int s0 = t.ranks[0];
-        // L(s0+r, s0+c) for r > c from row s0+r of the CSR copy
         std::vector<double> blkL((size_t)nr * nr, 0.0);
         for (int r = 1; r < nr; ++r) {
           const int k = s0 + r;
@@ -237,7 +254,13 @@ void packBlocked(const Factor& f, bool fwd, int cap, HostBlockSweep& out) {
         }
       }
     }
+    // pad the step's dense block to a multiple of 4 values (16-byte copies)
+    while ((out.dense.size() - (size_t)denseBase) % 4) out.dense.push_back(0.0);
+    out.maxRuns = std::max(out.maxRuns, runs);
+    out.maxDense = std::max(out.maxDense, (int)out.dense.size() - denseBase);
     out.stepTask.push_back((int)out.taskInfo.size());
+    out.stepRun.push_back(runBase + runs);
+    out.stepDense.push_back((int)out.dense.size());
   }
 }
 
@@ -295,8 +318,9 @@ struct pdx_ctx {
   bool dirty = true;
   bool ldsW = true;
   size_t lds = 0;
-  int block = 256;
+  int block = 512;
   int chainCap = 32;
+  bool idx16 = false;  // fast-mode run columns stored as 16-bit ranks (every instance has m <= 65535)
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double lastKernelMs = 0.0;
   int lastLaunches = 0;
@@ -367,9 +391,20 @@ double sweepBytes(const HostSweep& s, size_t w) {
          (double)s.levChunk.size() * 4;
 }
 
+size_t align16(size_t b) { return (b + 15) / 16 * 16; }
+size_t partBytes(const Instance& I, size_t w) {
+  return align16((size_t)std::max(I.bf.maxRuns, I.bb.maxRuns) * 3 * w);
+}
+size_t denseBytes(const Instance& I, size_t w) {
+  return align16((size_t)std::max(I.bf.maxDense, I.bb.maxDense) * w);
+}
+size_t stepBytes(const Instance& I) {
+  return align16((size_t)3 * (I.bf.nstep + 1 + I.bb.nstep + 1) * sizeof(int));
+}
+
 double blockBytes(const HostBlockSweep& s, size_t w) {
-  return (double)s.col.size() * (4 + w) + (double)s.dense.size() * w + (double)s.slotRank.size() * 8 +
-         (double)s.taskInfo.size() * 24 + (double)s.stepTask.size() * 4;
+  return (double)s.runCol.size() * (4 + w) + (double)s.dense.size() * w + (double)s.slotRank.size() * 12 +
+         (double)s.taskInfo.size() * 12 + (double)s.stepTask.size() * 12;
 }
 
 // Factor every instance whose stamps changed (parallel over instances), repack.
@@ -451,16 +486,24 @@ std::vector<T> castVec(const std::vector<double>& v) {
 int upload(pdx_ctx* c) {
   const bool fp32 = c->prec == PDX_PREC_MIXED32;
   const size_t w = fp32 ? 4 : 8;
-  size_t maxW = 0;
-  for (auto& I : c->inst) maxW = std::max(maxW, (size_t)3 * I.fac.m * w);
-  c->ldsW = maxW + kRedBytes <= kLdsLimit;
-  c->lds = kRedBytes + (c->ldsW ? maxW : 0);
+  c->idx16 = true;
+  for (auto& I : c->inst) c->idx16 = c->idx16 && I.fac.m <= 65535;
+  size_t maxW = 0, maxExtra = 0;
+  for (auto& I : c->inst) {
+    maxW = std::max(maxW, (size_t)3 * I.fac.m * w);
+    if (c->mode == PDX_SOLVE_FAST) maxExtra = std::max(maxExtra, partBytes(I, w) + denseBytes(I, w) + stepBytes(I));
+  }
+  c->ldsW = kRedBytes + maxW + maxExtra <= kLdsLimit;
+  c->lds = kRedBytes + (c->ldsW ? maxW : 0) + maxExtra;
+  if (c->lds > kLdsLimit)
+    return fail(c, PDX_EUNSUPPORTED, "sweep staging does not fit in LDS (" + std::to_string(c->lds) + " B)");
   Arena ar;
   struct Off {
     size_t x, v, xbs, xhat, prev, mass, pinned, pinpos, brhs, ea, eb, rest, kappa, vstart, vedges, vedgesC,
-        bstart, bst, bw, sa, sb, sc, bk, vrank, rankv, dinv, W, ist, lst, dst;
+        bstart, bst, bw, sa, sb, sc, bk, vrank, rankv, dinv, W, ist, lst, dst, recE, restE, kapE, recC,
+        restC, kapC, brA, brB, brC, brK;
     size_t f[9], b[9];
-    size_t bf[13], bb[13];
+    size_t bf[14], bb[14];
   };
   std::vector<Off> offs(c->inst.size());
   c->bytesIter = 0;
@@ -490,6 +533,42 @@ int upload(pdx_ctx* c) {
     std::vector<int> vc(I.topo.vedgesColor.begin(), I.topo.vedgesColor.end());
     std::vector<int> bs(I.topo.bstart.begin(), I.topo.bstart.end());
     std::vector<int> bst(I.topo.bstencil.begin(), I.topo.bstencil.end());
+    {
+      // flattened incidence / bend records (see InstDev)
+      std::vector<int> recE(ve.size()), recC(ve.size());
+      std::vector<double> restE(ve.size()), kapE(ve.size()), restC(ve.size()), kapC(ve.size());
+      for (int v = 0; v < s.n; ++v)
+        for (int p = vs[(size_t)v]; p < vs[(size_t)v + 1]; ++p)
+          for (int q = 0; q < 2; ++q) {
+            const int e = q == 0 ? ve[(size_t)p] : vc[(size_t)p];
+            const bool isB = s.eb[(size_t)e] == v;
+            const int u = isB ? s.ea[(size_t)e] : s.eb[(size_t)e];
+            const int r = (u << 2) | (s.pinned[(size_t)u] ? 2 : 0) | (isB ? 1 : 0);
+            (q == 0 ? recE : recC)[(size_t)p] = r;
+            (q == 0 ? restE : restC)[(size_t)p] = s.rest[(size_t)e];
+            (q == 0 ? kapE : kapC)[(size_t)p] = s.kappa[(size_t)e];
+          }
+      std::vector<int> bA(bst.size()), bB(bst.size()), bC(bst.size());
+      std::vector<double> bK(bst.size());
+      for (size_t p = 0; p < bst.size(); ++p) {
+        const int q = bst[p];
+        auto enc = [&](int x) { return (x << 1) | (s.pinned[(size_t)x] ? 1 : 0); };
+        bA[p] = enc(s.sa[(size_t)q]);
+        bB[p] = enc(s.sb[(size_t)q]);
+        bC[p] = enc(s.sc[(size_t)q]);
+        bK[p] = s.bk[(size_t)q];
+      }
+      o.recE = ar.vec(recE);
+      o.restE = ar.vec(restE);
+      o.kapE = ar.vec(kapE);
+      o.recC = ar.vec(recC);
+      o.restC = ar.vec(restC);
+      o.kapC = ar.vec(kapC);
+      o.brA = ar.vec(bA);
+      o.brB = ar.vec(bB);
+      o.brC = ar.vec(bC);
+      o.brK = ar.vec(bK);
+    }
     o.vstart = ar.vec(vs);
     o.vedges = ar.vec(ve);
     o.vedgesC = ar.vec(vc);
@@ -521,20 +600,27 @@ int upload(pdx_ctx* c) {
     for (int q = 0; q < 2; ++q) {
       HostBlockSweep& h = *bsw[q];
       bo[q][0] = ar.vec(h.stepTask);
-      bo[q][1] = ar.vec(h.taskInfo);
-      bo[q][2] = ar.vec(h.taskMaxLen);
-      bo[q][3] = ar.vec(h.taskRowBase);
-      bo[q][4] = ar.vec(h.taskEntBase);
-      bo[q][5] = ar.vec(h.taskDenseBase);
+      bo[q][1] = ar.vec(h.stepRun);
+      bo[q][2] = ar.vec(h.stepDense);
+      bo[q][3] = ar.vec(h.taskInfo);
+      bo[q][4] = ar.vec(h.taskSlot);
+      bo[q][5] = ar.vec(h.taskDense);
       bo[q][6] = ar.vec(h.slotRank);
-      bo[q][7] = ar.vec(h.slotLen);
-      bo[q][8] = ar.vec(h.col);
-      bo[q][9] = fp32 ? ar.vec(castVec<float>(h.val)) : ar.vec(h.val);
-      bo[q][10] = fp32 ? ar.vec(castVec<float>(h.dense)) : ar.vec(h.dense);
+      bo[q][7] = ar.vec(h.slotRunBeg);
+      bo[q][8] = ar.vec(h.slotRunEnd);
+      bo[q][9] = ar.vec(h.runCol);
+      bo[q][10] = fp32 ? ar.vec(castVec<float>(h.runVal)) : ar.vec(h.runVal);
+      bo[q][11] = fp32 ? ar.vec(castVec<float>(h.dense)) : ar.vec(h.dense);
+      bo[q][12] = ar.vec(h.laneMeta);
+      bo[q][13] = fp32 ? ar.vec(castVec<float>(h.laneDinv)) : ar.vec(h.laneDinv);
+      if (c->idx16) {
+        std::vector<uint16_t> c16(h.runCol.begin(), h.runCol.end());
+        bo[q][9] = ar.vec(c16);
+      }
     }
     o.W = c->ldsW ? 0 : ar.add(nullptr, (size_t)3 * I.fac.m * w);
     std::vector<int> ist(4, 0);
-    std::vector<long long> lst(2, 0);
+    std::vector<long long> lst(16, 0);
     std::vector<double> dst(2, -1.0);
     o.ist = ar.vec(ist);
     o.lst = ar.vec(lst);
@@ -610,6 +696,16 @@ int upload(pdx_ctx* c) {
     d.vstart = (const int*)(B + o.vstart);
     d.vedges = (const int*)(B + o.vedges);
     d.vedgesColor = (const int*)(B + o.vedgesC);
+    d.recE = (const int*)(B + o.recE);
+    d.restE = (const double*)(B + o.restE);
+    d.kapE = (const double*)(B + o.kapE);
+    d.recC = (const int*)(B + o.recC);
+    d.restC = (const double*)(B + o.restC);
+    d.kapC = (const double*)(B + o.kapC);
+    d.brA = (const int*)(B + o.brA);
+    d.brB = (const int*)(B + o.brB);
+    d.brC = (const int*)(B + o.brC);
+    d.brK = (const double*)(B + o.brK);
     d.bstart = (const int*)(B + o.bstart);
     d.bstencil = (const int*)(B + o.bst);
     d.bweight = (const int8_t*)(B + o.bw);
@@ -641,17 +737,23 @@ int upload(pdx_ctx* c) {
       bd[q]->nstep = bsw[q]->nstep;
       bd[q]->cap = c->chainCap;
       bd[q]->stepTask = (const int*)(B + bo[q][0]);
-      bd[q]->taskInfo = (const int*)(B + bo[q][1]);
-      bd[q]->taskMaxLen = (const int*)(B + bo[q][2]);
-      bd[q]->taskRowBase = (const int*)(B + bo[q][3]);
-      bd[q]->taskEntBase = (const int*)(B + bo[q][4]);
-      bd[q]->taskDenseBase = (const int*)(B + bo[q][5]);
+      bd[q]->stepRun = (const int*)(B + bo[q][1]);
+      bd[q]->stepDense = (const int*)(B + bo[q][2]);
+      bd[q]->taskInfo = (const int*)(B + bo[q][3]);
+      bd[q]->taskSlot = (const int*)(B + bo[q][4]);
+      bd[q]->taskDense = (const int*)(B + bo[q][5]);
       bd[q]->slotRank = (const int*)(B + bo[q][6]);
-      bd[q]->slotLen = (const int*)(B + bo[q][7]);
-      bd[q]->col = (const int*)(B + bo[q][8]);
-      bd[q]->val = B + bo[q][9];
-      bd[q]->dense = B + bo[q][10];
+      bd[q]->slotRunBeg = (const int*)(B + bo[q][7]);
+      bd[q]->slotRunEnd = (const int*)(B + bo[q][8]);
+      bd[q]->runCol = (const void*)(B + bo[q][9]);
+      bd[q]->runVal = B + bo[q][10];
+      bd[q]->dense = B + bo[q][11];
+      bd[q]->laneMeta = (const int*)(B + bo[q][12]);
+      bd[q]->laneDinv = B + bo[q][13];
     }
+    d.partOff = (int)(kRedBytes + (c->ldsW ? (size_t)3 * I.fac.m * w : 0));
+    d.denseOff = d.partOff + (int)(partBytes(I, w));
+    d.stepOff = d.denseOff + (int)(denseBytes(I, w));
     d.Wg = c->ldsW ? nullptr : (void*)(B + o.W);
     d.istats = (int*)(B + o.ist);
     d.lstats = (long long*)(B + o.lst);
@@ -704,9 +806,9 @@ int ensureReady(pdx_ctx* c) {
   if (c->uploaded) {
     for (auto& I : c->inst) {
       std::vector<int> a(4);
-      std::vector<long long> b(2);
+      std::vector<long long> b(16);
       HIPCHECK(c, hipMemcpy(a.data(), I.distats, 16, hipMemcpyDeviceToHost));
-      HIPCHECK(c, hipMemcpy(b.data(), I.dlstats, 16, hipMemcpyDeviceToHost));
+      HIPCHECK(c, hipMemcpy(b.data(), I.dlstats, 128, hipMemcpyDeviceToHost));
       ist.push_back(a);
       lst.push_back(b);
     }
@@ -715,7 +817,7 @@ int ensureReady(pdx_ctx* c) {
   if (rc) return rc;
   for (size_t i = 0; i < ist.size(); ++i) {
     HIPCHECK(c, hipMemcpy(c->inst[i].distats, ist[i].data(), 16, hipMemcpyHostToDevice));
-    HIPCHECK(c, hipMemcpy(c->inst[i].dlstats, lst[i].data(), 16, hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(c->inst[i].dlstats, lst[i].data(), 128, hipMemcpyHostToDevice));
   }
   return PDX_OK;
 }
@@ -724,7 +826,7 @@ int launch(pdx_ctx* c, int nsub) {
   HIPCHECK(c, hipSetDevice(c->device));
   HIPCHECK(c, hipEventRecord(c->ev0, c->stream));
   HIPCHECK(c, launchSubstep(c->dInst, c->dOrder, (int)c->inst.size(), nsub, c->block, c->lds,
-                            c->prec == PDX_PREC_MIXED32, c->mode == PDX_SOLVE_FAST, c->stream));
+                            c->prec == PDX_PREC_MIXED32, c->mode == PDX_SOLVE_FAST, c->idx16, c->stream));
   HIPCHECK(c, hipEventRecord(c->ev1, c->stream));
   c->lastLaunches = 1;
   return PDX_OK;
@@ -842,7 +944,7 @@ int pdx_step_graph(pdx_ctx* c, int nsub, int replays) {
   hipGraphExec_t ge = nullptr;
   HIPCHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   const hipError_t le = launchSubstep(c->dInst, c->dOrder, (int)c->inst.size(), nsub, c->block, c->lds,
-                                      c->prec == PDX_PREC_MIXED32, c->mode == PDX_SOLVE_FAST, c->stream);
+                                      c->prec == PDX_PREC_MIXED32, c->mode == PDX_SOLVE_FAST, c->idx16, c->stream);
   HIPCHECK(c, hipStreamEndCapture(c->stream, &g));
   if (le != hipSuccess) {
     hipGraphDestroy(g);
@@ -937,11 +1039,15 @@ int pdx_get_stats(pdx_ctx* c, int i, pdx_stats* s) {
   s->last_residual = -1.0;
   if (c->uploaded) {
     int ist[4];
-    long long lst[2];
+    long long lst[16];
     double dst[2];
     HIPCHECK(c, hipStreamSynchronize(c->stream));
     HIPCHECK(c, hipMemcpy(ist, I.distats, 16, hipMemcpyDeviceToHost));
-    HIPCHECK(c, hipMemcpy(lst, I.dlstats, 16, hipMemcpyDeviceToHost));
+    HIPCHECK(c, hipMemcpy(lst, I.dlstats, 128, hipMemcpyDeviceToHost));
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->device) != hipSuccess || khz <= 0)
+      khz = 100000;
+    for (int q = 0; q < 9; ++q) s->phase_seconds[q] = (double)lst[2 + q] / (khz * 1e3);
     HIPCHECK(c, hipMemcpy(dst, I.ddstats, 16, hipMemcpyDeviceToHost));
     s->iterations_used = ist[0];
     s->converged = ist[1];
@@ -968,7 +1074,7 @@ int pdx_solve3(pdx_ctx* c, int i, const double* b, double* x) {
   }
   HIPCHECK(c, hipMemcpyAsync(c->dScratchB, b, (size_t)n3 * 8, hipMemcpyHostToDevice, c->stream));
   HIPCHECK(c, launchSolve(c->dInst, i, c->dScratchB, c->dScratchX, c->block, c->lds,
-                          c->prec == PDX_PREC_MIXED32, c->mode == PDX_SOLVE_FAST, c->stream));
+                          c->prec == PDX_PREC_MIXED32, c->mode == PDX_SOLVE_FAST, c->idx16, c->stream));
   HIPCHECK(c, hipMemcpyAsync(x, c->dScratchX, (size_t)n3 * 8, hipMemcpyDeviceToHost, c->stream));
   HIPCHECK(c, hipStreamSynchronize(c->stream));
   return PDX_OK;
@@ -992,6 +1098,22 @@ int pdx_get_factor(pdx_ctx* c, int i, int* m, long long* nnz, int32_t* ptr, int3
   if (dinv) std::copy(f.dinv.begin(), f.dinv.end(), dinv);
   if (rank_vertex) std::copy(f.rankVertex[0].begin(), f.rankVertex[0].end(), rank_vertex);
   if (perm3) std::copy(f.perm3.begin(), f.perm3.end(), perm3);
+  return PDX_OK;
+}
+
+int pdx_set_schedule(pdx_ctx* c, int block_threads, int chain_cap) {
+  if (!c || block_threads < 64 || block_threads > 512 || block_threads % 64 != 0 || chain_cap < 1 ||
+      chain_cap > 32)
+    return PDX_EINVAL;
+  if (c->chainCap != chain_cap) {
+    const int rc = pullState(c);
+    if (rc) return rc;
+    c->chainCap = chain_cap;
+    for (auto& I : c->inst) I.factored = false;  // repack (refactor of the same matrix)
+    for (auto& I : c->inst) I.factorizeCount -= 1;
+    c->dirty = true;
+  }
+  c->block = block_threads;
   return PDX_OK;
 }
 

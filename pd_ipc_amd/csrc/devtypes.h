@@ -24,33 +24,40 @@ struct SweepDev {
 };
 
 // Block-scheduled sweep (fast mode).  The ranks are cut into contiguous blocks:
-// elimination-tree chains of at most `cap` rows, and single rows.  Step st owns
-// tasks [stepTask[st], stepTask[st+1]); a task is one wavefront's work:
-//  * a row batch (<= 64 single-row blocks, one per lane, gather form), or
-//  * a chain block (lane t = rank start+t): the external part of every row is
-//    gathered in parallel, then the block's own dense lower triangle is applied
-//    column by column, each finished value broadcast to the other lanes.
-// External entries are ELL: entry j of lane t at entBase + j*nrows + t.
+// elimination-tree chains of at most `cap` rows, and single rows; each block
+// level is one step.  A step streams the external factor entries of all its rows
+// as runs of kRun entries (run r of the sweep: runCol/runVal[r*kRun ...]), one
+// partial sum per run, then its tasks ([stepTask[st], stepTask[st+1])) finish:
+//  * a row batch (<= 64 single-row blocks, one per lane), or
+//  * a chain block (lane t = rank start+t): after its external partials, the
+//    block's own dense lower triangle is applied column by column, each finished
+//    value broadcast to the other lanes.
 struct BlockSweepDev {
   int nstep;
   int cap;
-  const int* stepTask;
-  const int* taskInfo;      // nrows | (chain << 16)
-  const int* taskMaxLen;
-  const int* taskRowBase;
-  const int* taskEntBase;
-  const int* taskDenseBase;
+  const int* stepTask;    // nstep+1
+  const int* stepRun;     // nstep+1, global run index
+  const int* stepDense;   // nstep+1, offsets into dense
+  const int* taskInfo;    // nrows | (chain << 16)
+  const int* taskSlot;    // first slot of the task
+  const int* taskDense;   // offset of the chain's block inside its step's dense range
   const int* slotRank;
-  const int* slotLen;
-  const int* col;
-  const void* val;
-  const void* dense;        // chain blocks: cap x cap, [s * nrows + t] = L(start+t, start+s) (forward)
-                            // or [r * nrows + t] = L(start+r, start+t) (backward)
+  const int* slotRunBeg;  // run range of the slot, relative to its step
+  const int* slotRunEnd;
+  const void* runCol;     // int32, or uint16 when every rank fits
+  const void* runVal;
+  const int* laneMeta;    // per task, 64 lanes x {rank or -1, run begin, run end,
+                          //   nrows | chain << 7 | dense offset << 8}
+  const void* laneDinv;   // per task, 64 lanes x 1/D of the lane's rank (Real)
+  const void* dense;      // forward: [s * nrows + t] = L(start+t, start+s)
+                          // backward: [r * nrows + t] = L(start+r, start+t)
 };
 
 struct InstDev {
   int n, E, B, m;
   int maxIter, wantResidual, hasBendRhs, ldsW;
+  int partOff, denseOff;  // byte offsets of the run partials and dense staging in LDS
+  int stepOff, pad2;      // byte offset of the step tables of both sweeps in LDS
   double h, invH, invH2, dampInvH;
   double gh2[3];
   double gScaleH2;   // length(g) * h * h
@@ -73,6 +80,20 @@ struct InstDev {
   const int* vstart;
   const int* vedges;       // incidence, edge order (residual)
   const int* vedgesColor;  // incidence, colour order (scatter)
+  // Flattened incidence records (one per vertex-edge slot), so that the gather
+  // has one dependent load (the neighbour's position) per edge:
+  //   rec = (u << 2) | (u pinned << 1) | (v is the edge's b end)
+  const int* recE;         // edge order
+  const double* restE;
+  const double* kapE;
+  const int* recC;         // colour order
+  const double* restC;
+  const double* kapC;
+  // bend records per vertex-stencil slot: vertex = (idx << 1) | pinned
+  const int* brA;
+  const int* brB;
+  const int* brC;
+  const double* brK;
   const int* bstart;
   const int* bstencil;
   const int8_t* bweight;
@@ -89,7 +110,8 @@ struct InstDev {
   void* Wg;           // global W scratch (3m Real) when W is not LDS-resident
   // statistics
   int* istats;        // [0] iterations used, [1] converged, [2] early exits
-  long long* lstats;  // [0] total iterations, [1] substeps
+  long long* lstats;  // [0] total iterations, [1] substeps, [2..6] phase ticks
+                      // (gather, forward, backward, predict+velocity, criterion)
   double* dstats;     // [0] last residual
 };
 

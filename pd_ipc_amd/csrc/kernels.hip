@@ -48,6 +48,21 @@ __device__ inline double blockMax(double v, double* red) {
 }
 
 template <typename Real>
+__device__ inline Real* ldsAt(int off) {
+  extern __shared__ double smem[];
+  return reinterpret_cast<Real*>(reinterpret_cast<char*>(smem) + off);
+}
+
+// Barrier that orders LDS only.  __syncthreads() also waits for every
+// outstanding global access of the calling wave (loads and stores share one
+// counter), which would drain the factor prefetch issued just before it.
+__device__ inline void ldsBarrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+template <typename Real>
 __device__ inline Real subMul(Real acc, Real l, Real w);
 template <>
 __device__ inline double subMul<double>(double acc, double l, double w) {
@@ -126,74 +141,219 @@ __device__ inline float readlaneT(float v, int l) {
 }
 
 constexpr int kCap = 32;
+constexpr int kRun = 16;
 
-// Block-scheduled sweep: per step, each wavefront takes whole tasks (a chain block
-// or a batch of single rows).  Chain blocks keep their dense triangle in
-// registers (lane t holds row/column t) and broadcast each finished value with
-// readlane, so a chain of c rows costs c short ALU steps instead of c barriers.
-// Forward rows keep the reference's ascending column order (external columns
-// precede the block's own); backward rows apply the block's own rows last.
-template <typename Real, bool kFwd, typename Store>
-__device__ void blockSweep(const BlockSweepDev& sw, Real* W, const Real* dinv, Store store) {
-  const int lane = threadIdx.x & (kWave - 1);
-  const int wave = threadIdx.x / kWave;
-  const int nw = blockDim.x / kWave;
-  const Real* val = static_cast<const Real*>(sw.val);
-  const Real* dense = static_cast<const Real*>(sw.dense);
-  for (int st = 0; st < sw.nstep; ++st) {
-    const int t1 = sw.stepTask[st + 1];
-    for (int t = sw.stepTask[st] + wave; t < t1; t += nw) {
-      const int info = sw.taskInfo[t];
-      const int nrows = info & 0xffff;
-      const bool chain = (info >> 16) != 0;
-      const int maxlen = sw.taskMaxLen[t];
-      const int rbase = sw.taskRowBase[t];
-      const int ebase = sw.taskEntBase[t];
-      const bool act = lane < nrows;
-      int k = 0, len = 0;
-      if (act) {
-        k = sw.slotRank[rbase + lane];
-        len = sw.slotLen[rbase + lane];
-      }
-      Real dreg[kCap];
-      if (chain) {
-        const Real* D = dense + sw.taskDenseBase[t];
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef double v2d __attribute__((ext_vector_type(2)));
+
+// One run of kRun factor entries as loaded: columns packed (16-bit: 2 x 16 B,
+// 32-bit: 4 x 16 B), values FP32 (4 x 16 B) or FP64 (8 x 16 B); every load is
+// independent, so a thread keeps several runs in flight.
+template <typename Real, bool kIdx16>
+struct RunRaw {
+  v4i c[kIdx16 ? 2 : 4];
+  Real v[kRun];
+};
+
+// Global-address-space views: loads through them compile to global_load, which
+// only the vector-memory counter tracks, so an LDS barrier does not drain them
+// (flat loads also count on the LDS counter).
+#define PDX_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ inline const PDX_GLOBAL T* gview(const void* p) {
+  return (const PDX_GLOBAL T*)(p);
+}
+
+template <typename Real, bool kIdx16>
+__device__ inline void loadRun(const void* colv, const Real* val, int r, RunRaw<Real, kIdx16>& b) {
+  const PDX_GLOBAL v4i* c4 = kIdx16 ? gview<v4i>(static_cast<const uint16_t*>(colv) + (size_t)r * kRun)
+                                    : gview<v4i>(static_cast<const int*>(colv) + (size_t)r * kRun);
 #pragma unroll
-        for (int q = 0; q < kCap; ++q) dreg[q] = (act && q < nrows) ? D[q * nrows + lane] : Real(0);
+  for (int q = 0; q < (kIdx16 ? 2 : 4); ++q) b.c[q] = c4[q];
+  if (sizeof(Real) == 4) {
+    const PDX_GLOBAL v4f* v4 = gview<v4f>(val + (size_t)r * kRun);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const v4f t = v4[q];
+      b.v[4 * q] = t.x;
+      b.v[4 * q + 1] = t.y;
+      b.v[4 * q + 2] = t.z;
+      b.v[4 * q + 3] = t.w;
+    }
+  } else {
+    const PDX_GLOBAL v2d* v2 = gview<v2d>(val + (size_t)r * kRun);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const v2d t = v2[q];
+      b.v[2 * q] = t.x;
+      b.v[2 * q + 1] = t.y;
+    }
+  }
+}
+
+template <typename Real, bool kIdx16>
+__device__ inline void runPartial(const RunRaw<Real, kIdx16>& b, const Real* W, Real* P) {
+  Real p0 = 0, p1 = 0, p2 = 0;
+#pragma unroll
+  for (int q = 0; q < kRun; ++q) {
+    int c;
+    if (kIdx16) {
+      const int w = b.c[q >> 3][(q >> 1) & 3];
+      c = (q & 1) ? ((unsigned)w >> 16) : (w & 0xffff);
+    } else {
+      c = b.c[q >> 2][q & 3];
+    }
+    const Real l = b.v[q];
+    p0 = p0 + l * W[3 * c + 0];
+    p1 = p1 + l * W[3 * c + 1];
+    p2 = p2 + l * W[3 * c + 2];
+  }
+  P[0] = p0;
+  P[1] = p1;
+  P[2] = p2;
+}
+
+constexpr int kPreRuns = 2;  // runs per thread loaded during the previous step
+
+// Block-scheduled sweep.  Step st: (1) the workgroup turns the step's runs into
+// partial sums in LDS (each thread's first kPreRuns runs, the first two tasks'
+// lane metadata of its wavefront and the step's dense blocks were loaded during
+// the previous step) and copies the dense chain blocks into LDS; (2) each
+// wavefront takes whole tasks: a row subtracts its partials in run order, a chain
+// block then applies its dense triangle column by column, broadcasting each
+// finished value with readlane.  Two barriers per step; fixed order, no float
+// atomics.  `steps` is the step table in LDS: run, dense and task offsets.
+struct SweepTimes {
+  long long stream = 0, wait1 = 0, fin = 0, wait2 = 0;
+};
+
+template <typename Real, bool kFwd, bool kIdx16, typename Store>
+__device__ void blockSweep(const BlockSweepDev& sw, Real* W, Real* P, Real* D, const int* steps,
+                           const Real* dinvUnused, Store store, SweepTimes& tstream, bool wInLds) {
+  (void)dinvUnused;
+  auto barrier = [wInLds]() {
+    if (wInLds)
+      ldsBarrier();
+    else
+      __syncthreads();
+  };
+  const int tid = threadIdx.x, bs = blockDim.x;
+  const int lane = tid & (kWave - 1);
+  const int wave = tid / kWave;
+  const int nw = bs / kWave;
+  const int ns = sw.nstep;
+  if (ns <= 0) return;
+  const int* sRun = steps;
+  const int* sDense = steps + (ns + 1);
+  const int* sTask = steps + 2 * (ns + 1);
+  const Real* rval = static_cast<const Real*>(sw.runVal);
+  const Real* dense = static_cast<const Real*>(sw.dense);
+  const v4i* meta = reinterpret_cast<const v4i*>(sw.laneMeta);
+  const Real* ldinv = static_cast<const Real*>(sw.laneDinv);
+  RunRaw<Real, kIdx16> pre[kPreRuns];
+  bool havePre[kPreRuns];
+  Real dpre[4] = {0, 0, 0, 0};
+  bool haveD = false;
+  v4i mpre[2];
+  Real dvpre[2] = {0, 0};
+  bool haveM[2] = {false, false};
+  // Every thread issues its loads unconditionally (indices clamped to valid
+  // entries) so that no branch or register copy waits for them before use.
+  const int lastRun = max(sRun[ns] - 1, 0);
+  const int lastDense = max(sDense[ns] - 4, 0);
+  const int lastTask = max(sTask[ns] - 1, 0);
+  const PDX_GLOBAL Real* gdense = gview<Real>(dense);
+  const PDX_GLOBAL v4i* gmeta = gview<v4i>(meta);
+  const PDX_GLOBAL Real* gldinv = gview<Real>(ldinv);
+  auto prefetch = [&](int st) {
+    const int rb = sRun[st], re = sRun[st + 1];
+#pragma unroll
+    for (int q = 0; q < kPreRuns; ++q) {
+      const int r = rb + tid + q * bs;
+      havePre[q] = r < re;
+      loadRun<Real, kIdx16>(sw.runCol, rval, min(r, lastRun), pre[q]);
+    }
+    const int d = sDense[st] + 4 * tid;
+    haveD = d < sDense[st + 1];
+    const int dc = min(d, lastDense);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) dpre[q] = gdense[dc + q];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int t = sTask[st] + wave + q * nw;
+      haveM[q] = t < sTask[st + 1];
+      const size_t ti = (size_t)min(t, lastTask) * kWave + lane;
+      mpre[q] = gmeta[ti];
+      if (!kFwd) dvpre[q] = gldinv[ti];
+    }
+  };
+  prefetch(0);
+  for (int st = 0; st < ns; ++st) {
+    const long long ts0 = wall_clock64();
+    const int base = sRun[st], end = sRun[st + 1];
+    const int dbase = sDense[st], dend = sDense[st + 1];
+    const int tb = sTask[st], te = sTask[st + 1];
+#pragma unroll
+    for (int q = 0; q < kPreRuns; ++q)
+      if (havePre[q]) runPartial<Real, kIdx16>(pre[q], W, P + 3 * (tid + q * bs));
+    for (int r = base + tid + kPreRuns * bs; r < end; r += bs) {
+      RunRaw<Real, kIdx16> tmp;
+      loadRun<Real, kIdx16>(sw.runCol, rval, r, tmp);
+      runPartial<Real, kIdx16>(tmp, W, P + 3 * (r - base));
+    }
+    if (haveD)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) D[4 * tid + q] = dpre[q];
+    for (int d = dbase + 4 * (tid + bs); d < dend; d += 4 * bs)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) D[d - dbase + q] = gdense[d + q];
+    const v4i m0 = mpre[0], m1 = mpre[1];
+    const Real dv0 = dvpre[0], dv1 = dvpre[1];
+    if (st + 1 < ns) prefetch(st + 1);
+    const long long ts1 = wall_clock64();
+    barrier();
+    const long long ts2 = wall_clock64();
+    int slot = 0;
+    for (int t = tb + wave; t < te; t += nw, ++slot) {
+      v4i mt;
+      Real dv = 0;
+      if (slot == 0) {
+        mt = m0;
+        dv = dv0;
+      } else if (slot == 1) {
+        mt = m1;
+        dv = dv1;
+      } else {
+        mt = gmeta[(size_t)t * kWave + lane];
+        if (!kFwd) dv = gldinv[(size_t)t * kWave + lane];
       }
+      const int k = mt.x;
+      const int info = mt.w;
+      const int nrows = info & 0x7f;
+      const bool chain = (info & 0x80) != 0;
+      const bool act = k >= 0;
       Real a0 = 0, a1 = 0, a2 = 0;
       if (act) {
         a0 = W[3 * k + 0];
         a1 = W[3 * k + 1];
         a2 = W[3 * k + 2];
         if (!kFwd) {
-          const Real dv = dinv[k];
           a0 = a0 * dv;
           a1 = a1 * dv;
           a2 = a2 * dv;
         }
-      }
-      for (int j = 0; j < maxlen; j += 4) {
-        int cc[4];
-        Real lv[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const bool ok = act && (j + q < maxlen);
-          const int e = ebase + (j + q) * nrows + lane;
-          cc[q] = ok ? sw.col[e] : 0;
-          lv[q] = ok ? val[e] : Real(0);
-        }
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          if (j + q < len) {
-            const int c = cc[q];
-            a0 = subMul<Real>(a0, lv[q], W[3 * c + 0]);
-            a1 = subMul<Real>(a1, lv[q], W[3 * c + 1]);
-            a2 = subMul<Real>(a2, lv[q], W[3 * c + 2]);
-          }
+        for (int q = mt.y; q < mt.z; ++q) {
+          a0 = a0 - P[3 * q + 0];
+          a1 = a1 - P[3 * q + 1];
+          a2 = a2 - P[3 * q + 2];
         }
       }
       if (chain) {
+        const Real* Dt = D + (info >> 8);
+        Real dreg[kCap];
+#pragma unroll
+        for (int q = 0; q < kCap; ++q) dreg[q] = (act && q < nrows) ? Dt[q * nrows + lane] : Real(0);
         if (kFwd) {
 #pragma unroll
           for (int q = 0; q < kCap; ++q) {
@@ -227,7 +387,29 @@ __device__ void blockSweep(const BlockSweepDev& sw, Real* W, const Real* dinv, S
         store(k, a0, a1, a2);
       }
     }
-    __syncthreads();
+    const long long ts3 = wall_clock64();
+    barrier();
+    if (tid == 0) {
+      tstream.stream += ts1 - ts0;
+      tstream.wait1 += ts2 - ts1;
+      tstream.fin += ts3 - ts2;
+      tstream.wait2 += wall_clock64() - ts3;
+    }
+  }
+}
+
+// Copies the step tables (run, dense and task offsets) of both sweeps into LDS.
+__device__ inline void loadStepTables(const InstDev& I) {
+  int* dst = ldsAt<int>(I.stepOff);
+  const BlockSweepDev* sws[2] = {&I.bfwd, &I.bbwd};
+  for (int q = 0; q < 2; ++q) {
+    const int n1 = sws[q]->nstep + 1;
+    for (int i = threadIdx.x; i < n1; i += blockDim.x) {
+      dst[i] = sws[q]->stepRun[i];
+      dst[n1 + i] = sws[q]->stepDense[i];
+      dst[2 * n1 + i] = sws[q]->stepTask[i];
+    }
+    dst += 3 * n1;
   }
 }
 
@@ -252,117 +434,182 @@ __device__ inline Acc3 rhsExact(const InstDev& I, int v) {
     b.z = b.z + 0.0;
   }
   const double* x = I.x;
-  for (int p = I.vstart[v]; p < I.vstart[v + 1]; ++p) {
-    const int e = I.vedgesColor[p];
-    const int a = I.ea[e], bb = I.eb[e];
-    const bool aP = I.pinned[a] != 0, bP = I.pinned[bb] != 0;
-    if (aP && bP) continue;
-    const double rx = x[3 * a] - x[3 * bb], ry = x[3 * a + 1] - x[3 * bb + 1],
-                 rz = x[3 * a + 2] - x[3 * bb + 2];
+  const double xv0 = x[3 * v], xv1 = x[3 * v + 1], xv2 = x[3 * v + 2];
+  const int p1 = I.vstart[v + 1];
+#pragma unroll 4
+  for (int p = I.vstart[v]; p < p1; ++p) {
+    const int rc = I.recC[p];
+    const double rest = I.restC[p], k = I.kapC[p];
+    const int u = rc >> 2;
+    const bool uP = (rc & 2) != 0, isB = (rc & 1) != 0;
+    const double u0 = x[3 * u], u1 = x[3 * u + 1], u2 = x[3 * u + 2];
+    const double rx = isB ? u0 - xv0 : xv0 - u0;
+    const double ry = isB ? u1 - xv1 : xv1 - u1;
+    const double rz = isB ? u2 - xv2 : xv2 - u2;
     const double len = sqrt(rx * rx + ry * ry + rz * rz);
     double dx = 0.0, dy = 0.0, dz = 0.0;
     if (!(len <= kMinLength)) {
-      const double s = I.rest[e] / len;
-      dx = rx * s;
-      dy = ry * s;
-      dz = rz * s;
+      const double sc = rest / len;
+      dx = rx * sc;
+      dy = ry * sc;
+      dz = rz * sc;
     }
-    const double k = I.kappa[e];
     const double cx = dx * k, cy = dy * k, cz = dz * k;
-    if (a == v) {
+    if (!isB) {
       b.x += cx;
       b.y += cy;
       b.z += cz;
-      if (bP) {
-        b.x += k * I.pinpos[3 * bb];
-        b.y += k * I.pinpos[3 * bb + 1];
-        b.z += k * I.pinpos[3 * bb + 2];
-      }
     } else {
       b.x -= cx;
       b.y -= cy;
       b.z -= cz;
-      if (aP) {
-        b.x += k * I.pinpos[3 * a];
-        b.y += k * I.pinpos[3 * a + 1];
-        b.z += k * I.pinpos[3 * a + 2];
-      }
+    }
+    if (uP) {
+      b.x += k * I.pinpos[3 * u];
+      b.y += k * I.pinpos[3 * u + 1];
+      b.z += k * I.pinpos[3 * u + 2];
     }
   }
   return b;
 }
 
+constexpr int kMaxE = 4;  // incident edges gathered per group, all loads in flight
+constexpr int kMaxS = 2;  // incident bend stencils per group likewise
+
+// Per-edge term of the residual gather and of the update right-hand side.
+template <bool kRhs>
+__device__ inline void edgeTerm(const InstDev& I, int rc, double rest, double k, double u0, double u1,
+                                double u2, double xv0, double xv1, double xv2, Acc3& f, Acc3& r) {
+  const int u = rc >> 2;
+  const bool uP = (rc & 2) != 0, isB = (rc & 1) != 0;
+  const double rx = isB ? u0 - xv0 : xv0 - u0;
+  const double ry = isB ? u1 - xv1 : xv1 - u1;
+  const double rz = isB ? u2 - xv2 : xv2 - u2;
+  const double len = sqrt(rx * rx + ry * ry + rz * rz);
+  double dx = 0.0, dy = 0.0, dz = 0.0;
+  if (!(len <= kMinLength)) {
+    const double sc = rest / len;
+    dx = rx * sc;
+    dy = ry * sc;
+    dz = rz * sc;
+  }
+  const double cx = rx * k - dx * k, cy = ry * k - dy * k, cz = rz * k - dz * k;
+  if (!isB) {
+    f.x += cx;
+    f.y += cy;
+    f.z += cz;
+  } else {
+    f.x -= cx;
+    f.y -= cy;
+    f.z -= cz;
+  }
+  if (kRhs) {
+    const double sg = isB ? -1.0 : 1.0;
+    const double t0 = uP ? I.pinpos[3 * u] : u0;
+    const double t1 = uP ? I.pinpos[3 * u + 1] : u1;
+    const double t2 = uP ? I.pinpos[3 * u + 2] : u2;
+    r.x += sg * dx * k - k * (xv0 - t0);
+    r.y += sg * dy * k - k * (xv1 - t1);
+    r.z += sg * dz * k - k * (xv2 - t2);
+  }
+}
+
+template <bool kRhs>
+__device__ inline void bendTerm(const InstDev& I, int ea, int eb, int ec, double K, double w,
+                                const double* xa, const double* xb, const double* xc, Acc3& fb, Acc3& r) {
+  const double s0 = xa[0] - 2.0 * xb[0] + xc[0];
+  const double s1 = xa[1] - 2.0 * xb[1] + xc[1];
+  const double s2 = xa[2] - 2.0 * xb[2] + xc[2];
+  const double coef = -K * w;
+  fb.x += s0 * coef;
+  fb.y += s1 * coef;
+  fb.z += s2 * coef;
+  if (kRhs) {
+    const int vv[3] = {ea, eb, ec};
+    const double* xs[3] = {xa, xb, xc};
+    double t0 = 0.0, t1 = 0.0, t2 = 0.0;
+    const double wt[3] = {1.0, -2.0, 1.0};
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const int u = vv[q] >> 1;
+      const bool uP = (vv[q] & 1) != 0;
+      t0 += wt[q] * (uP ? I.pinpos[3 * u] : xs[q][0]);
+      t1 += wt[q] * (uP ? I.pinpos[3 * u + 1] : xs[q][1]);
+      t2 += wt[q] * (uP ? I.pinpos[3 * u + 2] : xs[q][2]);
+    }
+    r.x += coef * t0;
+    r.y += coef * t1;
+    r.z += coef * t2;
+  }
+}
+
 // Residual of one free vertex at x (Integrator.cpp:92-109 + :230-263), and, when
 // `rhs` is non-null, the update right-hand side b(x) - (L x) of the same vertex
 // (mixed form).  L x uses the pinned targets q for pinned columns, which is
-// what the assembled L and the pin compensation in b amount to.
-__device__ inline double residualAndRhs(const InstDev& I, int v, Acc3* rhs) {
+// what the assembled L and the pin compensation in b amount to.  Edges and
+// stencils are taken in groups of kMaxE / kMaxS whose loads are all issued
+// before any use; the summation order stays the incidence order.
+template <bool kRhs>
+__device__ inline double residualAndRhsT(const InstDev& I, int v, Acc3& rhsOut) {
   const double* x = I.x;
   const double xv0 = x[3 * v], xv1 = x[3 * v + 1], xv2 = x[3 * v + 2];
   Acc3 f{0.0, 0.0, 0.0};
   Acc3 r{0.0, 0.0, 0.0};
-  for (int p = I.vstart[v]; p < I.vstart[v + 1]; ++p) {
-    const int e = I.vedges[p];
-    const int a = I.ea[e], bb = I.eb[e];
-    const double rx = x[3 * a] - x[3 * bb], ry = x[3 * a + 1] - x[3 * bb + 1],
-                 rz = x[3 * a + 2] - x[3 * bb + 2];
-    const double len = sqrt(rx * rx + ry * ry + rz * rz);
-    double dx = 0.0, dy = 0.0, dz = 0.0;
-    if (!(len <= kMinLength)) {
-      const double s = I.rest[e] / len;
-      dx = rx * s;
-      dy = ry * s;
-      dz = rz * s;
-    }
-    const double k = I.kappa[e];
-    const double cx = rx * k - dx * k, cy = ry * k - dy * k, cz = rz * k - dz * k;
-    if (a == v) {
-      f.x += cx;
-      f.y += cy;
-      f.z += cz;
-    } else {
-      f.x -= cx;
-      f.y -= cy;
-      f.z -= cz;
-    }
-    if (rhs) {
-      const int u = (a == v) ? bb : a;
-      const double sg = (a == v) ? 1.0 : -1.0;
-      const bool uP = I.pinned[u] != 0;
-      const double u0 = uP ? I.pinpos[3 * u] : x[3 * u];
-      const double u1 = uP ? I.pinpos[3 * u + 1] : x[3 * u + 1];
-      const double u2 = uP ? I.pinpos[3 * u + 2] : x[3 * u + 2];
-      r.x += sg * dx * k - k * (xv0 - u0);
-      r.y += sg * dy * k - k * (xv1 - u1);
-      r.z += sg * dz * k - k * (xv2 - u2);
+  {
+    const int p0 = I.vstart[v], ne = I.vstart[v + 1] - p0;
+    for (int g = 0; g < ne; g += kMaxE) {
+      int rc[kMaxE];
+      double rest[kMaxE], kap[kMaxE], ux[kMaxE][3];
+#pragma unroll
+      for (int q = 0; q < kMaxE; ++q)
+        if (g + q < ne) {
+          rc[q] = I.recE[p0 + g + q];
+          rest[q] = I.restE[p0 + g + q];
+          kap[q] = I.kapE[p0 + g + q];
+        }
+#pragma unroll
+      for (int q = 0; q < kMaxE; ++q)
+        if (g + q < ne) {
+          const int u = rc[q] >> 2;
+          ux[q][0] = x[3 * u];
+          ux[q][1] = x[3 * u + 1];
+          ux[q][2] = x[3 * u + 2];
+        }
+#pragma unroll
+      for (int q = 0; q < kMaxE; ++q)
+        if (g + q < ne)
+          edgeTerm<kRhs>(I, rc[q], rest[q], kap[q], ux[q][0], ux[q][1], ux[q][2], xv0, xv1, xv2, f, r);
     }
   }
   Acc3 fb{0.0, 0.0, 0.0};
-  for (int p = I.bstart[v]; p < I.bstart[v + 1]; ++p) {
-    const int s = I.bstencil[p];
-    const int ia = I.sa[s], ib = I.sb[s], ic = I.sc[s];
-    const double s0 = x[3 * ia] - 2.0 * x[3 * ib] + x[3 * ic];
-    const double s1 = x[3 * ia + 1] - 2.0 * x[3 * ib + 1] + x[3 * ic + 1];
-    const double s2 = x[3 * ia + 2] - 2.0 * x[3 * ib + 2] + x[3 * ic + 2];
-    const double w = (double)I.bweight[p];
-    const double coef = -I.bk[s] * w;
-    fb.x += s0 * coef;
-    fb.y += s1 * coef;
-    fb.z += s2 * coef;
-    if (rhs) {
-      const int vv[3] = {ia, ib, ic};
-      double t0 = 0.0, t1 = 0.0, t2 = 0.0;
-      const double wt[3] = {1.0, -2.0, 1.0};
-      for (int q = 0; q < 3; ++q) {
-        const int u = vv[q];
-        const bool uP = I.pinned[u] != 0;
-        t0 += wt[q] * (uP ? I.pinpos[3 * u] : x[3 * u]);
-        t1 += wt[q] * (uP ? I.pinpos[3 * u + 1] : x[3 * u + 1]);
-        t2 += wt[q] * (uP ? I.pinpos[3 * u + 2] : x[3 * u + 2]);
-      }
-      r.x += coef * t0;
-      r.y += coef * t1;
-      r.z += coef * t2;
+  {
+    const int p0 = I.bstart[v], nb = I.bstart[v + 1] - p0;
+    for (int g = 0; g < nb; g += kMaxS) {
+      int ia[kMaxS], ib[kMaxS], ic[kMaxS];
+      double K[kMaxS], w[kMaxS], xa[kMaxS][3], xb[kMaxS][3], xc[kMaxS][3];
+#pragma unroll
+      for (int q = 0; q < kMaxS; ++q)
+        if (g + q < nb) {
+          ia[q] = I.brA[p0 + g + q];
+          ib[q] = I.brB[p0 + g + q];
+          ic[q] = I.brC[p0 + g + q];
+          K[q] = I.brK[p0 + g + q];
+          w[q] = (double)I.bweight[p0 + g + q];
+        }
+#pragma unroll
+      for (int q = 0; q < kMaxS; ++q)
+        if (g + q < nb) {
+          const int a = ia[q] >> 1, b = ib[q] >> 1, c = ic[q] >> 1;
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            xa[q][d] = x[3 * a + d];
+            xb[q][d] = x[3 * b + d];
+            xc[q][d] = x[3 * c + d];
+          }
+        }
+#pragma unroll
+      for (int q = 0; q < kMaxS; ++q)
+        if (g + q < nb) bendTerm<kRhs>(I, ia[q], ib[q], ic[q], K[q], w[q], xa[q], xb[q], xc[q], fb, r);
     }
   }
   f.x += fb.x;
@@ -372,13 +619,18 @@ __device__ inline double residualAndRhs(const InstDev& I, int v, Acc3* rhs) {
   const double mi = mass * I.invH2;
   const double i0 = (xv0 - I.xhat[3 * v]) * mi, i1 = (xv1 - I.xhat[3 * v + 1]) * mi,
                i2 = (xv2 - I.xhat[3 * v + 2]) * mi;
-  if (rhs) {
-    rhs->x = r.x - i0;
-    rhs->y = r.y - i1;
-    rhs->z = r.z - i2;
+  if (kRhs) {
+    rhsOut.x = r.x - i0;
+    rhsOut.y = r.y - i1;
+    rhsOut.z = r.z - i2;
   }
   const double t0 = i0 + f.x, t1 = i1 + f.y, t2 = i2 + f.z;
   return sqrt(t0 * t0 + t1 * t1 + t2 * t2) / mass;
+}
+
+__device__ inline double residualAndRhs(const InstDev& I, int v, Acc3* rhs) {
+  Acc3 dummy;
+  return rhs ? residualAndRhsT<true>(I, v, *rhs) : residualAndRhsT<false>(I, v, dummy);
 }
 
 template <typename Real>
@@ -407,42 +659,45 @@ __device__ void gatherFirst(const InstDev& I, Real* W) {
   }
 }
 
-template <typename Real, bool kExact, bool kBlocked>
-__device__ void runSweeps(const InstDev& I, Real* W) {
+
+template <typename Real, bool kBlocked, bool kIdx16>
+__device__ void runForward(const InstDev& I, Real* W, SweepTimes& tst) {
   const Real* dinv = static_cast<const Real*>(I.dinv);
   auto none = [](int, Real, Real, Real) {};
   if (kBlocked)
-    blockSweep<Real, true>(I.bfwd, W, dinv, none);
+    blockSweep<Real, true, kIdx16>(I.bfwd, W, ldsAt<Real>(I.partOff), ldsAt<Real>(I.denseOff), ldsAt<int>(I.stepOff), dinv, none, tst, I.ldsW != 0);
   else
     sweep<Real, true>(I.fwd, W, dinv, none);
+}
+
+template <typename Real, bool kExact, bool kBlocked, bool kIdx16>
+__device__ void runBackward(const InstDev& I, Real* W, SweepTimes& tst) {
+  const Real* dinv = static_cast<const Real*>(I.dinv);
   double* x = I.x;
   const int* rankv = I.rankv;
   const uint8_t* pinned = I.pinned;
   const double* pinpos = I.pinpos;
+  // exact form: the sweep's result is the new position, stored as it finishes;
+  // mixed form: the update is applied after the sweep (applyUpdate), off the
+  // sweep's critical path.
   auto put = [=](int k, Real a0, Real a1, Real a2) {
-    const int v = rankv[k];
     if (kExact) {
+      const int v = rankv[k];
       x[3 * v] = (double)a0;
       x[3 * v + 1] = (double)a1;
       x[3 * v + 2] = (double)a2;
-    } else if (pinned[v]) {
-      x[3 * v] = pinpos[3 * v];
-      x[3 * v + 1] = pinpos[3 * v + 1];
-      x[3 * v + 2] = pinpos[3 * v + 2];
-    } else {
-      x[3 * v] += (double)a0;
-      x[3 * v + 1] += (double)a1;
-      x[3 * v + 2] += (double)a2;
     }
   };
+  (void)pinned;
+  (void)pinpos;
   if (kBlocked)
-    blockSweep<Real, false>(I.bbwd, W, dinv, put);
+    blockSweep<Real, false, kIdx16>(I.bbwd, W, ldsAt<Real>(I.partOff), ldsAt<Real>(I.denseOff), ldsAt<int>(I.stepOff) + 3 * (I.bfwd.nstep + 1), dinv, put, tst, I.ldsW != 0);
   else
     sweep<Real, false>(I.bwd, W, dinv, put);
 }
 
-template <typename Real, bool kExact, bool kBlocked>
-__global__ void __launch_bounds__(1024) substepKernel(const InstDev* __restrict__ insts,
+template <typename Real, bool kExact, bool kBlocked, bool kIdx16>
+__global__ void __launch_bounds__(512) substepKernel(const InstDev* __restrict__ insts,
                                                       const int* __restrict__ order, int nsub) {
   extern __shared__ double smem[];
   const InstDev& I = insts[order[blockIdx.x]];
@@ -451,7 +706,12 @@ __global__ void __launch_bounds__(1024) substepKernel(const InstDev* __restrict_
   const int n = I.n;
   const int tid = threadIdx.x, bs = blockDim.x;
 
+  if (kBlocked) loadStepTables(I);  // visible after the first barrier below
+  long long tk[4] = {0, 0, 0, 0};
+  long long tacc[5] = {0, 0, 0, 0, 0};
+  SweepTimes swt;
   for (int sub = 0; sub < nsub; ++sub) {
+    const long long tp0 = wall_clock64();
     // predict (Integrator.cpp:447-463) and the displacement scale
     double lscale = 0.0;
     for (int v = tid; v < n; v += bs) {
@@ -466,13 +726,40 @@ __global__ void __launch_bounds__(1024) substepKernel(const InstDev* __restrict_
     }
     double scale = blockMax(lscale, red);
     scale = fmax(scale, I.gScaleH2);
+    const long long tp1 = wall_clock64();
     gatherFirst<Real, kExact>(I, W);
     __syncthreads();
+    if (tid == 0) {
+      tacc[3] += tp1 - tp0;
+      tacc[0] += wall_clock64() - tp1;
+    }
 
     int used = 0, conv = 0;
     double lastRes = -1.0;
     for (int k = 0; k < I.maxIter; ++k) {
-      runSweeps<Real, kExact, kBlocked>(I, W);
+      if (tid == 0) tk[0] = wall_clock64();
+      runForward<Real, kBlocked, kIdx16>(I, W, swt);
+      if (tid == 0) tk[1] = wall_clock64();
+      runBackward<Real, kExact, kBlocked, kIdx16>(I, W, swt);
+      if (kExact && kBlocked) __syncthreads();  // the sweep stored x; the gather reads it
+      if (!kExact) {
+        // x += delta (pinned: x = q) for every vertex, then a barrier so that the
+        // gather below sees every neighbour's new position
+        for (int v = tid; v < n; v += bs) {
+          const int r = I.vrank[v];
+          if (I.pinned[v]) {
+            I.x[3 * v] = I.pinpos[3 * v];
+            I.x[3 * v + 1] = I.pinpos[3 * v + 1];
+            I.x[3 * v + 2] = I.pinpos[3 * v + 2];
+          } else {
+            I.x[3 * v] += (double)W[3 * r];
+            I.x[3 * v + 1] += (double)W[3 * r + 1];
+            I.x[3 * v + 2] += (double)W[3 * r + 2];
+          }
+        }
+        __syncthreads();
+      }
+      if (tid == 0) tk[2] = wall_clock64();
       const bool more = (k + 1 < I.maxIter);
       double ldiff = 0.0, lres = 0.0;
       for (int v = tid; v < n; v += bs) {
@@ -505,10 +792,17 @@ __global__ void __launch_bounds__(1024) substepKernel(const InstDev* __restrict_
       // criterion (Integrator.cpp:673-751); two slot sets alternate so one barrier suffices
       double* rs = red + 16 * ((k & 1) ^ 1);
       const double diff = blockMax(ldiff, rs);
+      if (tid == 0) tk[3] = wall_clock64();
       double residual = -1.0;
       if (I.wantResidual) {
         __syncthreads();
         residual = blockMax(lres, rs);
+      }
+      if (tid == 0) {
+        tacc[1] += tk[1] - tk[0];
+        tacc[2] += tk[2] - tk[1];
+        tacc[0] += tk[3] - tk[2];
+        tacc[4] += wall_clock64() - tk[3];
       }
       const double rel = (scale > 0.0) ? diff / scale : diff;
       const bool absHit = (I.absTol > 0.0) && (diff <= I.absTol);
@@ -523,6 +817,7 @@ __global__ void __launch_bounds__(1024) substepKernel(const InstDev* __restrict_
       }
     }
     // velocity + damping + re-pin (Integrator.cpp:774-791)
+    const long long tv0 = wall_clock64();
     for (int v = tid; v < n; v += bs) {
       const bool pinnedV = I.pinned[v] != 0;
       for (int d = 0; d < 3; ++d) {
@@ -535,6 +830,7 @@ __global__ void __launch_bounds__(1024) substepKernel(const InstDev* __restrict_
       }
     }
     if (tid == 0) {
+      tacc[3] += wall_clock64() - tv0;
       I.istats[0] = used;
       I.istats[1] = conv;
       I.istats[2] += conv;
@@ -544,25 +840,34 @@ __global__ void __launch_bounds__(1024) substepKernel(const InstDev* __restrict_
     }
     __syncthreads();
   }
+  if (tid == 0) {
+    for (int q = 0; q < 5; ++q) I.lstats[2 + q] += tacc[q];
+    I.lstats[7] += swt.stream;
+    I.lstats[8] += swt.wait1;
+    I.lstats[9] += swt.fin;
+    I.lstats[10] += swt.wait2;
+  }
 }
 
 // Stand-alone global solve x = L^{-1} b (EigenDirectSolver::solve) on one instance.
-template <typename Real, bool kBlocked>
-__global__ void __launch_bounds__(1024) solveKernel(const InstDev* __restrict__ insts, int inst,
+template <typename Real, bool kBlocked, bool kIdx16>
+__global__ void __launch_bounds__(512) solveKernel(const InstDev* __restrict__ insts, int inst,
                                                     const double* __restrict__ b,
                                                     double* __restrict__ xout) {
   extern __shared__ double smem[];
   const InstDev& I = insts[inst];
   Real* W = I.ldsW ? reinterpret_cast<Real*>(smem + 32) : static_cast<Real*>(I.Wg);
+  if (kBlocked) loadStepTables(I);
   for (int v = threadIdx.x; v < I.n; v += blockDim.x) {
     const int r = I.vrank[v];
     putW(W, r, b[3 * v], b[3 * v + 1], b[3 * v + 2]);
   }
   __syncthreads();
   const Real* dinv = static_cast<const Real*>(I.dinv);
+  SweepTimes tst;
   auto none = [](int, Real, Real, Real) {};
   if (kBlocked)
-    blockSweep<Real, true>(I.bfwd, W, dinv, none);
+    blockSweep<Real, true, kIdx16>(I.bfwd, W, ldsAt<Real>(I.partOff), ldsAt<Real>(I.denseOff), ldsAt<int>(I.stepOff), dinv, none, tst, I.ldsW != 0);
   else
     sweep<Real, true>(I.fwd, W, dinv, none);
   const int* rankv = I.rankv;
@@ -573,21 +878,23 @@ __global__ void __launch_bounds__(1024) solveKernel(const InstDev* __restrict__ 
     xout[3 * v + 2] = (double)a2;
   };
   if (kBlocked)
-    blockSweep<Real, false>(I.bbwd, W, dinv, put);
+    blockSweep<Real, false, kIdx16>(I.bbwd, W, ldsAt<Real>(I.partOff), ldsAt<Real>(I.denseOff), ldsAt<int>(I.stepOff) + 3 * (I.bfwd.nstep + 1), dinv, put, tst, I.ldsW != 0);
   else
     sweep<Real, false>(I.bwd, W, dinv, put);
 }
 
 }  // namespace
 
-#define PDX_KERNEL_TABLE(X) X(float, false, true) X(float, false, false) X(double, true, true) X(double, true, false)
+#define PDX_KERNEL_TABLE(X)                                                                  \
+  X(float, false, true, true) X(float, false, true, false) X(float, false, false, false)   \
+  X(double, true, true, true) X(double, true, true, false) X(double, true, false, false)
 
 hipError_t launchSubstep(const InstDev* d, const int* order, int ninst, int nsub, int block,
-                         size_t lds, bool fp32, bool blocked, hipStream_t s) {
-#define PDX_L(R, E, B)                                                                            \
-  if (fp32 == (sizeof(R) == 4) && blocked == B) {                                                 \
-    hipLaunchKernelGGL((substepKernel<R, E, B>), dim3(ninst), dim3(block), lds, s, d, order, nsub); \
-    return hipGetLastError();                                                                     \
+                         size_t lds, bool fp32, bool blocked, bool idx16, hipStream_t s) {
+#define PDX_L(R, E, B, X16)                                                                             \
+  if (fp32 == (sizeof(R) == 4) && blocked == B && (idx16 && blocked) == X16) {                          \
+    hipLaunchKernelGGL((substepKernel<R, E, B, X16>), dim3(ninst), dim3(block), lds, s, d, order, nsub); \
+    return hipGetLastError();                                                                           \
   }
   PDX_KERNEL_TABLE(PDX_L)
 #undef PDX_L
@@ -595,11 +902,11 @@ hipError_t launchSubstep(const InstDev* d, const int* order, int ninst, int nsub
 }
 
 hipError_t launchSolve(const InstDev* d, int inst, const double* b, double* x, int block,
-                       size_t lds, bool fp32, bool blocked, hipStream_t s) {
-#define PDX_L(R, E, B)                                                                  \
-  if (fp32 == (sizeof(R) == 4) && blocked == B) {                                       \
-    hipLaunchKernelGGL((solveKernel<R, B>), dim3(1), dim3(block), lds, s, d, inst, b, x); \
-    return hipGetLastError();                                                           \
+                       size_t lds, bool fp32, bool blocked, bool idx16, hipStream_t s) {
+#define PDX_L(R, E, B, X16)                                                                       \
+  if (fp32 == (sizeof(R) == 4) && blocked == B && (idx16 && blocked) == X16) {                    \
+    hipLaunchKernelGGL((solveKernel<R, B, X16>), dim3(1), dim3(block), lds, s, d, inst, b, x);    \
+    return hipGetLastError();                                                                     \
   }
   PDX_KERNEL_TABLE(PDX_L)
 #undef PDX_L
@@ -608,12 +915,12 @@ hipError_t launchSolve(const InstDev* d, int inst, const double* b, double* x, i
 
 hipError_t setLdsLimit(size_t bytes) {
   hipError_t e = hipSuccess;
-#define PDX_L(R, E, B)                                                                                \
-  e = hipFuncSetAttribute((const void*)substepKernel<R, E, B>, hipFuncAttributeMaxDynamicSharedMemorySize, \
-                          (int)bytes);                                                                \
-  if (e != hipSuccess) return e;                                                                      \
-  e = hipFuncSetAttribute((const void*)solveKernel<R, B>, hipFuncAttributeMaxDynamicSharedMemorySize,     \
-                          (int)bytes);                                                                \
+#define PDX_L(R, E, B, X16)                                                                                 \
+  e = hipFuncSetAttribute((const void*)substepKernel<R, E, B, X16>, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                          (int)bytes);                                                                     \
+  if (e != hipSuccess) return e;                                                                           \
+  e = hipFuncSetAttribute((const void*)solveKernel<R, B, X16>, hipFuncAttributeMaxDynamicSharedMemorySize,     \
+                          (int)bytes);                                                                     \
   if (e != hipSuccess) return e;
   PDX_KERNEL_TABLE(PDX_L)
 #undef PDX_L

@@ -54,7 +54,7 @@ class _Stats(C.Structure):
                 ("last_residual", C.c_double), ("factor_nnz", C.c_longlong),
                 ("fwd_levels", C.c_int), ("bwd_levels", C.c_int), ("fwd_steps", C.c_int),
                 ("bwd_steps", C.c_int), ("analyze_seconds", C.c_double),
-                ("factorize_seconds", C.c_double)]
+                ("factorize_seconds", C.c_double), ("phase_seconds", C.c_double * 9)]
 
 
 _lib = None
@@ -89,6 +89,7 @@ def load_library(path=LIB_PATH):
         "pdx_solve3": (I, [P, I, P, P]),
         "pdx_get_factor": (I, [P, I, C.POINTER(I), C.POINTER(C.c_longlong), P, P, P, P, P, P]),
         "pdx_last_kernel_ms": (D, [P]),
+        "pdx_set_schedule": (I, [P, I, I]),
         "pdx_bytes_per_iteration": (D, [P]),
         "pdx_grid_counts": (I, [I, I, I, I, C.POINTER(I), C.POINTER(I), C.POINTER(I)]),
         "pdx_grid_fill": (I, [I, I, D, D, D, D, D, P, P, P, P, P, P, P, P, P]),
@@ -280,7 +281,9 @@ class Engine:
     def stats(self, i=0):
         s = _Stats()
         self._check(self.lib.pdx_get_stats(self.h, i, C.byref(s)), "get_stats")
-        return {k: getattr(s, k) for k, _ in _Stats._fields_}
+        out = {k: getattr(s, k) for k, _ in _Stats._fields_}
+        out["phase_seconds"] = list(s.phase_seconds)
+        return out
 
     def solve(self, i, b):
         b = np.ascontiguousarray(b, np.float64)
@@ -300,6 +303,9 @@ class Engine:
                                             _ptr(out["dinv"]), _ptr(out["rank_vertex"]),
                                             _ptr(out["perm3"])), "get_factor")
         return out
+
+    def set_schedule(self, block_threads, chain_cap=32):
+        self._check(self.lib.pdx_set_schedule(self.h, block_threads, chain_cap), "set_schedule")
 
     def last_kernel_ms(self):
         return self.lib.pdx_last_kernel_ms(self.h)
