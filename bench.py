@@ -43,11 +43,14 @@ def build_batch(cloths, cfg, precision, solve):
 
 
 def time_batch(e, steps, warmup):
+    """W untimed substeps, then exactly K substeps: a HIP graph holding one
+    persistent launch of one substep, replayed K times.  Returns the wall time and
+    the average launch duration from HIP events on the context's stream."""
     if warmup:
-        e.step(warmup)
+        e.step_graph(1, warmup)
     e.sync()
     t0 = time.perf_counter()
-    e.step(steps)
+    e.step_graph(1, steps)
     wall = time.perf_counter() - t0
     return wall, e.last_kernel_ms()
 
@@ -90,8 +93,41 @@ def cpu_baseline(cloths, cfg, perms, budget_s, threads):
     return inst, done_sub, t_tot
 
 
+def measured_traffic(precision, instances, seed, solve):
+    """HBM bytes per launch of the substep kernel from the committed counter runs
+    (FETCH_SIZE and WRITE_SIZE, each scaled by a plain copy kernel's reading), or
+    None when no run of this exact workload is on record."""
+    path = os.path.join(ROOT, "profiles", "substep_kernel_512mix_hbm_traffic.md")
+    try:
+        lines = open(path).read().splitlines()
+    except OSError:
+        return None
+    key = f"traffic_bytes_per_launch {precision} {instances} {seed} {solve} "
+    for ln in lines:
+        if ln.startswith("machine-readable: " + key):
+            return float(ln.split()[-1])
+    return None
+
+
+def copy_calibration(mib, reps=4):
+    """A plain device-to-device copy of known size (the runtime's copy kernel),
+    used to scale the FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    n = int(mib) << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(n)) == 0
+    for _ in range(reps):
+        assert hip.hipMemcpy(b, a, C.c_size_t(n), 3) == 0  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    print(json.dumps({"copy_calibration_bytes": n, "reps": reps}))
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--copy-calibration", type=int, default=0,
+                    help="only run a device copy of this many MiB (counter calibration)")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=2)
@@ -103,6 +139,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-dropin", action="store_true")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; the batch shards by instance across "
                                    "GPUs by design but no multi-GPU path is built (use --gpus 1)"}))
@@ -120,9 +159,10 @@ def main():
 
     # correctness: FP32 form vs the FP64 path over the full run length
     wall32, kms32 = time_batch(mixed, args.steps, args.warmup)
-    log(f"mixed32: {args.steps} steps in {wall32:.3f}s (kernel {kms32:.1f} ms)")
+    log(f"mixed32: {args.steps} steps in {wall32:.3f}s (kernel {kms32:.2f} ms per launch, "
+        f"{mixed.bytes_per_substep() / 1e6:.1f} MB algorithmic per launch)")
     wall64, kms64 = time_batch(fp64, args.steps, args.warmup)
-    log(f"fp64 batched: {args.steps} steps in {wall64:.3f}s (kernel {kms64:.1f} ms)")
+    log(f"fp64 batched: {args.steps} steps in {wall64:.3f}s (kernel {kms64:.2f} ms per launch)")
     errs = []
     for i in range(len(cloths)):
         x32, _ = mixed.state(i)
@@ -135,7 +175,7 @@ def main():
     value = ninst * args.steps / wall32
     ms_per_step = 1000.0 * wall32 / args.steps
     iters = cfg.max_iterations
-    bytes_launch = mixed.bytes_per_iteration() * iters * args.steps
+    bytes_launch = mixed.bytes_per_substep()  # one launch = one substep of the batch
     achieved = bytes_launch / (kms32 * 1e-3) / 1e9 if kms32 > 0 else None
 
     out = {
@@ -156,10 +196,11 @@ def main():
                    "seed": args.seed, "solve": args.solve},
         "max_pos_error_m": max_err,
         "fp64_batched": {"value": round(ninst * args.steps / wall64, 2), "unit": "cloth-substeps/s"},
+        "kernel_ms_per_launch": round(kms32, 4),
         "roofline": {"bound": "hbm", "achieved": None if achieved is None else round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": None if achieved is None else round(achieved / HBM_PEAK_GBS, 5),
-                     "traffic": None},
+                     "traffic": measured_traffic("mixed32", ninst, args.seed, args.solve)},
     }
     del fp64
     if not args.no_dropin:

@@ -156,9 +156,12 @@ int pdx_set_schedule(pdx_ctx* ctx, int block_threads, int chain_cap);
 /* Average duration (ms) of the substep kernel over the last `pdx_step*` call,
  * measured with HIP events on the stream the kernel runs on. */
 double pdx_last_kernel_ms(const pdx_ctx* ctx);
-/* Bytes of factor + schedule + scene arrays the kernel streams per PD iteration,
- * summed over instances (algorithmic, each array once per phase). */
+/* Algorithmic bytes of one PD iteration (factor both sweeps, maps, gather of
+ * edges/incidence for the right-hand side and the residual, vector passes),
+ * summed over instances; each array counted once per phase at storage width. */
 double pdx_bytes_per_iteration(const pdx_ctx* ctx);
+/* Same for one substep: iterations x the above + predict/velocity passes. */
+double pdx_bytes_per_substep(const pdx_ctx* ctx);
 
 /* --- scene builders (grid cloth, Mesh.cpp:12-54, :183-207, :219-285) ---------- */
 /* Fills caller buffers for an nx*ny grid in the xz-plane, spacing sx, sz.

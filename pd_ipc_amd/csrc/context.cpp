@@ -284,6 +284,7 @@ struct Instance {
   long long* dlstats = nullptr;
   double* ddstats = nullptr;
   double bytesPerIter = 0;
+  double bytesPerSubstepFixed = 0;
 };
 
 // Byte arena: one host staging buffer mirrored by one device allocation.
@@ -625,14 +626,21 @@ int upload(pdx_ctx* c) {
     o.ist = ar.vec(ist);
     o.lst = ar.vec(lst);
     o.dst = ar.vec(dst);
-    // algorithmic bytes per PD iteration: both sweeps + dinv + maps, the gather's
-    // scene arrays twice (rhs + residual), the per-vertex state passes
-    const double nb = (double)s.n, Eb = (double)s.E, Bb = (double)s.B;
-    double bi = sweepBytes(I.hf, w) + sweepBytes(I.hb, w) + blockBytes(I.bf, w) + blockBytes(I.bb, w) +
-                nb * (w + 8);
-    bi += 2.0 * (Eb * (8 + 16) + (nb + 1) * 4 + 2 * Eb * 4);
-    bi += nb * 24 * 4 + nb * 8 + nb;
-    if (Bb > 0) bi += Bb * (12 + 8) + (nb + 1) * 4 + 3 * Bb * 5 + nb * 24;
+    // Algorithmic (compulsory) bytes of one PD iteration, each array counted once
+    // per phase at its storage width: w = factor value width, i = factor index
+    // width (2 for 16-bit ranks), scene data and state in FP64:
+    //   2F(w+i) + 2(m+1)*4 + m(w+4)            factor forward + backward, maps, 1/D
+    // + 2[E(2*4 + 2*8) + (n+1)*4 + 2E*4]       edges + incidence, rhs and residual
+    // + 24n*8 + n*8 + n                        vector passes of 3n, mass, pin mask
+    // + [B>0](B(3*4 + 8) + (n+1)*4 + 3B(4+8) + 3n*8)   stencils, bend CSR, bend rhs
+    const double nb = (double)s.n, Eb = (double)s.E, Bb = (double)s.B, Fb = (double)I.fac.nnz,
+                 mb = (double)I.fac.m;
+    const double iw = (c->mode == PDX_SOLVE_FAST && c->idx16) ? 2.0 : 4.0;
+    double bi = 2.0 * Fb * ((double)w + iw) + 2.0 * (mb + 1) * 4 + mb * ((double)w + 4);
+    bi += 2.0 * (Eb * (2 * 4 + 2 * 8) + (nb + 1) * 4 + 2 * Eb * 4);
+    bi += 24.0 * nb * 8 + nb * 8 + nb;
+    if (Bb > 0) bi += Bb * (3 * 4 + 8) + (nb + 1) * 4 + 3 * Bb * (4 + 8) + 3 * nb * 8;
+    I.bytesPerSubstepFixed = 27.0 * nb * 8;  // predict, inertial rhs, velocity
     I.bytesPerIter = bi;
     c->bytesIter += bi;
   }
@@ -1119,5 +1127,12 @@ int pdx_set_schedule(pdx_ctx* c, int block_threads, int chain_cap) {
 
 double pdx_last_kernel_ms(const pdx_ctx* c) { return c ? c->lastKernelMs : 0.0; }
 double pdx_bytes_per_iteration(const pdx_ctx* c) { return c ? c->bytesIter : 0.0; }
+
+double pdx_bytes_per_substep(const pdx_ctx* c) {
+  if (!c) return 0.0;
+  double b = 0.0;
+  for (const auto& I : c->inst) b += I.bytesPerIter * I.cfg.maxIter + I.bytesPerSubstepFixed;
+  return b;
+}
 
 }  // extern "C"

@@ -91,6 +91,7 @@ def load_library(path=LIB_PATH):
         "pdx_last_kernel_ms": (D, [P]),
         "pdx_set_schedule": (I, [P, I, I]),
         "pdx_bytes_per_iteration": (D, [P]),
+        "pdx_bytes_per_substep": (D, [P]),
         "pdx_grid_counts": (I, [I, I, I, I, C.POINTER(I), C.POINTER(I), C.POINTER(I)]),
         "pdx_grid_fill": (I, [I, I, D, D, D, D, D, P, P, P, P, P, P, P, P, P]),
     }
@@ -312,3 +313,6 @@ class Engine:
 
     def bytes_per_iteration(self):
         return self.lib.pdx_bytes_per_iteration(self.h)
+
+    def bytes_per_substep(self):
+        return self.lib.pdx_bytes_per_substep(self.h)
