@@ -147,6 +147,16 @@ int pdx_get_factor(pdx_ctx* ctx, int instance, int* m, long long* nnz, int32_t* 
                    int32_t* rows, double* vals, double* dinv, int32_t* rank_vertex,
                    int32_t* perm3);
 
+/* Topology of one instance (ConstraintColoring.h:34-81, Mesh::constraintColoring and
+ * Mesh::bendAdjacency): colour of each edge color_of[E], edges by colour order[E],
+ * colour ranges start[C+1], vertex -> edge incidence in edge order vertex_start[n+1] /
+ * vertex_edges[2E], vertex -> stencil incidence bend_vertex_start[n+1] /
+ * bend_stencils[3B].  Any pointer may be NULL. */
+int pdx_get_topology(pdx_ctx* ctx, int instance, int* color_count, uint32_t* color_of,
+                     uint32_t* order, uint32_t* start, uint32_t* vertex_start,
+                     uint32_t* vertex_edges, uint32_t* bend_vertex_start,
+                     uint32_t* bend_stencils);
+
 /* Launch shape: threads per workgroup (64..512, multiple of 64) and the longest
  * elimination-tree chain one wavefront takes as a block (1..32, fast mode).
  * Changing the chain cap re-packs the schedules (no new factorisation is counted). */

@@ -1109,6 +1109,30 @@ int pdx_get_factor(pdx_ctx* c, int i, int* m, long long* nnz, int32_t* ptr, int3
   return PDX_OK;
 }
 
+int pdx_get_topology(pdx_ctx* c, int i, int* color_count, uint32_t* color_of, uint32_t* order,
+                     uint32_t* start, uint32_t* vertex_start, uint32_t* vertex_edges,
+                     uint32_t* bend_vertex_start, uint32_t* bend_stencils) {
+  if (!c || i < 0 || i >= (int)c->inst.size()) return PDX_EINVAL;
+  Instance& I = c->inst[(size_t)i];
+  if (!I.topoBuilt) {
+    buildTopology(I.scene, I.topo);
+    I.topoBuilt = true;
+  }
+  const Topology& t = I.topo;
+  if (color_count) *color_count = t.colorCount;
+  auto put = [](const std::vector<uint32_t>& a, uint32_t* out) {
+    if (out) std::copy(a.begin(), a.end(), out);
+  };
+  put(t.colorOf, color_of);
+  put(t.corder, order);
+  put(t.cstart, start);
+  put(t.vstart, vertex_start);
+  put(t.vedges, vertex_edges);
+  put(t.bstart, bend_vertex_start);
+  put(t.bstencil, bend_stencils);
+  return PDX_OK;
+}
+
 int pdx_set_schedule(pdx_ctx* c, int block_threads, int chain_cap) {
   if (!c || block_threads < 64 || block_threads > 512 || block_threads % 64 != 0 || chain_cap < 1 ||
       chain_cap > 32)

@@ -88,6 +88,7 @@ def load_library(path=LIB_PATH):
         "pdx_get_stats": (I, [P, I, C.POINTER(_Stats)]),
         "pdx_solve3": (I, [P, I, P, P]),
         "pdx_get_factor": (I, [P, I, C.POINTER(I), C.POINTER(C.c_longlong), P, P, P, P, P, P]),
+        "pdx_get_topology": (I, [P, I, C.POINTER(I), P, P, P, P, P, P, P]),
         "pdx_last_kernel_ms": (D, [P]),
         "pdx_set_schedule": (I, [P, I, I]),
         "pdx_bytes_per_iteration": (D, [P]),
@@ -303,6 +304,26 @@ class Engine:
                                             _ptr(out["ptr"]), _ptr(out["rows"]), _ptr(out["vals"]),
                                             _ptr(out["dinv"]), _ptr(out["rank_vertex"]),
                                             _ptr(out["perm3"])), "get_factor")
+        return out
+
+    def topology(self, i=0):
+        """Edge colouring and incidence lists of instance i (Mesh::constraintColoring,
+        Mesh::bendAdjacency)."""
+        c = self.cloths[i]
+        ncol = C.c_int()
+        self._check(self.lib.pdx_get_topology(self.h, i, C.byref(ncol), None, None, None, None, None,
+                                              None, None), "get_topology")
+        n, E, B = c.n, len(c.edge_a), len(c.bend_a)
+        out = dict(color_of=np.empty(E, np.uint32), order=np.empty(E, np.uint32),
+                   start=np.empty(ncol.value + 1, np.uint32), vertex_start=np.empty(n + 1, np.uint32),
+                   vertex_edges=np.empty(2 * E, np.uint32), bend_vertex_start=np.empty(n + 1, np.uint32),
+                   bend_stencils=np.empty(3 * B, np.uint32))
+        self._check(self.lib.pdx_get_topology(
+            self.h, i, C.byref(ncol), _ptr(out["color_of"]), _ptr(out["order"]), _ptr(out["start"]),
+            _ptr(out["vertex_start"]), _ptr(out["vertex_edges"]),
+            _ptr(out["bend_vertex_start"]) if B else None, _ptr(out["bend_stencils"]) if B else None),
+            "get_topology")
+        out["color_count"] = ncol.value
         return out
 
     def set_schedule(self, block_threads, chain_cap=32):
