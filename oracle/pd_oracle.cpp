@@ -981,6 +981,200 @@ void orc_set_pin_positions(orc_ctx* c, const double* x) {
   for (int v = 0; v < c->n; ++v) c->pinPositions[(size_t)v] = {x[3 * v], x[3 * v + 1], x[3 * v + 2]};
 }
 void orc_set_edge_kappa(orc_ctx* c, const double* kappa) { c->kappa.assign(kappa, kappa + c->E); }
+void orc_set_bend_k(orc_ctx* c, const double* k) { c->bk.assign(k, k + c->B); }
+
+// Reverse-mode derivative of `nsub` substeps with a fixed iteration count
+// (the unrolled map of stepOnce above: predict, K x [projection + colour-ordered
+// scatter + pin rows + solve], velocity/damping/re-pin).  There is no adjoint in
+// the reference; this is the FP64 CPU adjoint that the GPU adjoint is checked
+// against, itself checked against central finite differences of stepOnce.
+//
+// Per iteration L x^{k+1} = b(x^k) with b(x) = c + P(x): c holds the inertial
+// term (m/h^2) x^, the bend constant, the pin compensation kappa q and the pin
+// rows q; P(x) the projections kappa d_e(x^k).  With mu = L^{-1} xbar^{k+1} (L is
+// symmetric, so the same factor serves the transposed solve):
+//   xbar^k  = J_P(x^k)^T mu,   cbar += mu,
+//   kbar_e += mu . dP/dkappa_e - mu^T (dL/dkappa_e) x^{k+1},
+//   kbar_s += -mu^T (dL/dk_s) x^{k+1}.
+// After the iterations, cbar feeds x_n and v_n (through x^), the pins and the
+// stiffnesses (bend constant, pin compensation).
+int orc_rollout_adjoint(orc_ctx* c, int nsub, const double* pin_traj, const double* seed_x,
+                        const double* seed_v, double* g_x0, double* g_v0, double* g_kappa,
+                        double* g_bend, double* g_pin) {
+  const int n = c->n;
+  const int K = c->cfg.max_iterations;
+  const size_t n3 = (size_t)3 * n;
+  std::vector<int> pins, pinIdx((size_t)n, -1);
+  for (int v = 0; v < n; ++v)
+    if (c->isPinned(v)) {
+      pinIdx[(size_t)v] = (int)pins.size();
+      pins.push_back(v);
+    }
+  const size_t np = pins.size();
+  // forward with a tape of every iterate: x^0 .. x^K per substep, v_n, q_n
+  std::vector<double> tape((size_t)nsub * (K + 1) * n3), vtape((size_t)nsub * n3),
+      qtape((size_t)nsub * np * 3);
+  auto getX = [&](double* out) { orc_get_positions(c, out); };
+  for (int s = 0; s < nsub; ++s) {
+    if (pin_traj)
+      for (size_t j = 0; j < np; ++j)
+        c->pinPositions[(size_t)pins[j]] = {pin_traj[(s * np + j) * 3], pin_traj[(s * np + j) * 3 + 1],
+                                            pin_traj[(s * np + j) * 3 + 2]};
+    for (size_t j = 0; j < np; ++j) {
+      const V3& q = c->pinPositions[(size_t)pins[j]];
+      qtape[(s * np + j) * 3] = q.x;
+      qtape[(s * np + j) * 3 + 1] = q.y;
+      qtape[(s * np + j) * 3 + 2] = q.z;
+    }
+    orc_get_velocities(c, &vtape[(size_t)s * n3]);
+    getX(&tape[((size_t)s * (K + 1)) * n3]);
+    // run the substep one iteration at a time through the reference step with
+    // max_iterations = k+1 would change the factor stamp only through dt (not
+    // touched), so instead the iterates are recomputed here by the same calls:
+    const orc_config saved = c->cfg;
+    std::vector<V3> x0 = c->positions, v0 = c->velocities, pbs = c->positionsBeforeStep,
+                    ppbs = c->positionsBeforePrevStep;
+    for (int k = 1; k <= K; ++k) {
+      c->positions = x0;
+      c->velocities = v0;
+      c->positionsBeforeStep = pbs;
+      c->positionsBeforePrevStep = ppbs;
+      c->cfg.max_iterations = k;
+      const int used = stepOnce(*c);
+      if (used < 0) return used;
+      if (used != k) {
+        c->cfg = saved;
+        return -5;  // the stop test fired: the unrolled map is not fixed-K
+      }
+      // stepOnce re-pins after the velocity update; x^k of the iteration is
+      // xSolution (pinned rows hold q, equal to the re-pinned positions)
+      std::memcpy(&tape[((size_t)s * (K + 1) + k) * n3], c->xSolution.data(), n3 * sizeof(double));
+    }
+    c->cfg = saved;
+  }
+
+  const double h = c->cfg.dt, invH2 = 1.0 / (h * h);
+  const double cv = (1.0 - c->cfg.damping) * (1.0 / h);
+  std::vector<double> xbar(seed_x, seed_x + n3), vbar(n3, 0.0);
+  if (seed_v) vbar.assign(seed_v, seed_v + n3);
+  std::vector<double> gk((size_t)c->E, 0.0), gb((size_t)c->B, 0.0), gp((size_t)nsub * np * 3, 0.0);
+  std::vector<double> cur(n3), xnbar(n3), bbar(n3), mu(n3), work;
+  const double w3[3] = {1.0, -2.0, 1.0};
+  for (int s = nsub - 1; s >= 0; --s) {
+    const double* q = &qtape[(size_t)s * np * 3];
+    auto qOf = [&](int v, int d) { return q[(size_t)pinIdx[(size_t)v] * 3 + d]; };
+    double* gq = &gp[(size_t)s * np * 3];
+    for (int v = 0; v < n; ++v)
+      for (int d = 0; d < 3; ++d) {
+        const size_t i = (size_t)3 * v + d;
+        if (c->isPinned(v)) {
+          gq[(size_t)pinIdx[(size_t)v] * 3 + d] += xbar[i];
+          cur[i] = 0.0;
+          xnbar[i] = 0.0;
+        } else {
+          cur[i] = xbar[i] + cv * vbar[i];
+          xnbar[i] = -cv * vbar[i];
+        }
+      }
+    std::fill(bbar.begin(), bbar.end(), 0.0);
+    for (int k = K - 1; k >= 0; --k) {
+      for (int comp = 0; comp < c->components; ++comp)
+        solveComponent(*c, c->components == 1 ? -1 : comp, cur.data(), mu.data(), work);
+      for (size_t i = 0; i < n3; ++i) bbar[i] += mu[i];
+      const double* xk = &tape[((size_t)s * (K + 1) + k) * n3];
+      const double* xk1 = &tape[((size_t)s * (K + 1) + k + 1) * n3];
+      std::fill(cur.begin(), cur.end(), 0.0);
+      for (int e = 0; e < c->E; ++e) {
+        const int a = c->ea[(size_t)e], b = c->eb[(size_t)e];
+        const bool aP = c->isPinned(a), bP = c->isPinned(b);
+        if (aP && bP) continue;
+        const double kap = c->kappa[(size_t)e], l = c->rest[(size_t)e];
+        const V3 r{xk[3 * a] - xk[3 * b], xk[3 * a + 1] - xk[3 * b + 1], xk[3 * a + 2] - xk[3 * b + 2]};
+        const double len = length(r);
+        const V3 ma = aP ? V3{} : V3{mu[3 * a], mu[3 * a + 1], mu[3 * a + 2]};
+        const V3 mb = bP ? V3{} : V3{mu[3 * b], mu[3 * b + 1], mu[3 * b + 2]};
+        const V3 gd = (ma - mb) * kap;
+        if (len > kMinLength) {
+          const V3 rh = r / len;
+          const V3 rbar = (gd - rh * dot(rh, gd)) * (l / len);
+          cur[3 * a] += rbar.x;
+          cur[3 * a + 1] += rbar.y;
+          cur[3 * a + 2] += rbar.z;
+          cur[3 * b] -= rbar.x;
+          cur[3 * b + 1] -= rbar.y;
+          cur[3 * b + 2] -= rbar.z;
+          gk[(size_t)e] += dot(ma - mb, r * (l / len));
+        }
+        const V3 xa{xk1[3 * a], xk1[3 * a + 1], xk1[3 * a + 2]};
+        const V3 xb{xk1[3 * b], xk1[3 * b + 1], xk1[3 * b + 2]};
+        if (!aP && !bP) gk[(size_t)e] -= dot(ma - mb, xa - xb);
+        else if (aP) gk[(size_t)e] -= dot(mb, xb);
+        else gk[(size_t)e] -= dot(ma, xa);
+      }
+      for (int t = 0; t < c->B; ++t) {
+        const int vv[3] = {c->sa[(size_t)t], c->sb[(size_t)t], c->sc[(size_t)t]};
+        V3 sm{}, sx{};
+        for (int p = 0; p < 3; ++p) {
+          if (c->isPinned(vv[p])) continue;
+          const size_t i = (size_t)3 * vv[p];
+          sm += V3{mu[i], mu[i + 1], mu[i + 2]} * w3[p];
+          sx += V3{xk1[i], xk1[i + 1], xk1[i + 2]} * w3[p];
+        }
+        gb[(size_t)t] -= dot(sm, sx);
+      }
+    }
+    for (int v = 0; v < n; ++v)
+      for (int d = 0; d < 3; ++d) {
+        const size_t i = (size_t)3 * v + d;
+        if (c->isPinned(v)) {
+          xnbar[i] += cur[i];
+          gq[(size_t)pinIdx[(size_t)v] * 3 + d] += bbar[i];
+          vbar[i] = 0.0;
+        } else {
+          const double xh = c->masses[(size_t)v] * invH2 * bbar[i];
+          xnbar[i] += cur[i] + xh;
+          vbar[i] = xh * h;
+        }
+      }
+    for (int e = 0; e < c->E; ++e) {
+      const int a = c->ea[(size_t)e], b = c->eb[(size_t)e];
+      const bool aP = c->isPinned(a), bP = c->isPinned(b);
+      if (aP == bP) continue;
+      const int u = aP ? a : b, f = aP ? b : a;
+      for (int d = 0; d < 3; ++d) {
+        gq[(size_t)pinIdx[(size_t)u] * 3 + d] += c->kappa[(size_t)e] * bbar[(size_t)3 * f + d];
+        gk[(size_t)e] += bbar[(size_t)3 * f + d] * qOf(u, d);
+      }
+    }
+    for (int t = 0; t < c->B; ++t) {
+      const int vv[3] = {c->sa[(size_t)t], c->sb[(size_t)t], c->sc[(size_t)t]};
+      double C3[3] = {0.0, 0.0, 0.0};
+      bool any = false;
+      for (int r = 0; r < 3; ++r)
+        if (c->isPinned(vv[r])) {
+          any = true;
+          for (int d = 0; d < 3; ++d) C3[d] += w3[r] * qOf(vv[r], d);
+        }
+      if (!any) continue;
+      for (int p = 0; p < 3; ++p) {
+        if (c->isPinned(vv[p])) continue;
+        const double* bb = &bbar[(size_t)3 * vv[p]];
+        gb[(size_t)t] -= w3[p] * (bb[0] * C3[0] + bb[1] * C3[1] + bb[2] * C3[2]);
+        for (int r = 0; r < 3; ++r)
+          if (c->isPinned(vv[r]))
+            for (int d = 0; d < 3; ++d)
+              gq[(size_t)pinIdx[(size_t)vv[r]] * 3 + d] -= c->bk[(size_t)t] * w3[p] * w3[r] * bb[d];
+      }
+    }
+    xbar = xnbar;
+  }
+  if (g_x0) std::copy(xbar.begin(), xbar.end(), g_x0);
+  if (g_v0) std::copy(vbar.begin(), vbar.end(), g_v0);
+  if (g_kappa) std::copy(gk.begin(), gk.end(), g_kappa);
+  if (g_bend) std::copy(gb.begin(), gb.end(), g_bend);
+  if (g_pin) std::copy(gp.begin(), gp.end(), g_pin);
+  return 0;
+}
 void orc_set_config(orc_ctx* c, const orc_config* cfg) { c->cfg = *cfg; }
 
 void orc_stats(const orc_ctx* c, int64_t* s) {

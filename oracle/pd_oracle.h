@@ -57,6 +57,18 @@ void orc_set_velocities(orc_ctx* c, const double* v3n);
 void orc_set_pin_positions(orc_ctx* c, const double* q3n);
 void orc_set_edge_kappa(orc_ctx* c, const double* kappa);
 void orc_set_config(orc_ctx* c, const orc_config* cfg);
+void orc_set_bend_k(orc_ctx* c, const double* k);
+
+/* FP64 reverse-mode derivative of nsub substeps run from the current state with
+ * a fixed iteration count (max_iterations every substep; returns -5 if the stop
+ * test fired).  pin_traj (nsub x pins x 3, pins in ascending vertex order, or
+ * NULL = current pin positions) sets the pin targets of each substep.  Seeds:
+ * dLoss/dx_T (3n) and dLoss/dv_T (3n or NULL).  Outputs (any may be NULL):
+ * dLoss/dx_0 (3n), dLoss/dv_0 (3n), dLoss/dkappa_e (E), dLoss/dk_s (B),
+ * dLoss/dq (nsub x pins x 3).  The state is advanced by the nsub substeps. */
+int orc_rollout_adjoint(orc_ctx* c, int nsub, const double* pin_traj, const double* seed_x,
+                        const double* seed_v, double* g_x0, double* g_v0, double* g_kappa,
+                        double* g_bend, double* g_pin);
 
 /* stats[0]=iterationsUsed, [1]=converged, [2]=earlyExitCount, [3]=factorizeCount,
  * [4]=analyzeCount, [5]=colorCount, [6]=compact factor nnz (one component),

@@ -45,6 +45,8 @@ def lib():
             ("orc_coloring", I, [P, P, P, P]), ("orc_factor", I, [P]),
             ("orc_compact_factor", I, [P, P, P, P, P, P, P]), ("orc_matrix", C.c_int64, [P, P, P, P]),
             ("orc_solve3", I, [P, P, P]), ("orc_nonlinear_residual", D, [P]), ("orc_bending_rhs", None, [P, P]),
+            ("orc_set_bend_k", None, [P, P]),
+            ("orc_rollout_adjoint", I, [P, I, P, P, P, P, P, P, P, P]),
         ]:
             fn = getattr(L, name)
             fn.restype = res
@@ -121,6 +123,29 @@ class Oracle:
 
     def set_config(self, cfg):
         self.L.orc_set_config(self.h, C.byref(_cfg(cfg)))
+
+    def set_bend_k(self, k):
+        self.L.orc_set_bend_k(self.h, _p(np.ascontiguousarray(k, np.float64)))
+
+    def rollout_adjoint(self, nsub, pin_traj, seed_x, seed_v=None):
+        """FP64 CPU adjoint of nsub fixed-iteration substeps (advances the state).
+        Returns dict of gradients: x0, v0, kappa (per edge), bend (per stencil),
+        pin (nsub x pins x 3)."""
+        n, E, B = self.cloth.n, self.cloth.n_edges, len(self.cloth.bend_a)
+        npin = int(self.cloth.pinned.sum())
+        out = dict(x0=np.zeros(3 * n), v0=np.zeros(3 * n), kappa=np.zeros(E), bend=np.zeros(max(B, 1)),
+                   pin=np.zeros(nsub * npin * 3))
+        traj = None if pin_traj is None else np.ascontiguousarray(pin_traj, np.float64).reshape(-1)
+        sx = np.ascontiguousarray(seed_x, np.float64)
+        sv = None if seed_v is None else np.ascontiguousarray(seed_v, np.float64)
+        rc = self.L.orc_rollout_adjoint(self.h, nsub, None if traj is None else _p(traj), _p(sx),
+                                        None if sv is None else _p(sv), _p(out["x0"]), _p(out["v0"]),
+                                        _p(out["kappa"]), _p(out["bend"]), _p(out["pin"]))
+        if rc != 0:
+            raise RuntimeError(f"orc_rollout_adjoint failed: {rc}")
+        out["bend"] = out["bend"][:B]
+        out["pin"] = out["pin"].reshape(nsub, npin, 3)
+        return out
 
     def stats(self):
         s = np.zeros(9, np.int64)
