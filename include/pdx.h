@@ -135,6 +135,8 @@ int pdx_set_state(pdx_ctx* ctx, int instance, const double* positions, const dou
 int pdx_set_pin_positions(pdx_ctx* ctx, int instance, const double* pin_positions);
 /* Changes edge stiffness; the next step refactors once (stamp change). */
 int pdx_set_edge_kappa(pdx_ctx* ctx, int instance, const double* kappa);
+/* Changes bend stencil stiffness (BendStencil::stiffness); refactors once likewise. */
+int pdx_set_bend_k(pdx_ctx* ctx, int instance, const double* k);
 int pdx_set_config(pdx_ctx* ctx, int instance, const pdx_config* cfg);
 int pdx_get_stats(pdx_ctx* ctx, int instance, pdx_stats* out);
 
@@ -161,6 +163,34 @@ int pdx_get_topology(pdx_ctx* ctx, int instance, int* color_count, uint32_t* col
  * elimination-tree chain one wavefront takes as a block (1..32, fast mode).
  * Changing the chain cap re-packs the schedules (no new factorisation is counted). */
 int pdx_set_schedule(pdx_ctx* ctx, int block_threads, int chain_cap);
+
+/* --- differentiable rollout and adjoint ---------------------------------------
+ * No counterpart in the reference (it has no adjoint); these entry points are the
+ * "seed dLoss/dx_T, return dLoss/dv_0, dLoss/dkappa, dLoss/dq(t)" boundary of the
+ * differentiable batched form.  The rollout runs every instance for a fixed
+ * iteration count (max_iterations every substep: no stop test), so the map it
+ * differentiates is the unrolled substep. */
+/* Number of pinned vertices of an instance (the pin index order is ascending vertex). */
+int pdx_pin_count(const pdx_ctx* ctx, int instance);
+/* Pin targets for each substep of the next rollout: traj[nsub][pins][3].  NULL
+ * clears it (the pins then stay at their current targets). */
+int pdx_set_pin_trajectory(pdx_ctx* ctx, int instance, int nsub, const double* traj);
+/* Runs nsub substeps from the current state on every instance, checkpointing the
+ * state every `checkpoint_every` substeps; the state advances as pdx_step would. */
+int pdx_rollout(pdx_ctx* ctx, int nsub, int checkpoint_every);
+/* dLoss/dx and dLoss/dv of the state after the rollout (3n each, NULL = zero). */
+int pdx_set_adjoint_seed(pdx_ctx* ctx, int instance, const double* dx, const double* dv);
+/* Reverse pass of the last rollout: per checkpoint segment, recompute the forward
+ * recording every iterate, then run the adjoint kernel (transposed solves on the
+ * same factor + projection Jacobians).  Leaves the state as after the rollout. */
+int pdx_adjoint(pdx_ctx* ctx);
+/* Gradients of the last pdx_adjoint: dLoss/dx_0 and dLoss/dv_0 (3n), dLoss/dkappa
+ * per edge (E), dLoss/dk per bend stencil (B), dLoss/dq [nsub][pins][3].  Any
+ * pointer may be NULL. */
+int pdx_get_gradients(pdx_ctx* ctx, int instance, double* g_x0, double* g_v0, double* g_kappa,
+                      double* g_bend, double* g_pin);
+/* Device time (ms) of the last pdx_rollout and of the last pdx_adjoint. */
+int pdx_rollout_timing(const pdx_ctx* ctx, double* forward_ms, double* adjoint_ms);
 
 /* --- timing ---------------------------------------------------------------- */
 /* Average duration (ms) of the substep kernel over the last `pdx_step*` call,

@@ -19,6 +19,9 @@
 
 namespace pdx {
 hipError_t launchSubstep(const InstDev* d, const int* order, int ninst, int nsub, int block,
+                         size_t lds, bool fp32, bool blocked, bool idx16, hipStream_t s, int subBase,
+                         int flags);
+hipError_t launchAdjoint(const InstDev* d, const int* order, int ninst, int nsub, int subBase, int block,
                          size_t lds, bool fp32, bool blocked, bool idx16, hipStream_t s);
 hipError_t launchSolve(const InstDev* d, int inst, const double* b, double* x, int block,
                        size_t lds, bool fp32, bool blocked, bool idx16, hipStream_t s);
@@ -31,6 +34,8 @@ namespace {
 
 constexpr size_t kLdsLimit = 160 * 1024;
 constexpr size_t kRedBytes = 32 * sizeof(double);
+// launch flags of the differentiable rollout (the substep kernel's kRoll*)
+constexpr int kRollTraj = 1, kRollTape = 2, kRollFixed = 4;
 
 struct HostSweep {
   std::vector<int> levChunk, chunkRows, chunkRowBase, chunkEntBase, slotRank, slotLen, col;
@@ -330,6 +335,18 @@ struct pdx_ctx {
   double* dScratchB = nullptr;
   double* dScratchX = nullptr;
   int scratchN = 0;
+  int uploadEpoch = 0;
+  // differentiable rollout: pin trajectories (host), device buffers, seeds
+  struct Roll {
+    int nsub = 0, every = 0, nseg = 0, epoch = -1;
+    bool forwardDone = false;
+    char* dbuf = nullptr;
+    std::vector<std::vector<double>> traj;   // per instance: nsub x npin x 3, or empty
+    std::vector<int> trajSub;                // substeps of each set trajectory
+    std::vector<size_t> ckpt;                // per instance: offset of (nseg+1) x {x, v}
+    std::vector<std::vector<double>> seedX, seedV;
+    double forwardMs = 0.0, adjointMs = 0.0;
+  } roll;
 };
 
 namespace {
@@ -695,8 +712,8 @@ int upload(pdx_ctx* c) {
     d.prev = (double*)(B + o.prev);
     d.mass = (const double*)(B + o.mass);
     d.pinned = (const uint8_t*)(B + o.pinned);
-    d.pinpos = (const double*)(B + o.pinpos);
-    d.brhs = s.B > 0 ? (const double*)(B + o.brhs) : nullptr;
+    d.pinpos = (double*)(B + o.pinpos);
+    d.brhs = s.B > 0 ? (double*)(B + o.brhs) : nullptr;
     d.ea = (const int*)(B + o.ea);
     d.eb = (const int*)(B + o.eb);
     d.rest = (const double*)(B + o.rest);
@@ -781,6 +798,7 @@ int upload(pdx_ctx* c) {
   HIPCHECK(c, setLdsLimit(kLdsLimit));
   c->uploaded = true;
   c->dirty = false;
+  c->uploadEpoch += 1;
   return PDX_OK;
 }
 
@@ -834,7 +852,7 @@ int launch(pdx_ctx* c, int nsub) {
   HIPCHECK(c, hipSetDevice(c->device));
   HIPCHECK(c, hipEventRecord(c->ev0, c->stream));
   HIPCHECK(c, launchSubstep(c->dInst, c->dOrder, (int)c->inst.size(), nsub, c->block, c->lds,
-                            c->prec == PDX_PREC_MIXED32, c->mode == PDX_SOLVE_FAST, c->idx16, c->stream));
+                            c->prec == PDX_PREC_MIXED32, c->mode == PDX_SOLVE_FAST, c->idx16, c->stream, 0, 0));
   HIPCHECK(c, hipEventRecord(c->ev1, c->stream));
   c->lastLaunches = 1;
   return PDX_OK;
@@ -864,6 +882,7 @@ int pdx_destroy(pdx_ctx* c) {
     hipStreamSynchronize(c->stream);
   }
   if (c->dbuf) hipFree(c->dbuf);
+  if (c->roll.dbuf) hipFree(c->roll.dbuf);
   if (c->dScratchB) hipFree(c->dScratchB);
   if (c->dScratchX) hipFree(c->dScratchX);
   if (c->ev0) hipEventDestroy(c->ev0);
@@ -952,7 +971,8 @@ int pdx_step_graph(pdx_ctx* c, int nsub, int replays) {
   hipGraphExec_t ge = nullptr;
   HIPCHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   const hipError_t le = launchSubstep(c->dInst, c->dOrder, (int)c->inst.size(), nsub, c->block, c->lds,
-                                      c->prec == PDX_PREC_MIXED32, c->mode == PDX_SOLVE_FAST, c->idx16, c->stream);
+                                      c->prec == PDX_PREC_MIXED32, c->mode == PDX_SOLVE_FAST, c->idx16, c->stream,
+                                      0, 0);
   HIPCHECK(c, hipStreamEndCapture(c->stream, &g));
   if (le != hipSuccess) {
     hipGraphDestroy(g);
@@ -1016,6 +1036,16 @@ int pdx_set_edge_kappa(pdx_ctx* c, int i, const double* kappa) {
   const int rc = pullState(c);
   if (rc) return rc;
   std::memcpy(I.scene.kappa.data(), kappa, I.scene.kappa.size() * 8);
+  if (numericStamp(I.scene, I.cfg.dt) != I.numStamp) c->dirty = true;
+  return PDX_OK;
+}
+
+int pdx_set_bend_k(pdx_ctx* c, int i, const double* k) {
+  if (!c || !k || i < 0 || i >= (int)c->inst.size()) return PDX_EINVAL;
+  Instance& I = c->inst[(size_t)i];
+  const int rc = pullState(c);
+  if (rc) return rc;
+  std::memcpy(I.scene.bk.data(), k, I.scene.bk.size() * 8);
   if (numericStamp(I.scene, I.cfg.dt) != I.numStamp) c->dirty = true;
   return PDX_OK;
 }
@@ -1108,6 +1138,299 @@ int pdx_get_factor(pdx_ctx* c, int i, int* m, long long* nnz, int32_t* ptr, int3
   if (perm3) std::copy(f.perm3.begin(), f.perm3.end(), perm3);
   return PDX_OK;
 }
+
+}  // extern "C"
+
+namespace {
+
+// Pin lists of one instance for the rollout: pinned vertices (ascending), edges
+// with exactly one pinned end {edge, pin index}, stencils holding a pinned vertex
+// {stencil, pin index or -1 per vertex}.
+void pinLists(const Scene& s, std::vector<int>& pins, std::vector<int>& pe, std::vector<int>& ps) {
+  std::vector<int> idx((size_t)s.n, -1);
+  pins.clear();
+  pe.clear();
+  ps.clear();
+  for (int v = 0; v < s.n; ++v)
+    if (s.pinned[(size_t)v]) {
+      idx[(size_t)v] = (int)pins.size();
+      pins.push_back(v);
+    }
+  for (int e = 0; e < s.E; ++e) {
+    const int a = s.ea[(size_t)e], b = s.eb[(size_t)e];
+    const bool ap = s.pinned[(size_t)a] != 0, bp = s.pinned[(size_t)b] != 0;
+    if (ap == bp) continue;
+    pe.push_back(e);
+    pe.push_back(ap ? idx[(size_t)a] : idx[(size_t)b]);
+  }
+  for (int t = 0; t < s.B; ++t) {
+    const int vv[3] = {s.sa[(size_t)t], s.sb[(size_t)t], s.sc[(size_t)t]};
+    if (!s.pinned[(size_t)vv[0]] && !s.pinned[(size_t)vv[1]] && !s.pinned[(size_t)vv[2]]) continue;
+    ps.push_back(t);
+    for (int k = 0; k < 3; ++k) ps.push_back(idx[(size_t)vv[k]]);
+  }
+}
+
+size_t alignUp(size_t b) { return (b + 255) / 256 * 256; }
+
+int rollLaunch(pdx_ctx* c, int nsub, int subBase, int flags) {
+  HIPCHECK(c, launchSubstep(c->dInst, c->dOrder, (int)c->inst.size(), nsub, c->block, c->lds,
+                            c->prec == PDX_PREC_MIXED32, c->mode == PDX_SOLVE_FAST, c->idx16, c->stream,
+                            subBase, flags));
+  return PDX_OK;
+}
+
+int rollCheckpoint(pdx_ctx* c, int seg, bool save) {
+  for (size_t i = 0; i < c->inst.size(); ++i) {
+    Instance& I = c->inst[i];
+    const size_t b3 = I.scene.x.size() * 8;
+    char* ck = c->roll.dbuf + c->roll.ckpt[i] + (size_t)seg * 2 * b3;
+    if (save) {
+      HIPCHECK(c, hipMemcpyAsync(ck, I.dx, b3, hipMemcpyDeviceToDevice, c->stream));
+      HIPCHECK(c, hipMemcpyAsync(ck + b3, I.dv, b3, hipMemcpyDeviceToDevice, c->stream));
+    } else {
+      HIPCHECK(c, hipMemcpyAsync(I.dx, ck, b3, hipMemcpyDeviceToDevice, c->stream));
+      HIPCHECK(c, hipMemcpyAsync(I.dv, ck + b3, b3, hipMemcpyDeviceToDevice, c->stream));
+    }
+  }
+  return PDX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pdx_pin_count(const pdx_ctx* c, int i) {
+  if (!c || i < 0 || i >= (int)c->inst.size()) return PDX_EINVAL;
+  int np = 0;
+  for (uint8_t p : c->inst[(size_t)i].scene.pinned) np += p ? 1 : 0;
+  return np;
+}
+
+int pdx_set_pin_trajectory(pdx_ctx* c, int i, int nsub, const double* traj) {
+  if (!c || i < 0 || i >= (int)c->inst.size() || nsub < 0) return PDX_EINVAL;
+  c->roll.traj.resize(c->inst.size());
+  c->roll.trajSub.resize(c->inst.size(), 0);
+  if (!traj || nsub == 0) {
+    c->roll.traj[(size_t)i].clear();
+    c->roll.trajSub[(size_t)i] = 0;
+    return PDX_OK;
+  }
+  const size_t np = (size_t)pdx_pin_count(c, i);
+  c->roll.traj[(size_t)i].assign(traj, traj + (size_t)nsub * np * 3);
+  c->roll.trajSub[(size_t)i] = nsub;
+  return PDX_OK;
+}
+
+int pdx_rollout(pdx_ctx* c, int nsub, int every) {
+  if (!c || nsub <= 0 || every <= 0) return PDX_EINVAL;
+  int rc = ensureReady(c);
+  if (rc) return rc;
+  const size_t ni = c->inst.size();
+  c->roll.traj.resize(ni);
+  c->roll.trajSub.resize(ni, 0);
+  for (size_t i = 0; i < ni; ++i)
+    if (!c->roll.traj[i].empty() && c->roll.trajSub[i] != nsub)
+      return fail(c, PDX_EINVAL, "pin trajectory of instance " + std::to_string(i) + " holds " +
+                                     std::to_string(c->roll.trajSub[i]) + " substeps, rollout asks " +
+                                     std::to_string(nsub));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  if (c->roll.dbuf) {
+    HIPCHECK(c, hipFree(c->roll.dbuf));
+    c->roll.dbuf = nullptr;
+  }
+  auto& R = c->roll;
+  R.nsub = nsub;
+  R.every = every;
+  R.nseg = (nsub + every - 1) / every;
+  R.forwardDone = false;
+  R.ckpt.assign(ni, 0);
+  R.seedX.resize(ni);
+  R.seedV.resize(ni);
+  // layout: small per-instance tables are staged on the host, big buffers are device-only
+  struct RO {
+    size_t pins, pe, ps, traj, tape, xbar, vbar, cur, bbar, xnbar, gk, gb, gp;
+    std::vector<int> lpins, lpe, lps;
+    std::vector<double> ltraj;
+  };
+  std::vector<RO> ro(ni);
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    off = alignUp(off);
+    const size_t o = off;
+    off += std::max<size_t>(bytes, 8);
+    return o;
+  };
+  for (size_t i = 0; i < ni; ++i) {
+    Instance& I = c->inst[i];
+    RO& o = ro[i];
+    pinLists(I.scene, o.lpins, o.lpe, o.lps);
+    const size_t np = o.lpins.size(), n3 = (size_t)3 * I.scene.n;
+    if (R.traj[i].empty()) {
+      o.ltraj.resize((size_t)nsub * np * 3);
+      for (int st = 0; st < nsub; ++st)
+        for (size_t j = 0; j < np; ++j)
+          for (int d = 0; d < 3; ++d)
+            o.ltraj[((size_t)st * np + j) * 3 + d] = I.scene.pinpos[(size_t)3 * o.lpins[j] + d];
+    } else {
+      o.ltraj = R.traj[i];
+    }
+    o.pins = take(np * 4);
+    o.pe = take(o.lpe.size() * 4);
+    o.ps = take(o.lps.size() * 4);
+    o.traj = take(o.ltraj.size() * 8);
+    R.ckpt[i] = take((size_t)(R.nseg + 1) * 2 * n3 * 8);
+    o.tape = take((size_t)every * (I.cfg.maxIter + 1) * n3 * 8);
+    o.xbar = take(n3 * 8);
+    o.vbar = take(n3 * 8);
+    o.cur = take(n3 * 8);
+    o.bbar = take(n3 * 8);
+    o.xnbar = take(n3 * 8);
+    o.gk = take((size_t)I.scene.E * 8);
+    o.gb = take((size_t)I.scene.B * 8);
+    o.gp = take((size_t)nsub * np * 3 * 8);
+  }
+  const size_t total = alignUp(off);
+  if (hipMalloc(&R.dbuf, total) != hipSuccess) {
+    R.dbuf = nullptr;
+    return fail(c, PDX_ENOMEM, "rollout buffers: " + std::to_string(total) + " bytes");
+  }
+  HIPCHECK(c, hipMemset(R.dbuf, 0, total));
+  char* B = R.dbuf;
+  for (size_t i = 0; i < ni; ++i) {
+    RO& o = ro[i];
+    InstDev& d = c->hInst[i];
+    auto put = [&](size_t at, const void* src, size_t bytes) -> int {
+      if (bytes) HIPCHECK(c, hipMemcpy(B + at, src, bytes, hipMemcpyHostToDevice));
+      return PDX_OK;
+    };
+    if ((rc = put(o.pins, o.lpins.data(), o.lpins.size() * 4))) return rc;
+    if ((rc = put(o.pe, o.lpe.data(), o.lpe.size() * 4))) return rc;
+    if ((rc = put(o.ps, o.lps.data(), o.lps.size() * 4))) return rc;
+    if ((rc = put(o.traj, o.ltraj.data(), o.ltraj.size() * 8))) return rc;
+    d.npin = (int)o.lpins.size();
+    d.npinEdge = (int)(o.lpe.size() / 2);
+    d.npinSten = (int)(o.lps.size() / 4);
+    d.rollK = c->inst[i].cfg.maxIter;
+    d.pinList = (const int*)(B + o.pins);
+    d.pinEdge = (const int*)(B + o.pe);
+    d.pinSten = (const int*)(B + o.ps);
+    d.pinTraj = (const double*)(B + o.traj);
+    d.tape = (double*)(B + o.tape);
+    d.xbar = (double*)(B + o.xbar);
+    d.vbar = (double*)(B + o.vbar);
+    d.cur = (double*)(B + o.cur);
+    d.bbar = (double*)(B + o.bbar);
+    d.xnbar = (double*)(B + o.xnbar);
+    d.gkap = (double*)(B + o.gk);
+    d.gbend = (double*)(B + o.gb);
+    d.gpin = (double*)(B + o.gp);
+  }
+  HIPCHECK(c, hipMemcpy(c->dInst, c->hInst.data(), sizeof(InstDev) * ni, hipMemcpyHostToDevice));
+  R.epoch = c->uploadEpoch;
+  // forward, fixed iterations, checkpoint (x, v) at the start of every segment
+  HIPCHECK(c, hipEventRecord(c->ev0, c->stream));
+  for (int seg = 0; seg < R.nseg; ++seg) {
+    const int s0 = seg * every, len = std::min(every, nsub - s0);
+    if ((rc = rollCheckpoint(c, seg, true))) return rc;
+    if ((rc = rollLaunch(c, len, s0, kRollTraj | kRollFixed))) return rc;
+  }
+  if ((rc = rollCheckpoint(c, R.nseg, true))) return rc;
+  HIPCHECK(c, hipEventRecord(c->ev1, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  float ms = 0.f;
+  HIPCHECK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  R.forwardMs = ms;
+  // the pins stay where the trajectory left them
+  for (auto& I : c->inst) {
+    HIPCHECK(c, hipMemcpy(I.scene.pinpos.data(), I.dpin, I.scene.pinpos.size() * 8, hipMemcpyDeviceToHost));
+    if (I.scene.B > 0) bendingRhs(I.scene, I.brhs);
+  }
+  R.forwardDone = true;
+  return PDX_OK;
+}
+
+int pdx_set_adjoint_seed(pdx_ctx* c, int i, const double* dx, const double* dv) {
+  if (!c || i < 0 || i >= (int)c->inst.size()) return PDX_EINVAL;
+  const size_t n3 = c->inst[(size_t)i].scene.x.size();
+  c->roll.seedX.resize(c->inst.size());
+  c->roll.seedV.resize(c->inst.size());
+  c->roll.seedX[(size_t)i] = dx ? std::vector<double>(dx, dx + n3) : std::vector<double>();
+  c->roll.seedV[(size_t)i] = dv ? std::vector<double>(dv, dv + n3) : std::vector<double>();
+  return PDX_OK;
+}
+
+int pdx_adjoint(pdx_ctx* c) {
+  if (!c) return PDX_EINVAL;
+  auto& R = c->roll;
+  if (!R.forwardDone || R.epoch != c->uploadEpoch || !R.dbuf)
+    return fail(c, PDX_ENOTREADY, "no rollout to differentiate (pdx_rollout first; the context changed since?)");
+  const size_t ni = c->inst.size();
+  R.seedX.resize(ni);
+  R.seedV.resize(ni);
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  for (size_t i = 0; i < ni; ++i) {
+    const InstDev& d = c->hInst[i];
+    const size_t b3 = c->inst[i].scene.x.size() * 8;
+    if (R.seedX[i].empty()) HIPCHECK(c, hipMemset(d.xbar, 0, b3));
+    else HIPCHECK(c, hipMemcpy(d.xbar, R.seedX[i].data(), b3, hipMemcpyHostToDevice));
+    if (R.seedV[i].empty()) HIPCHECK(c, hipMemset(d.vbar, 0, b3));
+    else HIPCHECK(c, hipMemcpy(d.vbar, R.seedV[i].data(), b3, hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemset(d.gkap, 0, std::max<size_t>((size_t)d.E * 8, 8)));
+    if (d.B > 0) HIPCHECK(c, hipMemset(d.gbend, 0, (size_t)d.B * 8));
+    if (d.npin > 0) HIPCHECK(c, hipMemset(d.gpin, 0, (size_t)R.nsub * d.npin * 3 * 8));
+  }
+  int rc = 0;
+  HIPCHECK(c, hipEventRecord(c->ev0, c->stream));
+  for (int seg = R.nseg - 1; seg >= 0; --seg) {
+    const int s0 = seg * R.every, len = std::min(R.every, R.nsub - s0);
+    if ((rc = rollCheckpoint(c, seg, false))) return rc;
+    if ((rc = rollLaunch(c, len, s0, kRollTraj | kRollTape | kRollFixed))) return rc;
+    HIPCHECK(c, launchAdjoint(c->dInst, c->dOrder, (int)ni, len, s0, c->block, c->lds,
+                              c->prec == PDX_PREC_MIXED32, c->mode == PDX_SOLVE_FAST, c->idx16, c->stream));
+  }
+  HIPCHECK(c, hipEventRecord(c->ev1, c->stream));
+  // back to the state after the rollout, pins where the trajectory ended
+  if ((rc = rollCheckpoint(c, R.nseg, false))) return rc;
+  for (auto& I : c->inst) {
+    HIPCHECK(c, hipMemcpyAsync(I.dpin, I.scene.pinpos.data(), I.scene.pinpos.size() * 8, hipMemcpyHostToDevice,
+                               c->stream));
+    if (I.scene.B > 0)
+      HIPCHECK(c, hipMemcpyAsync(I.dbrhs, I.brhs.data(), I.brhs.size() * 8, hipMemcpyHostToDevice, c->stream));
+  }
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  float ms = 0.f;
+  HIPCHECK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  R.adjointMs = ms;
+  return PDX_OK;
+}
+
+int pdx_get_gradients(pdx_ctx* c, int i, double* g_x0, double* g_v0, double* g_kappa, double* g_bend,
+                      double* g_pin) {
+  if (!c || i < 0 || i >= (int)c->inst.size()) return PDX_EINVAL;
+  if (!c->roll.dbuf || c->roll.epoch != c->uploadEpoch) return fail(c, PDX_ENOTREADY, "no adjoint run");
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  const InstDev& d = c->hInst[(size_t)i];
+  const size_t b3 = (size_t)3 * d.n * 8;
+  if (g_x0) HIPCHECK(c, hipMemcpy(g_x0, d.xbar, b3, hipMemcpyDeviceToHost));
+  if (g_v0) HIPCHECK(c, hipMemcpy(g_v0, d.vbar, b3, hipMemcpyDeviceToHost));
+  if (g_kappa && d.E > 0) HIPCHECK(c, hipMemcpy(g_kappa, d.gkap, (size_t)d.E * 8, hipMemcpyDeviceToHost));
+  if (g_bend && d.B > 0) HIPCHECK(c, hipMemcpy(g_bend, d.gbend, (size_t)d.B * 8, hipMemcpyDeviceToHost));
+  if (g_pin && d.npin > 0)
+    HIPCHECK(c, hipMemcpy(g_pin, d.gpin, (size_t)c->roll.nsub * d.npin * 3 * 8, hipMemcpyDeviceToHost));
+  return PDX_OK;
+}
+
+int pdx_rollout_timing(const pdx_ctx* c, double* forward_ms, double* adjoint_ms) {
+  if (!c) return PDX_EINVAL;
+  if (forward_ms) *forward_ms = c->roll.forwardMs;
+  if (adjoint_ms) *adjoint_ms = c->roll.adjointMs;
+  return PDX_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
 
 int pdx_get_topology(pdx_ctx* c, int i, int* color_count, uint32_t* color_of, uint32_t* order,
                      uint32_t* start, uint32_t* vertex_start, uint32_t* vertex_edges,

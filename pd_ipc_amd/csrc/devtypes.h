@@ -71,8 +71,8 @@ struct InstDev {
   // scene
   const double* mass;
   const uint8_t* pinned;
-  const double* pinpos;
-  const double* brhs;  // bending constant rhs (3n) or null
+  double* pinpos;      // pin targets (3n); rewritten per substep along a pin trajectory
+  double* brhs;        // bending constant rhs (3n) or null; follows the pins
   const int* ea;
   const int* eb;
   const double* rest;
@@ -108,6 +108,21 @@ struct InstDev {
   SweepDev fwd, bwd;
   BlockSweepDev bfwd, bbwd;
   void* Wg;           // global W scratch (3m Real) when W is not LDS-resident
+  // differentiable rollout (null / 0 outside it)
+  int npin, npinEdge, npinSten, rollK;
+  const int* pinList;      // pinned vertices, ascending
+  const int* pinEdge;      // per edge with exactly one pinned end: {edge, pin index}
+  const int* pinSten;      // per stencil with a pinned vertex: {stencil, pin index or -1 x 3}
+  const double* pinTraj;   // [substep][pin][3] pin targets
+  double* tape;            // [substep of the segment][iterate 0..K][3n] FP64 iterates
+  double* xbar;            // adjoint of x at the current substep boundary (3n)
+  double* vbar;            // adjoint of v (3n)
+  double* cur;             // adjoint of the current iterate (3n)
+  double* bbar;            // adjoint of the substep's constant right-hand side (3n)
+  double* xnbar;           // adjoint of x_n being accumulated (3n)
+  double* gkap;            // dLoss/dkappa_e (E)
+  double* gbend;           // dLoss/dk_s (B)
+  double* gpin;            // dLoss/dq [substep][pin][3]
   // statistics
   int* istats;        // [0] iterations used, [1] converged, [2] early exits
   long long* lstats;  // [0] total iterations, [1] substeps, [2..6] phase ticks

@@ -696,21 +696,82 @@ __device__ void runBackward(const InstDev& I, Real* W, SweepTimes& tst) {
     sweep<Real, false>(I.bwd, W, dinv, put);
 }
 
-template <typename Real, bool kExact, bool kBlocked, bool kIdx16>
+// Pin targets of substep s from the instance's pin trajectory, and the bending
+// constant right-hand side that follows them (Assembler.cpp:94-137 restricted to
+// the stencils that hold a pinned vertex, in stencil order, by one thread, so the
+// sums are those of the host assembly).
+__device__ void applyPinTrajectory(const InstDev& I, int s) {
+  const double* q = I.pinTraj + (size_t)s * I.npin * 3;
+  for (int j = threadIdx.x; j < I.npin; j += blockDim.x) {
+    const int v = I.pinList[j];
+    I.pinpos[3 * v] = q[3 * j];
+    I.pinpos[3 * v + 1] = q[3 * j + 1];
+    I.pinpos[3 * v + 2] = q[3 * j + 2];
+  }
+  __syncthreads();
+  if (I.brhs && threadIdx.x == 0) {
+    const double w[3] = {1.0, -2.0, 1.0};
+    for (int t = 0; t < I.npinSten; ++t) {
+      const int* ps = I.pinSten + 4 * t;
+      const int st = ps[0];
+      const int vv[3] = {I.sa[st], I.sb[st], I.sc[st]};
+      for (int k = 0; k < 3; ++k)
+        if (ps[1 + k] < 0)
+          for (int d = 0; d < 3; ++d) I.brhs[3 * vv[k] + d] = 0.0;
+    }
+    for (int t = 0; t < I.npinSten; ++t) {
+      const int* ps = I.pinSten + 4 * t;
+      const int st = ps[0];
+      if (I.bk[st] == 0.0) continue;
+      const int vv[3] = {I.sa[st], I.sb[st], I.sc[st]};
+      double c0 = 0.0, c1 = 0.0, c2 = 0.0;
+      for (int k = 0; k < 3; ++k) {
+        if (ps[1 + k] < 0) continue;
+        const double* p = I.pinpos + 3 * vv[k];
+        c0 += p[0] * w[k];
+        c1 += p[1] * w[k];
+        c2 += p[2] * w[k];
+      }
+      for (int k = 0; k < 3; ++k) {
+        if (ps[1 + k] >= 0) continue;
+        const double f = -I.bk[st] * w[k];
+        double* slot = I.brhs + 3 * vv[k];
+        slot[0] += c0 * f;
+        slot[1] += c1 * f;
+        slot[2] += c2 * f;
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// Launch flags of the differentiable rollout.
+constexpr int kRollTraj = 1;   // pin targets of each substep from the pin trajectory
+constexpr int kRollTape = 2;   // record every iterate of every substep (FP64)
+constexpr int kRollFixed = 4;  // fixed iteration count: no residual, no stop test
+
+template <typename Real, bool kExact, bool kBlocked, bool kIdx16, bool kRoll>
 __global__ void __launch_bounds__(512) substepKernel(const InstDev* __restrict__ insts,
-                                                      const int* __restrict__ order, int nsub) {
+                                                      const int* __restrict__ order, int nsub,
+                                                      int subBase, int flags) {
+  if (!kRoll) flags = 0;
   extern __shared__ double smem[];
   const InstDev& I = insts[order[blockIdx.x]];
   double* red = smem;  // 2 x 16 slots
   Real* W = I.ldsW ? reinterpret_cast<Real*>(smem + 32) : static_cast<Real*>(I.Wg);
   const int n = I.n;
   const int tid = threadIdx.x, bs = blockDim.x;
+  const bool fixedIter = kRoll && (flags & kRollFixed) != 0;
+  const bool wantRes = I.wantResidual && !fixedIter;
+  const size_t n3 = (size_t)3 * n;
 
   if (kBlocked) loadStepTables(I);  // visible after the first barrier below
   long long tk[4] = {0, 0, 0, 0};
   long long tacc[5] = {0, 0, 0, 0, 0};
   SweepTimes swt;
   for (int sub = 0; sub < nsub; ++sub) {
+    if (kRoll && (flags & kRollTraj)) applyPinTrajectory(I, subBase + sub);
+    double* tape = (kRoll && (flags & kRollTape)) ? I.tape + (size_t)sub * (I.maxIter + 1) * n3 : nullptr;
     const long long tp0 = wall_clock64();
     // predict (Integrator.cpp:447-463) and the displacement scale
     double lscale = 0.0;
@@ -718,6 +779,7 @@ __global__ void __launch_bounds__(512) substepKernel(const InstDev* __restrict__
       for (int d = 0; d < 3; ++d) {
         const double p = I.x[3 * v + d];
         I.xbs[3 * v + d] = p;
+        if (tape) tape[3 * v + d] = p;
         const double xh = p + I.vel[3 * v + d] * I.h + I.gh2[d];
         I.xhat[3 * v + d] = xh;
         I.prev[3 * v + d] = p;
@@ -761,9 +823,15 @@ __global__ void __launch_bounds__(512) substepKernel(const InstDev* __restrict__
       }
       if (tid == 0) tk[2] = wall_clock64();
       const bool more = (k + 1 < I.maxIter);
+      double* tapeK = tape ? tape + (size_t)(k + 1) * n3 : nullptr;
       double ldiff = 0.0, lres = 0.0;
       for (int v = tid; v < n; v += bs) {
         const double x0 = I.x[3 * v], x1 = I.x[3 * v + 1], x2 = I.x[3 * v + 2];
+        if (tapeK) {
+          tapeK[3 * v] = x0;
+          tapeK[3 * v + 1] = x1;
+          tapeK[3 * v + 2] = x2;
+        }
         const double d = fmax(fabs(x0 - I.prev[3 * v]),
                               fmax(fabs(x1 - I.prev[3 * v + 1]), fabs(x2 - I.prev[3 * v + 2])));
         ldiff = fmax(ldiff, d);
@@ -773,7 +841,7 @@ __global__ void __launch_bounds__(512) substepKernel(const InstDev* __restrict__
         const bool pinnedV = I.pinned[v] != 0;
         const int r = I.vrank[v];
         if (kExact) {
-          if (I.wantResidual && !pinnedV && I.mass[v] > 0.0) lres = fmax(lres, residualAndRhs(I, v, nullptr));
+          if (wantRes && !pinnedV && I.mass[v] > 0.0) lres = fmax(lres, residualAndRhs(I, v, nullptr));
           if (more) {
             const Acc3 b = rhsExact(I, v);
             putW(W, r, b.x, b.y, b.z);
@@ -782,12 +850,20 @@ __global__ void __launch_bounds__(512) substepKernel(const InstDev* __restrict__
           if (pinnedV) {
             if (more) putW(W, r, I.pinpos[3 * v] - x0, I.pinpos[3 * v + 1] - x1, I.pinpos[3 * v + 2] - x2);
           } else {
-            Acc3 rh;
-            const double rv = residualAndRhs(I, v, more ? &rh : nullptr);
-            if (I.wantResidual && I.mass[v] > 0.0) lres = fmax(lres, rv);
-            if (more) putW(W, r, rh.x, rh.y, rh.z);
+            if (more || wantRes) {
+              Acc3 rh;
+              const double rv = residualAndRhs(I, v, more ? &rh : nullptr);
+              if (wantRes && I.mass[v] > 0.0) lres = fmax(lres, rv);
+              if (more) putW(W, r, rh.x, rh.y, rh.z);
+            }
           }
         }
+      }
+      if (fixedIter) {
+        // the next iteration's right-hand side is in W; a barrier orders it
+        __syncthreads();
+        used = k + 1;
+        continue;
       }
       // criterion (Integrator.cpp:673-751); two slot sets alternate so one barrier suffices
       double* rs = red + 16 * ((k & 1) ^ 1);
@@ -849,6 +925,225 @@ __global__ void __launch_bounds__(512) substepKernel(const InstDev* __restrict__
   }
 }
 
+// Reverse mode of `nsub` substeps (global indices subBase ..), one workgroup per
+// instance, from the iterates recorded by the substep kernel (kRollTape) and the
+// adjoint (xbar, vbar) of the state after the last of them; leaves the adjoint of
+// the state before the first.  Per substep, in reverse: velocity/re-pin, then for
+// k = K-1 .. 0 with L x^{k+1} = c + P(x^k):
+//   mu = L^{-1} cur         (the same factor: L is symmetric, so the transposed
+//                            solve is the forward + backward sweep again)
+//   bbar += mu
+//   cur  = J_P(x^k)^T mu    (gather over the vertex's edges in edge order)
+//   gkap_e += mu . dP/dkappa_e - mu^T (dL/dkappa_e) x^{k+1}   (one thread per edge)
+//   gbend_s += -mu^T (dL/dk_s) x^{k+1}                        (one thread per stencil)
+// then bbar feeds x_n, v_n (prediction), the pins and the stiffnesses.  Every
+// sum has a fixed order; nothing is accumulated atomically.
+template <typename Real, bool kBlocked, bool kIdx16>
+__global__ void __launch_bounds__(512) adjointKernel(const InstDev* __restrict__ insts,
+                                                      const int* __restrict__ order, int nsub,
+                                                      int subBase) {
+  extern __shared__ double smem[];
+  const InstDev& I = insts[order[blockIdx.x]];
+  Real* W = I.ldsW ? reinterpret_cast<Real*>(smem + 32) : static_cast<Real*>(I.Wg);
+  const int n = I.n, K = I.maxIter;
+  const int tid = threadIdx.x, bs = blockDim.x;
+  const size_t n3 = (size_t)3 * n;
+  const double cv = I.dampInvH;
+  const double w3[3] = {1.0, -2.0, 1.0};
+  const Real* dinv = static_cast<const Real*>(I.dinv);
+  auto none = [](int, Real, Real, Real) {};
+  auto muOf = [&](int v, int d) { return (double)W[3 * I.vrank[v] + d]; };
+  if (kBlocked) loadStepTables(I);
+  SweepTimes swt;
+  for (int sub = nsub - 1; sub >= 0; --sub) {
+    const int s = subBase + sub;
+    const double* q = I.pinTraj + (size_t)s * I.npin * 3;
+    double* gq = I.gpin + (size_t)s * I.npin * 3;
+    // velocity / damping / re-pin (Integrator.cpp:774-791)
+    for (int v = tid; v < n; v += bs) {
+      const bool pv = I.pinned[v] != 0;
+      for (int d = 0; d < 3; ++d) {
+        const size_t i = (size_t)3 * v + d;
+        I.bbar[i] = 0.0;
+        if (pv) {
+          I.cur[i] = 0.0;
+          I.xnbar[i] = 0.0;
+        } else {
+          I.cur[i] = I.xbar[i] + cv * I.vbar[i];
+          I.xnbar[i] = -cv * I.vbar[i];
+        }
+      }
+    }
+    if (tid == 0)
+      for (int j = 0; j < I.npin; ++j)
+        for (int d = 0; d < 3; ++d) gq[3 * j + d] += I.xbar[(size_t)3 * I.pinList[j] + d];
+    __syncthreads();
+    const double* tp = I.tape + (size_t)sub * (K + 1) * n3;
+    for (int k = K - 1; k >= 0; --k) {
+      for (int v = tid; v < n; v += bs)
+        putW(W, I.vrank[v], I.cur[3 * v], I.cur[3 * v + 1], I.cur[3 * v + 2]);
+      __syncthreads();
+      if (kBlocked) {
+        blockSweep<Real, true, kIdx16>(I.bfwd, W, ldsAt<Real>(I.partOff), ldsAt<Real>(I.denseOff),
+                                       ldsAt<int>(I.stepOff), dinv, none, swt, I.ldsW != 0);
+        blockSweep<Real, false, kIdx16>(I.bbwd, W, ldsAt<Real>(I.partOff), ldsAt<Real>(I.denseOff),
+                                        ldsAt<int>(I.stepOff) + 3 * (I.bfwd.nstep + 1), dinv, none, swt,
+                                        I.ldsW != 0);
+      } else {
+        sweep<Real, true>(I.fwd, W, dinv, none);
+        sweep<Real, false>(I.bwd, W, dinv, none);
+      }
+      __syncthreads();
+      const double* xk = tp + (size_t)k * n3;
+      const double* xk1 = tp + (size_t)(k + 1) * n3;
+      // vertices: bbar += mu, cur = J_P^T mu (the edges' terms in edge order)
+      for (int v = tid; v < n; v += bs) {
+        const bool vP = I.pinned[v] != 0;
+        const double mv0 = muOf(v, 0), mv1 = muOf(v, 1), mv2 = muOf(v, 2);
+        I.bbar[3 * v] += mv0;
+        I.bbar[3 * v + 1] += mv1;
+        I.bbar[3 * v + 2] += mv2;
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+        const double xv0 = xk[3 * v], xv1 = xk[3 * v + 1], xv2 = xk[3 * v + 2];
+        for (int p = I.vstart[v]; p < I.vstart[v + 1]; ++p) {
+          const int rc = I.recE[p];
+          const int u = rc >> 2;
+          const bool uP = (rc & 2) != 0, isB = (rc & 1) != 0;
+          const bool aP = isB ? uP : vP, bP = isB ? vP : uP;
+          if (aP && bP) continue;
+          const double u0 = xk[3 * u], u1 = xk[3 * u + 1], u2 = xk[3 * u + 2];
+          const double rx = isB ? u0 - xv0 : xv0 - u0;
+          const double ry = isB ? u1 - xv1 : xv1 - u1;
+          const double rz = isB ? u2 - xv2 : xv2 - u2;
+          const double len = sqrt(rx * rx + ry * ry + rz * rz);
+          if (!(len > kMinLength)) continue;
+          const double k = I.kapE[p], l = I.restE[p];
+          const double mu0 = uP ? 0.0 : muOf(u, 0), mu1 = uP ? 0.0 : muOf(u, 1), mu2 = uP ? 0.0 : muOf(u, 2);
+          const double me0 = vP ? 0.0 : mv0, me1 = vP ? 0.0 : mv1, me2 = vP ? 0.0 : mv2;
+          const double ma0 = isB ? mu0 : me0, ma1 = isB ? mu1 : me1, ma2 = isB ? mu2 : me2;
+          const double mb0 = isB ? me0 : mu0, mb1 = isB ? me1 : mu1, mb2 = isB ? me2 : mu2;
+          const double g0 = (ma0 - mb0) * k, g1 = (ma1 - mb1) * k, g2 = (ma2 - mb2) * k;
+          const double h0 = rx / len, h1 = ry / len, h2 = rz / len;
+          const double t = h0 * g0 + h1 * g1 + h2 * g2;
+          const double sc = l / len;
+          const double r0 = (g0 - h0 * t) * sc, r1 = (g1 - h1 * t) * sc, r2 = (g2 - h2 * t) * sc;
+          if (!isB) {
+            a0 += r0;
+            a1 += r1;
+            a2 += r2;
+          } else {
+            a0 -= r0;
+            a1 -= r1;
+            a2 -= r2;
+          }
+        }
+        I.cur[3 * v] = a0;
+        I.cur[3 * v + 1] = a1;
+        I.cur[3 * v + 2] = a2;
+      }
+      // edges: dLoss/dkappa_e
+      for (int e = tid; e < I.E; e += bs) {
+        const int a = I.ea[e], b = I.eb[e];
+        const bool aP = I.pinned[a] != 0, bP = I.pinned[b] != 0;
+        if (aP && bP) continue;
+        const double k = I.kappa[e], l = I.rest[e];
+        const double rx = xk[3 * a] - xk[3 * b], ry = xk[3 * a + 1] - xk[3 * b + 1],
+                     rz = xk[3 * a + 2] - xk[3 * b + 2];
+        const double len = sqrt(rx * rx + ry * ry + rz * rz);
+        const double ma0 = aP ? 0.0 : muOf(a, 0), ma1 = aP ? 0.0 : muOf(a, 1), ma2 = aP ? 0.0 : muOf(a, 2);
+        const double mb0 = bP ? 0.0 : muOf(b, 0), mb1 = bP ? 0.0 : muOf(b, 1), mb2 = bP ? 0.0 : muOf(b, 2);
+        const double dm0 = ma0 - mb0, dm1 = ma1 - mb1, dm2 = ma2 - mb2;
+        double g = I.gkap[e];
+        if (len > kMinLength) {
+          const double sc = l / len;
+          g += dm0 * (rx * sc) + dm1 * (ry * sc) + dm2 * (rz * sc);
+        }
+        if (!aP && !bP) {
+          g -= dm0 * (xk1[3 * a] - xk1[3 * b]) + dm1 * (xk1[3 * a + 1] - xk1[3 * b + 1]) +
+               dm2 * (xk1[3 * a + 2] - xk1[3 * b + 2]);
+        } else if (aP) {
+          g -= mb0 * xk1[3 * b] + mb1 * xk1[3 * b + 1] + mb2 * xk1[3 * b + 2];
+        } else {
+          g -= ma0 * xk1[3 * a] + ma1 * xk1[3 * a + 1] + ma2 * xk1[3 * a + 2];
+        }
+        (void)k;
+        I.gkap[e] = g;
+      }
+      // stencils: dLoss/dk_s
+      for (int t = tid; t < I.B; t += bs) {
+        const int vv[3] = {I.sa[t], I.sb[t], I.sc[t]};
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, x0 = 0.0, x1 = 0.0, x2 = 0.0;
+        for (int p = 0; p < 3; ++p) {
+          const int u = vv[p];
+          if (I.pinned[u]) continue;
+          s0 += muOf(u, 0) * w3[p];
+          s1 += muOf(u, 1) * w3[p];
+          s2 += muOf(u, 2) * w3[p];
+          x0 += xk1[3 * u] * w3[p];
+          x1 += xk1[3 * u + 1] * w3[p];
+          x2 += xk1[3 * u + 2] * w3[p];
+        }
+        I.gbend[t] -= s0 * x0 + s1 * x1 + s2 * x2;
+      }
+      __syncthreads();
+    }
+    // prediction x^ = x_n + v_n h + g h^2 feeds the inertial term (m/h^2) x^
+    for (int v = tid; v < n; v += bs) {
+      const bool pv = I.pinned[v] != 0;
+      const double mh = I.mass[v] * I.invH2;
+      for (int d = 0; d < 3; ++d) {
+        const size_t i = (size_t)3 * v + d;
+        if (pv) {
+          I.xbar[i] = I.xnbar[i] + I.cur[i];
+          I.vbar[i] = 0.0;
+        } else {
+          const double xh = mh * I.bbar[i];
+          I.xbar[i] = I.xnbar[i] + (I.cur[i] + xh);
+          I.vbar[i] = xh * I.h;
+        }
+      }
+    }
+    // pins: their rows, the compensation kappa q of free neighbours, the bend constant
+    if (tid == 0) {
+      for (int j = 0; j < I.npin; ++j)
+        for (int d = 0; d < 3; ++d) gq[3 * j + d] += I.bbar[(size_t)3 * I.pinList[j] + d];
+      for (int t = 0; t < I.npinEdge; ++t) {
+        const int e = I.pinEdge[2 * t], j = I.pinEdge[2 * t + 1];
+        const int a = I.ea[e], b = I.eb[e];
+        const int f = I.pinned[a] ? b : a;
+        double g = I.gkap[e];
+        for (int d = 0; d < 3; ++d) {
+          const double bb = I.bbar[(size_t)3 * f + d];
+          gq[3 * j + d] += I.kappa[e] * bb;
+          g += bb * q[3 * j + d];
+        }
+        I.gkap[e] = g;
+      }
+      for (int t = 0; t < I.npinSten; ++t) {
+        const int* ps = I.pinSten + 4 * t;
+        const int st = ps[0];
+        const int vv[3] = {I.sa[st], I.sb[st], I.sc[st]};
+        double C0 = 0.0, C1 = 0.0, C2 = 0.0;
+        for (int r = 0; r < 3; ++r)
+          if (ps[1 + r] >= 0) {
+            C0 += w3[r] * q[3 * ps[1 + r]];
+            C1 += w3[r] * q[3 * ps[1 + r] + 1];
+            C2 += w3[r] * q[3 * ps[1 + r] + 2];
+          }
+        for (int p = 0; p < 3; ++p) {
+          if (ps[1 + p] >= 0) continue;
+          const double* bb = I.bbar + (size_t)3 * vv[p];
+          I.gbend[st] -= w3[p] * (bb[0] * C0 + bb[1] * C1 + bb[2] * C2);
+          for (int r = 0; r < 3; ++r)
+            if (ps[1 + r] >= 0)
+              for (int d = 0; d < 3; ++d) gq[3 * ps[1 + r] + d] -= I.bk[st] * w3[p] * w3[r] * bb[d];
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // Stand-alone global solve x = L^{-1} b (EigenDirectSolver::solve) on one instance.
 template <typename Real, bool kBlocked, bool kIdx16>
 __global__ void __launch_bounds__(512) solveKernel(const InstDev* __restrict__ insts, int inst,
@@ -890,11 +1185,30 @@ __global__ void __launch_bounds__(512) solveKernel(const InstDev* __restrict__ i
   X(double, true, true, true) X(double, true, true, false) X(double, true, false, false)
 
 hipError_t launchSubstep(const InstDev* d, const int* order, int ninst, int nsub, int block,
+                         size_t lds, bool fp32, bool blocked, bool idx16, hipStream_t s, int subBase,
+                         int flags) {
+#define PDX_L(R, E, B, X16)                                                                            \
+  if (fp32 == (sizeof(R) == 4) && blocked == B && (idx16 && blocked) == X16) {                         \
+    if (flags)                                                                                         \
+      hipLaunchKernelGGL((substepKernel<R, E, B, X16, true>), dim3(ninst), dim3(block), lds, s, d,     \
+                         order, nsub, subBase, flags);                                                 \
+    else                                                                                               \
+      hipLaunchKernelGGL((substepKernel<R, E, B, X16, false>), dim3(ninst), dim3(block), lds, s, d,    \
+                         order, nsub, subBase, 0);                                                     \
+    return hipGetLastError();                                                                          \
+  }
+  PDX_KERNEL_TABLE(PDX_L)
+#undef PDX_L
+  return hipErrorInvalidValue;
+}
+
+hipError_t launchAdjoint(const InstDev* d, const int* order, int ninst, int nsub, int subBase, int block,
                          size_t lds, bool fp32, bool blocked, bool idx16, hipStream_t s) {
-#define PDX_L(R, E, B, X16)                                                                             \
-  if (fp32 == (sizeof(R) == 4) && blocked == B && (idx16 && blocked) == X16) {                          \
-    hipLaunchKernelGGL((substepKernel<R, E, B, X16>), dim3(ninst), dim3(block), lds, s, d, order, nsub); \
-    return hipGetLastError();                                                                           \
+#define PDX_L(R, E, B, X16)                                                                           \
+  if (fp32 == (sizeof(R) == 4) && blocked == B && (idx16 && blocked) == X16) {                        \
+    hipLaunchKernelGGL((adjointKernel<R, B, X16>), dim3(ninst), dim3(block), lds, s, d, order, nsub, \
+                       subBase);                                                                      \
+    return hipGetLastError();                                                                         \
   }
   PDX_KERNEL_TABLE(PDX_L)
 #undef PDX_L
@@ -916,10 +1230,16 @@ hipError_t launchSolve(const InstDev* d, int inst, const double* b, double* x, i
 hipError_t setLdsLimit(size_t bytes) {
   hipError_t e = hipSuccess;
 #define PDX_L(R, E, B, X16)                                                                                 \
-  e = hipFuncSetAttribute((const void*)substepKernel<R, E, B, X16>, hipFuncAttributeMaxDynamicSharedMemorySize, \
-                          (int)bytes);                                                                     \
+  e = hipFuncSetAttribute((const void*)substepKernel<R, E, B, X16, false>,                                \
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);                         \
+  if (e != hipSuccess) return e;                                                                           \
+  e = hipFuncSetAttribute((const void*)substepKernel<R, E, B, X16, true>,                                 \
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);                         \
   if (e != hipSuccess) return e;                                                                           \
   e = hipFuncSetAttribute((const void*)solveKernel<R, B, X16>, hipFuncAttributeMaxDynamicSharedMemorySize,     \
+                          (int)bytes);                                                                     \
+  if (e != hipSuccess) return e;                                                                           \
+  e = hipFuncSetAttribute((const void*)adjointKernel<R, B, X16>, hipFuncAttributeMaxDynamicSharedMemorySize,   \
                           (int)bytes);                                                                     \
   if (e != hipSuccess) return e;
   PDX_KERNEL_TABLE(PDX_L)

@@ -84,11 +84,19 @@ def load_library(path=LIB_PATH):
         "pdx_set_state": (I, [P, I, P, P]),
         "pdx_set_pin_positions": (I, [P, I, P]),
         "pdx_set_edge_kappa": (I, [P, I, P]),
+        "pdx_set_bend_k": (I, [P, I, P]),
         "pdx_set_config": (I, [P, I, C.POINTER(_Config)]),
         "pdx_get_stats": (I, [P, I, C.POINTER(_Stats)]),
         "pdx_solve3": (I, [P, I, P, P]),
         "pdx_get_factor": (I, [P, I, C.POINTER(I), C.POINTER(C.c_longlong), P, P, P, P, P, P]),
         "pdx_get_topology": (I, [P, I, C.POINTER(I), P, P, P, P, P, P, P]),
+        "pdx_pin_count": (I, [P, I]),
+        "pdx_set_pin_trajectory": (I, [P, I, I, P]),
+        "pdx_rollout": (I, [P, I, I]),
+        "pdx_set_adjoint_seed": (I, [P, I, P, P]),
+        "pdx_adjoint": (I, [P]),
+        "pdx_get_gradients": (I, [P, I, P, P, P, P, P]),
+        "pdx_rollout_timing": (I, [P, C.POINTER(D), C.POINTER(D)]),
         "pdx_last_kernel_ms": (D, [P]),
         "pdx_set_schedule": (I, [P, I, I]),
         "pdx_bytes_per_iteration": (D, [P]),
@@ -276,6 +284,10 @@ class Engine:
         k = np.ascontiguousarray(kappa, np.float64)
         self._check(self.lib.pdx_set_edge_kappa(self.h, i, _ptr(k)), "set_edge_kappa")
 
+    def set_bend_k(self, i, k):
+        k = np.ascontiguousarray(k, np.float64)
+        self._check(self.lib.pdx_set_bend_k(self.h, i, _ptr(k)), "set_bend_k")
+
     def set_config(self, i, cfg):
         c = cfg._c()
         self._check(self.lib.pdx_set_config(self.h, i, C.byref(c)), "set_config")
@@ -325,6 +337,53 @@ class Engine:
             "get_topology")
         out["color_count"] = ncol.value
         return out
+
+    # --- differentiable rollout -------------------------------------------------
+    def pin_count(self, i=0):
+        return self.lib.pdx_pin_count(self.h, i)
+
+    def set_pin_trajectory(self, i, traj):
+        """traj: (nsub, pins, 3) pin targets per substep (pins in ascending vertex
+        order), or None to clear."""
+        if traj is None:
+            self._check(self.lib.pdx_set_pin_trajectory(self.h, i, 0, None), "set_pin_trajectory")
+            return
+        t = np.ascontiguousarray(traj, np.float64)
+        assert t.ndim == 3 and t.shape[1] == self.pin_count(i) and t.shape[2] == 3
+        self._check(self.lib.pdx_set_pin_trajectory(self.h, i, t.shape[0], _ptr(t)), "set_pin_trajectory")
+
+    def rollout(self, nsub, checkpoint_every=10):
+        self._check(self.lib.pdx_rollout(self.h, nsub, checkpoint_every), "rollout")
+        self._roll_nsub = nsub
+
+    def set_adjoint_seed(self, i, dx, dv=None):
+        self._seeds = getattr(self, "_seeds", {})
+        dx = np.ascontiguousarray(dx, np.float64).reshape(-1)
+        dv = None if dv is None else np.ascontiguousarray(dv, np.float64).reshape(-1)
+        self._seeds[i] = (dx, dv)
+        self._check(self.lib.pdx_set_adjoint_seed(self.h, i, _ptr(dx), None if dv is None else _ptr(dv)),
+                    "set_adjoint_seed")
+
+    def adjoint(self):
+        self._check(self.lib.pdx_adjoint(self.h), "adjoint")
+
+    def gradients(self, i=0):
+        c = self.cloths[i]
+        n, E, B, npin = c.n, len(c.edge_a), len(c.bend_a), self.pin_count(i)
+        out = dict(x0=np.empty(3 * n), v0=np.empty(3 * n), kappa=np.empty(max(E, 1)),
+                   bend=np.empty(max(B, 1)), pin=np.empty(max(self._roll_nsub * npin * 3, 1)))
+        self._check(self.lib.pdx_get_gradients(self.h, i, _ptr(out["x0"]), _ptr(out["v0"]),
+                                               _ptr(out["kappa"]), _ptr(out["bend"]), _ptr(out["pin"])),
+                    "get_gradients")
+        out["kappa"] = out["kappa"][:E]
+        out["bend"] = out["bend"][:B]
+        out["pin"] = out["pin"][:self._roll_nsub * npin * 3].reshape(self._roll_nsub, npin, 3)
+        return out
+
+    def rollout_timing(self):
+        f, a = C.c_double(), C.c_double()
+        self._check(self.lib.pdx_rollout_timing(self.h, C.byref(f), C.byref(a)), "rollout_timing")
+        return f.value, a.value
 
     def set_schedule(self, block_threads, chain_cap=32):
         self._check(self.lib.pdx_set_schedule(self.h, block_threads, chain_cap), "set_schedule")
