@@ -109,6 +109,115 @@ def measured_traffic(precision, instances, seed, solve):
     return None
 
 
+def _set_kappa(e, i, c, scale):
+    e.set_edge_kappa(i, c.edge_kappa * scale)
+    if len(c.bend_k):
+        e.set_bend_k(i, c.bend_k * scale)
+
+
+def _rollout_batch(cloths, cfg, nsub, every, precision, trajs, kappas=None, v0s=None):
+    e = Engine(precision, "fast")
+    for i, c in enumerate(cloths):
+        e.add(c, cfg)
+        e.set_pin_trajectory(i, trajs[i])
+    for i, c in enumerate(cloths):
+        if kappas is not None:
+            _set_kappa(e, i, c, kappas[i] / c.meta["kappa"])
+        if v0s is not None:
+            e.set_state(i, c.positions, v0s[i])
+    e.factor()
+    e.rollout(nsub, every)
+    return e
+
+
+def adjoint_workload(args):
+    """Differentiable batch: 64 x (128 x 128) cloths, 120 substeps, 20 fixed PD
+    iterations, checkpoints every 10 substeps.  Targets come from an FP64 run with
+    seeded hidden stiffness and pin trajectories; gradients of sum |x_T - target|^2
+    w.r.t. per-instance kappa, v0 and the pin trajectory are taken at the starting
+    guess (kappa 2300, v0 = 0, static pins) in mixed precision (FP32 solves) and in
+    FP64, and checked against central differences of the FP64 forward."""
+    cloths, cfg, nsub, hidden = scenes.config5(count=args.adj_instances)
+    every = 10
+    ni = len(cloths)
+    log(f"adjoint workload: {ni} instances x {cloths[0].n} vertices, {nsub} substeps")
+    t0 = time.perf_counter()
+    eh = _rollout_batch(cloths, cfg, nsub, every, "fp64", [h["traj"] for h in hidden],
+                        kappas=[h["kappa"] for h in hidden])
+    targets = [eh.state(i)[0] for i in range(ni)]
+    eh.close()
+    log(f"targets {time.perf_counter() - t0:.1f}s")
+    static = [scenes.pin_trajectory(c, nsub, cfg.dt, amplitude=0.0) for c in cloths]
+
+    def grads(precision):
+        e = _rollout_batch(cloths, cfg, nsub, every, precision, static)
+        fwd_ms, _ = e.rollout_timing()
+        loss = 0.0
+        xs = []
+        for i in range(ni):
+            x, _ = e.state(i)
+            xs.append(x)
+            loss += float(np.sum((x - targets[i]) ** 2))
+            e.set_adjoint_seed(i, 2.0 * (x - targets[i]))
+        e.adjoint()
+        _, adj_ms = e.rollout_timing()
+        g = [e.gradients(i) for i in range(ni)]
+        e.close()
+        gk = np.array([gi["kappa"] @ (c.edge_kappa / c.meta["kappa"]) + gi["bend"] @ (c.bend_k / c.meta["kappa"])
+                       for gi, c in zip(g, cloths)])
+        return dict(loss=loss, xs=xs, fwd_ms=fwd_ms, adj_ms=adj_ms, gk=gk,
+                    gv=np.stack([gi["v0"] for gi in g]), gp=np.stack([gi["pin"] for gi in g]))
+
+    g32 = grads("mixed32")
+    log(f"mixed32: forward {g32['fwd_ms']:.0f} ms, adjoint {g32['adj_ms']:.0f} ms, loss {g32['loss']:.6e}")
+    g64 = grads("fp64")
+    log(f"fp64: forward {g64['fwd_ms']:.0f} ms, adjoint {g64['adj_ms']:.0f} ms, loss {g64['loss']:.6e}")
+    rel = lambda a, b: float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+    xerr = max(float(np.max(np.abs(a - b))) for a, b in zip(g32["xs"], g64["xs"]))
+
+    def loss64(kappas=None, v0s=None, trajs=None):
+        e = _rollout_batch(cloths, cfg, nsub, every, "fp64", trajs if trajs is not None else static,
+                           kappas=kappas, v0s=v0s)
+        val = sum(float(np.sum((e.state(i)[0] - targets[i]) ** 2)) for i in range(ni))
+        e.close()
+        return val
+
+    rng = np.random.default_rng(11)
+    eps = 1e-6
+    dv = rng.standard_normal((ni, 3 * cloths[0].n))
+    for i, c in enumerate(cloths):
+        dv[i][np.repeat(c.pinned.astype(bool), 3)] = 0.0
+    fd_v = (loss64(v0s=eps * dv) - loss64(v0s=-eps * dv)) / (2 * eps)
+    dq = rng.standard_normal(g64["gp"].shape)
+    fd_q = (loss64(trajs=[s + eps * d for s, d in zip(static, dq)]) -
+            loss64(trajs=[s - eps * d for s, d in zip(static, dq)])) / (2 * eps)
+    ks = 1e-3  # relative step on the stiffness (see the adjoint tests)
+    dk = rng.standard_normal(ni)
+    k0 = np.array([c.meta["kappa"] for c in cloths])
+    fd_k = (loss64(kappas=k0 * (1 + ks * dk)) - loss64(kappas=k0 * (1 - ks * dk))) / (2 * ks)
+    ad_v = float(np.sum(g64["gv"] * dv))
+    ad_q = float(np.sum(g64["gp"] * dq))
+    ad_k = float(np.sum(g64["gk"] * k0 * dk))
+    ad32 = (float(np.sum(g32["gv"] * dv)), float(np.sum(g32["gp"] * dq)), float(np.sum(g32["gk"] * k0 * dk)))
+    fd = {"v0": [fd_v, ad_v, ad32[0]], "pins": [fd_q, ad_q, ad32[1]], "kappa": [fd_k, ad_k, ad32[2]]}
+    log("fd check (fd, adjoint fp64, adjoint mixed32): " + json.dumps(fd))
+    sub_total = ni * nsub
+    out = {
+        "workload": f"differentiable batch: {ni} x {cloths[0].n}-vertex cloths (128x128, distance+shear+bend), "
+                    f"{nsub} substeps, {cfg.max_iterations} PD iters, checkpoints every {every}",
+        "forward_ms_mixed32": round(g32["fwd_ms"], 2), "adjoint_ms_mixed32": round(g32["adj_ms"], 2),
+        "forward_ms_fp64": round(g64["fwd_ms"], 2), "adjoint_ms_fp64": round(g64["adj_ms"], 2),
+        "gradient_cloth_substeps_per_s_mixed32": round(sub_total / ((g32["fwd_ms"] + g32["adj_ms"]) * 1e-3), 2),
+        "max_pos_error_m_mixed32": xerr,
+        "grad_rel_err_mixed32_vs_fp64": {"kappa": rel(g32["gk"], g64["gk"]), "v0": rel(g32["gv"], g64["gv"]),
+                                          "pins": rel(g32["gp"], g64["gp"])},
+        "central_difference_rel_err_fp64": {k: abs(v[0] - v[1]) / abs(v[0]) for k, v in fd.items()},
+        "central_difference_rel_err_mixed32": {k: abs(v[0] - v[2]) / abs(v[0]) for k, v in fd.items()},
+        "fd_steps": {"v0": eps, "pins": eps, "kappa_relative": ks},
+    }
+    return out
+
+
 def copy_calibration(mib, reps=4):
     """A plain device-to-device copy of known size (the runtime's copy kernel),
     used to scale the FETCH_SIZE / WRITE_SIZE counters to bytes."""
@@ -138,6 +247,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-dropin", action="store_true")
+    ap.add_argument("--workload", default="batch", choices=["batch", "adjoint"])
+    ap.add_argument("--adj-instances", type=int, default=64)
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
@@ -146,6 +257,11 @@ def main():
         print(json.dumps({"error": "this engine runs on one GPU; the batch shards by instance across "
                                    "GPUs by design but no multi-GPU path is built (use --gpus 1)"}))
         sys.exit(2)
+
+    if args.workload == "adjoint":
+        print(json.dumps({"metric": "differentiable batch gradient (adjoint) throughput", **adjoint_workload(args)}),
+              flush=True)
+        return
 
     log("building scenes")
     cloths, cfg, _ = scenes.config4(seed=args.seed, count=args.instances)

@@ -171,3 +171,35 @@ def config4_instance(seed, index):
 def config4(seed=4, count=512):
     cloths = [config4_instance(seed, i) for i in range(count)]
     return cloths, SimConfig.fixed_iterations(40, H600), 200
+
+
+def pin_trajectory(cloth, nsub, h, amplitude=0.05, freq=2.0, phase=0.0, axis=(0.0, 1.0, 0.0)):
+    """Pin targets of substeps 1..nsub: q(t) = q0 + amplitude sin(2 pi freq t + phase) axis,
+    t = (s + 1) h, pins in ascending vertex order.  Shape (nsub, pins, 3)."""
+    pins = np.flatnonzero(cloth.pinned)
+    q0 = cloth.pin_positions.reshape(-1, 3)[pins]
+    t = (np.arange(nsub) + 1) * h
+    off = amplitude * np.sin(2.0 * np.pi * freq * t + phase)
+    return q0[None, :, :] + off[:, None, None] * np.asarray(axis, np.float64)[None, None, :]
+
+
+def config5(seed=5, count=64, side=128, nsub=120, iters=20):
+    """Differentiable batch: `count` side x side cloths, distance + shear + bend
+    with one stiffness scale kappa per instance (edges kappa, shear 0.2 kappa,
+    bends 0.02 kappa, the batch's fractions), two top corners pinned.
+
+    Returns (cloths at the starting guess, config, nsub, hidden) where the
+    starting guess is kappa = 2300, v0 = 0 and static pins, and `hidden` holds
+    per instance the seeded kappa ~ U[1e3, 5e3] and the seeded pin trajectory
+    (amplitude 0.05 m, 2 Hz, vertical, seeded phase) that produce the targets."""
+    cfg = SimConfig.fixed_iterations(iters, H600)
+    cloths, hidden = [], []
+    for i in range(count):
+        rng = SplitMix64((seed * 0x100000001B3) ^ (i * 0x9E3779B97F4A7C15))
+        k_hidden = 1.0e3 + 4.0e3 * rng.uniform()
+        phase = 2.0 * np.pi * rng.uniform()
+        c = grid_cloth(side, side, 2300.0, shear=0.2 * 2300.0, bend=0.02 * 2300.0)
+        c.meta.update(seed=seed, index=i, kappa=2300.0)
+        cloths.append(c)
+        hidden.append(dict(kappa=k_hidden, traj=pin_trajectory(c, nsub, H600, phase=phase)))
+    return cloths, cfg, nsub, hidden
