@@ -109,7 +109,7 @@ struct HostBlockSweep {
 // per run; each run yields one partial sum.  The tasks of a step (chain blocks,
 // one wavefront each, and batches of up to 64 single rows) then combine their
 // rows' partials in run order and finish the chains.
-void packBlocked(const Factor& f, bool fwd, int cap, HostBlockSweep& out) {
+void packBlocked(const Factor& f, bool fwd, int cap, int runCap, int denseCap, HostBlockSweep& out) {
   out = HostBlockSweep{};
   const int m = f.m;
   std::vector<int> parent((size_t)m, -1), nchild((size_t)m, 0);
@@ -118,10 +118,21 @@ void packBlocked(const Factor& f, bool fwd, int cap, HostBlockSweep& out) {
       parent[(size_t)j] = f.ci[(size_t)f.cp[(size_t)j]];
       nchild[(size_t)parent[(size_t)j]]++;
     }
+  // runs a row can stream at most (its whole row in this sweep's direction)
+  auto runsBound = [&](int k) {
+    const int len = fwd ? f.rp[(size_t)k + 1] - f.rp[(size_t)k] : f.cp[(size_t)k + 1] - f.cp[(size_t)k];
+    return (len + kRun - 1) / kRun;
+  };
   std::vector<int> bs{0};
+  int blockRuns = runsBound(0);
   for (int k = 1; k < m; ++k) {
     const bool chain = parent[(size_t)k - 1] == k && nchild[(size_t)k] == 1;
-    if (!chain || k - bs.back() >= cap) bs.push_back(k);
+    const int rk = runsBound(k);
+    if (!chain || k - bs.back() >= cap || blockRuns + rk > runCap) {
+      bs.push_back(k);
+      blockRuns = 0;
+    }
+    blockRuns += rk;
   }
   bs.push_back(m);
   const int nb = (int)bs.size() - 1;
@@ -171,6 +182,7 @@ void packBlocked(const Factor& f, bool fwd, int cap, HostBlockSweep& out) {
   out.stepTask.push_back(0);
   out.stepRun.push_back(0);
   out.stepDense.push_back(0);
+  int steps = 0;
   for (int l = 0; l < out.nstep; ++l) {
     std::vector<Task> tasks;
     std::vector<std::pair<int, int>> singles;
@@ -188,22 +200,52 @@ void packBlocked(const Factor& f, bool fwd, int cap, HostBlockSweep& out) {
       }
     }
     std::stable_sort(singles.begin(), singles.end(), [](auto& a, auto& b) { return a.first > b.first; });
-    for (size_t c0 = 0; c0 < singles.size(); c0 += 64) {
+    // batches of <= 64 rows (one per lane) and <= runCap runs
+    for (size_t c0 = 0; c0 < singles.size();) {
       Task t;
       t.chain = false;
-      for (size_t q = c0; q < std::min(singles.size(), c0 + 64); ++q) {
+      int tr = 0;
+      size_t q = c0;
+      for (; q < singles.size() && q < c0 + 64; ++q) {
+        const int r = (singles[q].first + kRun - 1) / kRun;
+        if (q > c0 && tr + r > runCap) break;
+        tr += r;
         t.ranks.push_back(bs[(size_t)singles[q].second]);
         t.blocks.push_back(singles[q].second);
       }
+      c0 = q;
       t.cost = 1.0;
       tasks.push_back(t);
     }
     std::stable_sort(tasks.begin(), tasks.end(), [](const Task& a, const Task& b) { return a.cost > b.cost; });
-    const int runBase = out.stepRun.back();
-    const int denseBase = out.stepDense.back();
+    int runBase = out.stepRun.back();
+    int denseBase = out.stepDense.back();
     int runs = 0;
+    // Closes the current step: its dense block padded to a multiple of 4 values
+    // (16-byte copies).  A level whose runs or dense blocks exceed the caps is
+    // cut into several steps (its tasks are independent of each other).
+    auto closeStep = [&]() {
+      while ((out.dense.size() - (size_t)denseBase) % 4) out.dense.push_back(0.0);
+      out.maxRuns = std::max(out.maxRuns, runs);
+      out.maxDense = std::max(out.maxDense, (int)out.dense.size() - denseBase);
+      out.stepTask.push_back((int)out.taskInfo.size());
+      out.stepRun.push_back(runBase + runs);
+      out.stepDense.push_back((int)out.dense.size());
+      ++steps;
+      runBase = out.stepRun.back();
+      denseBase = out.stepDense.back();
+      runs = 0;
+    };
     for (const Task& t : tasks) {
       const int nr = (int)t.ranks.size();
+      {
+        int tr = 0;
+        for (int r = 0; r < nr; ++r) tr += (extLen(t.ranks[(size_t)r], t.blocks[(size_t)r]) + kRun - 1) / kRun;
+        const int td = t.chain ? nr * nr + 3 : 0;
+        const bool nonEmpty = (int)out.taskInfo.size() > out.stepTask.back();
+        if (nonEmpty && (runs + tr > runCap || (int)(out.dense.size() - (size_t)denseBase) + td > denseCap))
+          closeStep();
+      }
       out.taskInfo.push_back(nr | (t.chain ? (1 << 16) : 0));
       out.taskSlot.push_back((int)out.slotRank.size());
       out.taskDense.push_back(t.chain ? (int)out.dense.size() - denseBase : 0);
@@ -259,14 +301,9 @@ void packBlocked(const Factor& f, bool fwd, int cap, HostBlockSweep& out) {
         }
       }
     }
-    // pad the step's dense block to a multiple of 4 values (16-byte copies)
-    while ((out.dense.size() - (size_t)denseBase) % 4) out.dense.push_back(0.0);
-    out.maxRuns = std::max(out.maxRuns, runs);
-    out.maxDense = std::max(out.maxDense, (int)out.dense.size() - denseBase);
-    out.stepTask.push_back((int)out.taskInfo.size());
-    out.stepRun.push_back(runBase + runs);
-    out.stepDense.push_back((int)out.dense.size());
+    closeStep();
   }
+  out.nstep = steps;
 }
 
 struct Instance {
@@ -469,8 +506,11 @@ int refreshFactors(pdx_ctx* c, bool& changed) {
         I.factored = true;
         I.numStamp = numericStamp(I.scene, I.cfg.dt);
         if (c->mode == PDX_SOLVE_FAST) {
-          packBlocked(I.fac, true, c->chainCap, I.bf);
-          packBlocked(I.fac, false, c->chainCap, I.bb);
+          // per-step LDS caps: run partials <= 48 KB, dense chain blocks <= 32 KB
+          const int w = c->prec == PDX_PREC_MIXED32 ? 4 : 8;
+          const int runCap = 48 * 1024 / (3 * w), denseCap = 32 * 1024 / w;
+          packBlocked(I.fac, true, c->chainCap, runCap, denseCap, I.bf);
+          packBlocked(I.fac, false, c->chainCap, runCap, denseCap, I.bb);
         } else {
           packSweep(I.fac.fwd, true, I.fac, I.hf);
           packSweep(I.fac.bwd, false, I.fac, I.hb);
