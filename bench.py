@@ -247,6 +247,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-dropin", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape (512 threads, chain cap 32)")
     ap.add_argument("--workload", default="batch", choices=["batch", "adjoint"])
     ap.add_argument("--adj-instances", type=int, default=64)
     args = ap.parse_args()
@@ -270,8 +271,12 @@ def main():
 
     mixed, tf32 = build_batch(cloths, cfg, "mixed32", args.solve)
     log(f"mixed32 factor+upload {tf32:.1f}s")
+    tune32 = mixed.autotune(1) if not args.no_tune else None
+    log(f"mixed32 autotune: {tune32}")
     fp64, tf64 = build_batch(cloths, cfg, "fp64", args.solve)
     log(f"fp64 factor+upload {tf64:.1f}s")
+    tune64 = fp64.autotune(1) if not args.no_tune else None
+    log(f"fp64 autotune: {tune64}")
 
     # correctness: FP32 form vs the FP64 path over the full run length
     wall32, kms32 = time_batch(mixed, args.steps, args.warmup)
@@ -309,7 +314,8 @@ def main():
         "data": "synthetic (seeded 512-instance cloth mix)",
         "config": {"workload": "512-instance cloth mix, sides 16-96, 40 PD iters, h=1/600",
                    "instances": ninst, "vertices": nverts, "pd_iterations": iters,
-                   "seed": args.seed, "solve": args.solve},
+                   "seed": args.seed, "solve": args.solve,
+                   "schedule": None if tune32 is None else {"threads": tune32[0], "chain_cap": tune32[1]}},
         "max_pos_error_m": max_err,
         "fp64_batched": {"value": round(ninst * args.steps / wall64, 2), "unit": "cloth-substeps/s"},
         "kernel_ms_per_launch": round(kms32, 4),

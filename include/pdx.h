@@ -192,6 +192,13 @@ int pdx_get_gradients(pdx_ctx* ctx, int instance, double* g_x0, double* g_v0, do
 /* Device time (ms) of the last pdx_rollout and of the last pdx_adjoint. */
 int pdx_rollout_timing(const pdx_ctx* ctx, double* forward_ms, double* adjoint_ms);
 
+/* Autotuner: times `probe_substeps` substeps of the whole context under each
+ * launch shape (threads per workgroup 256 / 512 x chain cap 8 / 16 / 24 / 32 in
+ * fast mode) and keeps the fastest.  State and statistics are restored, so the
+ * probes leave no trace.  No counterpart in the reference (its CPU path has no
+ * launch schedule). */
+int pdx_autotune(pdx_ctx* ctx, int probe_substeps, int* best_block, int* best_cap, double* best_ms);
+
 /* --- timing ---------------------------------------------------------------- */
 /* Average duration (ms) of the substep kernel over the last `pdx_step*` call,
  * measured with HIP events on the stream the kernel runs on. */

@@ -470,7 +470,7 @@ int refreshFactors(pdx_ctx* c, bool& changed) {
   for (int i = 0; i < ni; ++i) {
     Instance& I = c->inst[(size_t)i];
     const uint64_t ns = numericStamp(I.scene, I.cfg.dt);
-    if (!I.factored || ns != I.numStamp) todo.push_back(i);
+    if (!I.factored || ns != I.numStamp || !I.packed) todo.push_back(i);
   }
   if (todo.empty()) return PDX_OK;
   changed = true;
@@ -482,6 +482,25 @@ int refreshFactors(pdx_ctx* c, bool& changed) {
       if (t >= todo.size()) return;
       Instance& I = c->inst[(size_t)todo[t]];
       const double t0 = now();
+      auto pack = [&]() {
+        if (c->mode == PDX_SOLVE_FAST) {
+          // per-step LDS caps: run partials <= 48 KB, dense chain blocks <= 32 KB
+          const int w = c->prec == PDX_PREC_MIXED32 ? 4 : 8;
+          const int runCap = 48 * 1024 / (3 * w), denseCap = 32 * 1024 / w;
+          packBlocked(I.fac, true, c->chainCap, runCap, denseCap, I.bf);
+          packBlocked(I.fac, false, c->chainCap, runCap, denseCap, I.bb);
+        } else {
+          packSweep(I.fac.fwd, true, I.fac, I.hf);
+          packSweep(I.fac.bwd, false, I.fac, I.hb);
+        }
+        I.packed = true;
+      };
+      if (I.factored && I.numStamp == numericStamp(I.scene, I.cfg.dt)) {
+        // same matrix, new launch schedule: repack only
+        pack();
+        rcs[t] = 0;
+        continue;
+      }
       if (!I.topoBuilt) {
         buildTopology(I.scene, I.topo);
         I.topoBuilt = true;
@@ -505,17 +524,7 @@ int refreshFactors(pdx_ctx* c, bool& changed) {
         I.factorizeCount += 1;
         I.factored = true;
         I.numStamp = numericStamp(I.scene, I.cfg.dt);
-        if (c->mode == PDX_SOLVE_FAST) {
-          // per-step LDS caps: run partials <= 48 KB, dense chain blocks <= 32 KB
-          const int w = c->prec == PDX_PREC_MIXED32 ? 4 : 8;
-          const int runCap = 48 * 1024 / (3 * w), denseCap = 32 * 1024 / w;
-          packBlocked(I.fac, true, c->chainCap, runCap, denseCap, I.bf);
-          packBlocked(I.fac, false, c->chainCap, runCap, denseCap, I.bb);
-        } else {
-          packSweep(I.fac.fwd, true, I.fac, I.hf);
-          packSweep(I.fac.bwd, false, I.fac, I.hb);
-        }
-        I.packed = true;
+        pack();
       }
       I.factorSec = now() - t0;
       rcs[t] = rc;
@@ -1504,11 +1513,74 @@ int pdx_set_schedule(pdx_ctx* c, int block_threads, int chain_cap) {
     const int rc = pullState(c);
     if (rc) return rc;
     c->chainCap = chain_cap;
-    for (auto& I : c->inst) I.factored = false;  // repack (refactor of the same matrix)
-    for (auto& I : c->inst) I.factorizeCount -= 1;
+    for (auto& I : c->inst) I.packed = false;  // repack the same factor
     c->dirty = true;
   }
   c->block = block_threads;
+  return PDX_OK;
+}
+
+int pdx_autotune(pdx_ctx* c, int probe_substeps, int* best_block, int* best_cap, double* best_ms) {
+  if (!c || probe_substeps <= 0) return PDX_EINVAL;
+  int rc = ensureReady(c);
+  if (rc) return rc;
+  // the probes must leave no trace: state and statistics are restored afterwards
+  rc = pullState(c);
+  if (rc) return rc;
+  std::vector<std::vector<double>> xs, vs;
+  std::vector<std::vector<int>> ist;
+  std::vector<std::vector<long long>> lst;
+  for (auto& I : c->inst) {
+    xs.push_back(I.scene.x);
+    vs.push_back(I.scene.v);
+    std::vector<int> a(4);
+    std::vector<long long> b(16);
+    HIPCHECK(c, hipMemcpy(a.data(), I.distats, 16, hipMemcpyDeviceToHost));
+    HIPCHECK(c, hipMemcpy(b.data(), I.dlstats, 128, hipMemcpyDeviceToHost));
+    ist.push_back(a);
+    lst.push_back(b);
+  }
+  auto restore = [&]() -> int {
+    for (size_t i = 0; i < c->inst.size(); ++i) {
+      Instance& I = c->inst[i];
+      HIPCHECK(c, hipMemcpy(I.dx, xs[i].data(), xs[i].size() * 8, hipMemcpyHostToDevice));
+      HIPCHECK(c, hipMemcpy(I.dv, vs[i].data(), vs[i].size() * 8, hipMemcpyHostToDevice));
+      HIPCHECK(c, hipMemcpy(I.distats, ist[i].data(), 16, hipMemcpyHostToDevice));
+      HIPCHECK(c, hipMemcpy(I.dlstats, lst[i].data(), 128, hipMemcpyHostToDevice));
+      I.scene.x = xs[i];
+      I.scene.v = vs[i];
+    }
+    return PDX_OK;
+  };
+  const int blocks[2] = {256, 512};
+  const int caps[4] = {8, 16, 24, 32};
+  const bool fast = c->mode == PDX_SOLVE_FAST;
+  double bestMs = 1e300;
+  int bb = c->block, bc = c->chainCap;
+  for (int ci = 0; ci < (fast ? 4 : 1); ++ci) {
+    const int cap = fast ? caps[ci] : c->chainCap;
+    for (int bi = 0; bi < 2; ++bi) {
+      if ((rc = pdx_set_schedule(c, blocks[bi], cap))) return rc;
+      if ((rc = ensureReady(c))) {
+        if (rc == PDX_EUNSUPPORTED) continue;  // this schedule's staging does not fit in LDS
+        return rc;
+      }
+      if ((rc = restore())) return rc;
+      if ((rc = launch(c, probe_substeps))) return rc;
+      if ((rc = pdx_sync(c))) return rc;
+      if (c->lastKernelMs < bestMs) {
+        bestMs = c->lastKernelMs;
+        bb = blocks[bi];
+        bc = cap;
+      }
+    }
+  }
+  if ((rc = pdx_set_schedule(c, bb, bc))) return rc;
+  if ((rc = ensureReady(c))) return rc;
+  if ((rc = restore())) return rc;
+  if (best_block) *best_block = bb;
+  if (best_cap) *best_cap = bc;
+  if (best_ms) *best_ms = bestMs;
   return PDX_OK;
 }
 

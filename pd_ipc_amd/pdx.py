@@ -98,6 +98,7 @@ def load_library(path=LIB_PATH):
         "pdx_get_gradients": (I, [P, I, P, P, P, P, P]),
         "pdx_rollout_timing": (I, [P, C.POINTER(D), C.POINTER(D)]),
         "pdx_last_kernel_ms": (D, [P]),
+        "pdx_autotune": (I, [P, I, C.POINTER(I), C.POINTER(I), C.POINTER(D)]),
         "pdx_set_schedule": (I, [P, I, I]),
         "pdx_bytes_per_iteration": (D, [P]),
         "pdx_bytes_per_substep": (D, [P]),
@@ -387,6 +388,13 @@ class Engine:
 
     def set_schedule(self, block_threads, chain_cap=32):
         self._check(self.lib.pdx_set_schedule(self.h, block_threads, chain_cap), "set_schedule")
+
+    def autotune(self, probe_substeps=1):
+        """Picks the fastest launch shape for this context; returns (block, chain cap, ms)."""
+        b, cap, ms = C.c_int(), C.c_int(), C.c_double()
+        self._check(self.lib.pdx_autotune(self.h, probe_substeps, C.byref(b), C.byref(cap), C.byref(ms)),
+                    "autotune")
+        return b.value, cap.value, ms.value
 
     def last_kernel_ms(self):
         return self.lib.pdx_last_kernel_ms(self.h)

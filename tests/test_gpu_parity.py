@@ -129,3 +129,30 @@ def test_mixed32_fast_error_bound():
     err = np.abs(xg - o.positions()).max()
     print("max position error", err)
     assert err <= 1e-5
+
+
+@pytest.mark.gpu
+def test_autotune_leaves_no_trace():
+    """The autotuner's probe substeps are undone: stepping after tuning matches an
+    untuned context (within the fast form's bound), and no substep is counted."""
+    print("[case] autotune 3 instances fp64 fast", flush=True)
+    cl = [scenes.config4_instance(4, i) for i in range(3)]
+    cfg = SimConfig.fixed_iterations(10, 1.0 / 600.0)
+    a, b = Engine("fp64", "fast"), Engine("fp64", "fast")
+    for c in cl:
+        a.add(c, cfg)
+        b.add(c, cfg)
+    blk, cap, ms = a.autotune(1)
+    print("chosen", blk, cap, ms)
+    assert blk in (256, 512) and 1 <= cap <= 32 and ms > 0
+    for i in range(3):
+        assert a.stats(i)["substeps"] == 0
+    a.step(5)
+    b.step(5)
+    for i in range(3):
+        xa, _ = a.state(i)
+        xb, _ = b.state(i)
+        assert np.max(np.abs(xa - xb)) <= 1e-10 * np.max(np.abs(xb))
+        assert a.stats(i)["substeps"] == 5
+    a.close()
+    b.close()
