@@ -45,6 +45,12 @@ enum {
   PDX_SOLVE_FAST = 1    /* chain-blocked sweeps (forward order kept, backward reordered) */
 };
 
+enum {
+  PDX_LAUNCH_BATCH = 0, /* one workgroup per instance, persistent over the substeps */
+  PDX_LAUNCH_GRID = 1   /* every CU on one instance: cooperative launch, grid barrier per sweep step
+                           (fast solve mode, exactly one instance) */
+};
+
 typedef struct pdx_ctx pdx_ctx;
 
 /* Mirrors SceneConfig (core/sim/Scene.h:14-137): the fields the substep reads. */
@@ -191,6 +197,10 @@ int pdx_get_gradients(pdx_ctx* ctx, int instance, double* g_x0, double* g_v0, do
                       double* g_bend, double* g_pin);
 /* Device time (ms) of the last pdx_rollout and of the last pdx_adjoint. */
 int pdx_rollout_timing(const pdx_ctx* ctx, double* forward_ms, double* adjoint_ms);
+
+/* Launch form (PDX_LAUNCH_*); grid_blocks = workgroups of the grid form, 0 = all
+ * that can be co-resident.  Re-packs the schedules for the form. */
+int pdx_set_launch(pdx_ctx* ctx, int mode, int grid_blocks);
 
 /* Autotuner: times `probe_substeps` substeps of the whole context under each
  * launch shape (threads per workgroup 256 / 512 x chain cap 8 / 16 / 24 / 32 in

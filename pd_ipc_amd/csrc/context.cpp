@@ -26,6 +26,11 @@ hipError_t launchAdjoint(const InstDev* d, const int* order, int ninst, int nsub
 hipError_t launchSolve(const InstDev* d, int inst, const double* b, double* x, int block,
                        size_t lds, bool fp32, bool blocked, bool idx16, hipStream_t s);
 hipError_t setLdsLimit(size_t bytes);
+hipError_t launchGrid(const InstDev* d, int inst, int nsub, double* gslots, int waveRuns, int blocks,
+                      int block, size_t lds, bool fp32, bool idx16, hipStream_t s);
+hipError_t launchGridSolve(const InstDev* d, int inst, const double* b, double* x, int waveRuns, int blocks,
+                           int block, size_t lds, bool fp32, bool idx16, hipStream_t s);
+int gridBlocksMax(int block, size_t lds, bool fp32, bool idx16);
 }  // namespace pdx
 
 using namespace pdx;
@@ -89,6 +94,7 @@ constexpr int kRun = 16;  // factor entries per streamed run (matches the kernel
 struct HostBlockSweep {
   int nstep = 0;
   int maxRuns = 0;        // most runs in one step (partial-sum slots in LDS)
+  int maxTaskRuns = 0;    // most runs of one task (grid form: one wavefront's LDS slice)
   int maxDense = 0;       // most dense-block values in one step (LDS staging)
   std::vector<int> stepTask, stepRun, stepDense;  // nstep+1 each
   std::vector<int> taskInfo, taskSlot, taskDense; // per task
@@ -109,7 +115,8 @@ struct HostBlockSweep {
 // per run; each run yields one partial sum.  The tasks of a step (chain blocks,
 // one wavefront each, and batches of up to 64 single rows) then combine their
 // rows' partials in run order and finish the chains.
-void packBlocked(const Factor& f, bool fwd, int cap, int runCap, int denseCap, HostBlockSweep& out) {
+void packBlocked(const Factor& f, bool fwd, int cap, int runCap, int stepRunCap, int denseCap,
+                 HostBlockSweep& out) {
   out = HostBlockSweep{};
   const int m = f.m;
   std::vector<int> parent((size_t)m, -1), nchild((size_t)m, 0);
@@ -242,8 +249,9 @@ void packBlocked(const Factor& f, bool fwd, int cap, int runCap, int denseCap, H
         int tr = 0;
         for (int r = 0; r < nr; ++r) tr += (extLen(t.ranks[(size_t)r], t.blocks[(size_t)r]) + kRun - 1) / kRun;
         const int td = t.chain ? nr * nr + 3 : 0;
+        out.maxTaskRuns = std::max(out.maxTaskRuns, tr);
         const bool nonEmpty = (int)out.taskInfo.size() > out.stepTask.back();
-        if (nonEmpty && (runs + tr > runCap || (int)(out.dense.size() - (size_t)denseBase) + td > denseCap))
+        if (nonEmpty && (runs + tr > stepRunCap || (int)(out.dense.size() - (size_t)denseBase) + td > denseCap))
           closeStep();
       }
       out.taskInfo.push_back(nr | (t.chain ? (1 << 16) : 0));
@@ -373,6 +381,13 @@ struct pdx_ctx {
   double* dScratchX = nullptr;
   int scratchN = 0;
   int uploadEpoch = 0;
+  // launch form: 0 = one workgroup per instance (batch), 1 = the whole grid on one
+  // instance (cooperative launch)
+  int launchMode = 0;
+  int gridBlocks = 0;       // 0 = every co-resident workgroup
+  int gridLaunchBlocks = 0; // blocks of the last grid upload
+  int waveRuns = 0;         // runs per wavefront LDS slice (grid form)
+  double* dGridSlots = nullptr;
   // differentiable rollout: pin trajectories (host), device buffers, seeds
   struct Roll {
     int nsub = 0, every = 0, nseg = 0, epoch = -1;
@@ -484,11 +499,19 @@ int refreshFactors(pdx_ctx* c, bool& changed) {
       const double t0 = now();
       auto pack = [&]() {
         if (c->mode == PDX_SOLVE_FAST) {
-          // per-step LDS caps: run partials <= 48 KB, dense chain blocks <= 32 KB
           const int w = c->prec == PDX_PREC_MIXED32 ? 4 : 8;
-          const int runCap = 48 * 1024 / (3 * w), denseCap = 32 * 1024 / w;
-          packBlocked(I.fac, true, c->chainCap, runCap, denseCap, I.bf);
-          packBlocked(I.fac, false, c->chainCap, runCap, denseCap, I.bb);
+          if (c->launchMode == 1) {
+            // grid form: a task's runs fit one wavefront's LDS slice; steps are
+            // not staged in LDS
+            const int big = 1 << 29;
+            packBlocked(I.fac, true, c->chainCap, c->waveRuns, big, big, I.bf);
+            packBlocked(I.fac, false, c->chainCap, c->waveRuns, big, big, I.bb);
+          } else {
+            // per-step LDS caps: run partials <= 48 KB, dense chain blocks <= 32 KB
+            const int runCap = 48 * 1024 / (3 * w), denseCap = 32 * 1024 / w;
+            packBlocked(I.fac, true, c->chainCap, runCap, runCap, denseCap, I.bf);
+            packBlocked(I.fac, false, c->chainCap, runCap, runCap, denseCap, I.bb);
+          }
         } else {
           packSweep(I.fac.fwd, true, I.fac, I.hf);
           packSweep(I.fac.bwd, false, I.fac, I.hb);
@@ -562,6 +585,13 @@ int upload(pdx_ctx* c) {
   }
   c->ldsW = kRedBytes + maxW + maxExtra <= kLdsLimit;
   c->lds = kRedBytes + (c->ldsW ? maxW : 0) + maxExtra;
+  if (c->launchMode == 1) {
+    c->ldsW = false;
+    c->lds = kRedBytes + (size_t)(c->block / 64) * c->waveRuns * 3 * w;
+    for (auto& I : c->inst)
+      if (std::max(I.bf.maxTaskRuns, I.bb.maxTaskRuns) > c->waveRuns)
+        return fail(c, PDX_EUNSUPPORTED, "a factor row streams more runs than one wavefront's LDS slice");
+  }
   if (c->lds > kLdsLimit)
     return fail(c, PDX_EUNSUPPORTED, "sweep staging does not fit in LDS (" + std::to_string(c->lds) + " B)");
   Arena ar;
@@ -710,6 +740,7 @@ int upload(pdx_ctx* c) {
     I.bytesPerIter = bi;
     c->bytesIter += bi;
   }
+  const size_t slotOff = ar.add(nullptr, 2 * 1024 * sizeof(double));
   const size_t instOff = ar.add(nullptr, sizeof(InstDev) * c->inst.size());
   // LPT order: heaviest instance (factor entries) first
   std::vector<int> order(c->inst.size());
@@ -843,6 +874,7 @@ int upload(pdx_ctx* c) {
   std::memcpy(ar.host.data() + instOff, c->hInst.data(), sizeof(InstDev) * c->hInst.size());
   HIPCHECK(c, hipMemcpy(c->dbuf, ar.host.data(), ar.host.size(), hipMemcpyHostToDevice));
   c->dInst = (InstDev*)(B + instOff);
+  c->dGridSlots = (double*)(B + slotOff);
   c->dOrder = (int*)(B + orderOff);
   HIPCHECK(c, setLdsLimit(kLdsLimit));
   c->uploaded = true;
@@ -897,8 +929,30 @@ int ensureReady(pdx_ctx* c) {
   return PDX_OK;
 }
 
+int gridLaunchBlocks(pdx_ctx* c) {
+  const int maxb = gridBlocksMax(c->block, c->lds, c->prec == PDX_PREC_MIXED32, c->idx16);
+  const int want = c->gridBlocks > 0 ? c->gridBlocks : maxb;
+  return std::min(std::min(want, maxb), 1024);
+}
+
+int launchGridForm(pdx_ctx* c, int nsub) {
+  const int g = gridLaunchBlocks(c);
+  if (g <= 0) return fail(c, PDX_EHIP, "grid form: no co-resident workgroup fits");
+  HIPCHECK(c, launchGrid(c->dInst, 0, nsub, c->dGridSlots, c->waveRuns, g, c->block, c->lds,
+                         c->prec == PDX_PREC_MIXED32, c->idx16, c->stream));
+  return PDX_OK;
+}
+
 int launch(pdx_ctx* c, int nsub) {
   HIPCHECK(c, hipSetDevice(c->device));
+  if (c->launchMode == 1) {
+    HIPCHECK(c, hipEventRecord(c->ev0, c->stream));
+    const int rc = launchGridForm(c, nsub);
+    if (rc) return rc;
+    HIPCHECK(c, hipEventRecord(c->ev1, c->stream));
+    c->lastLaunches = 1;
+    return PDX_OK;
+  }
   HIPCHECK(c, hipEventRecord(c->ev0, c->stream));
   HIPCHECK(c, launchSubstep(c->dInst, c->dOrder, (int)c->inst.size(), nsub, c->block, c->lds,
                             c->prec == PDX_PREC_MIXED32, c->mode == PDX_SOLVE_FAST, c->idx16, c->stream, 0, 0));
@@ -1016,6 +1070,15 @@ int pdx_step_graph(pdx_ctx* c, int nsub, int replays) {
   if (!c || nsub <= 0 || replays < 0) return PDX_EINVAL;
   int rc = ensureReady(c);
   if (rc) return rc;
+  if (c->launchMode == 1) {
+    // cooperative launches are replayed directly (not captured)
+    HIPCHECK(c, hipEventRecord(c->ev0, c->stream));
+    for (int r = 0; r < replays; ++r)
+      if ((rc = launchGridForm(c, nsub))) return rc;
+    HIPCHECK(c, hipEventRecord(c->ev1, c->stream));
+    c->lastLaunches = std::max(1, replays);
+    return pdx_sync(c);
+  }
   hipGraph_t g = nullptr;
   hipGraphExec_t ge = nullptr;
   HIPCHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
@@ -1160,8 +1223,14 @@ int pdx_solve3(pdx_ctx* c, int i, const double* b, double* x) {
     c->scratchN = n3;
   }
   HIPCHECK(c, hipMemcpyAsync(c->dScratchB, b, (size_t)n3 * 8, hipMemcpyHostToDevice, c->stream));
-  HIPCHECK(c, launchSolve(c->dInst, i, c->dScratchB, c->dScratchX, c->block, c->lds,
-                          c->prec == PDX_PREC_MIXED32, c->mode == PDX_SOLVE_FAST, c->idx16, c->stream));
+  if (c->launchMode == 1) {
+    const int g = gridLaunchBlocks(c);
+    HIPCHECK(c, launchGridSolve(c->dInst, i, c->dScratchB, c->dScratchX, c->waveRuns, g, c->block, c->lds,
+                                c->prec == PDX_PREC_MIXED32, c->idx16, c->stream));
+  } else {
+    HIPCHECK(c, launchSolve(c->dInst, i, c->dScratchB, c->dScratchX, c->block, c->lds,
+                            c->prec == PDX_PREC_MIXED32, c->mode == PDX_SOLVE_FAST, c->idx16, c->stream));
+  }
   HIPCHECK(c, hipMemcpyAsync(x, c->dScratchX, (size_t)n3 * 8, hipMemcpyDeviceToHost, c->stream));
   HIPCHECK(c, hipStreamSynchronize(c->stream));
   return PDX_OK;
@@ -1273,6 +1342,7 @@ int pdx_set_pin_trajectory(pdx_ctx* c, int i, int nsub, const double* traj) {
 
 int pdx_rollout(pdx_ctx* c, int nsub, int every) {
   if (!c || nsub <= 0 || every <= 0) return PDX_EINVAL;
+  if (c->launchMode == 1) return fail(c, PDX_EUNSUPPORTED, "the differentiable rollout runs in the batch form");
   int rc = ensureReady(c);
   if (rc) return rc;
   const size_t ni = c->inst.size();
@@ -1517,6 +1587,22 @@ int pdx_set_schedule(pdx_ctx* c, int block_threads, int chain_cap) {
     c->dirty = true;
   }
   c->block = block_threads;
+  return PDX_OK;
+}
+
+int pdx_set_launch(pdx_ctx* c, int mode, int grid_blocks) {
+  if (!c || (mode != PDX_LAUNCH_BATCH && mode != PDX_LAUNCH_GRID) || grid_blocks < 0) return PDX_EINVAL;
+  if (mode == PDX_LAUNCH_GRID && (c->inst.size() != 1 || c->mode != PDX_SOLVE_FAST))
+    return fail(c, PDX_EUNSUPPORTED, "the grid form takes exactly one instance in fast solve mode");
+  if (mode != c->launchMode) {
+    const int rc = pullState(c);
+    if (rc) return rc;
+    c->launchMode = mode;
+    c->waveRuns = (c->prec == PDX_PREC_MIXED32) ? 1536 : 768;
+    for (auto& I : c->inst) I.packed = false;
+    c->dirty = true;
+  }
+  c->gridBlocks = grid_blocks;
   return PDX_OK;
 }
 

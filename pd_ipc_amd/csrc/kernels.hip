@@ -17,6 +17,7 @@
 //    and the residual are gathered in FP64 from the same projections.
 //
 // All FP64 code is compiled with -ffp-contract=off: no operation is fused.
+#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 
 #include "devtypes.h"
@@ -1144,6 +1145,294 @@ __global__ void __launch_bounds__(512) adjointKernel(const InstDev* __restrict__
   }
 }
 
+// ---------------------------------------------------------------------------
+// Cooperative form for one large instance: the whole grid (one workgroup per CU,
+// all co-resident) advances a single cloth.  Vertex phases are grid-strided; a
+// chain-blocked sweep step is taken by every wavefront of the grid, each
+// streaming the runs of whole tasks into its own slice of LDS and finishing
+// them, followed by one grid barrier; reductions go through per-workgroup
+// slots and every workgroup reduces them in the same order, so all take the
+// same stop decision.  W lives in HBM.  Same arithmetic per row as the
+// single-workgroup fast sweep: run partials summed in run order.
+
+namespace cg = cooperative_groups;
+
+// Max of v over the grid.  Slot set `buf` alternates between consecutive calls.
+__device__ inline double gridMax(cg::grid_group& grid, double v, double* red, double* gslots, int buf) {
+  const double b = blockMax(v, red);
+  if (threadIdx.x == 0) gslots[buf * gridDim.x + blockIdx.x] = b;
+  grid.sync();
+  double m = 0.0;
+  for (int q = threadIdx.x; q < (int)gridDim.x; q += blockDim.x) m = fmax(m, gslots[buf * gridDim.x + q]);
+  __syncthreads();
+  return blockMax(m, red + 16);
+}
+
+template <typename Real, bool kFwd, bool kIdx16, typename Store>
+__device__ void gridSweep(cg::grid_group& grid, const BlockSweepDev& sw, Real* W, Real* Pw, Store store) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int nw = blockDim.x / kWave;
+  const int gw = blockIdx.x * nw + (int)(threadIdx.x / kWave);
+  const int GW = gridDim.x * nw;
+  const Real* rval = static_cast<const Real*>(sw.runVal);
+  const PDX_GLOBAL Real* gdense = gview<Real>(sw.dense);
+  const PDX_GLOBAL v4i* gmeta = gview<v4i>(sw.laneMeta);
+  const PDX_GLOBAL Real* gldinv = gview<Real>(sw.laneDinv);
+  for (int st = 0; st < sw.nstep; ++st) {
+    const int tb = sw.stepTask[st], te = sw.stepTask[st + 1];
+    const int rbase = sw.stepRun[st], dbase = sw.stepDense[st];
+    for (int t = tb + gw; t < te; t += GW) {
+      const v4i mt = gmeta[(size_t)t * kWave + lane];
+      const Real dv = kFwd ? Real(0) : gldinv[(size_t)t * kWave + lane];
+      const int k = mt.x;
+      const int info = mt.w;
+      const int nrows = info & 0x7f;
+      const bool chain = (info & 0x80) != 0;
+      const bool act = k >= 0;
+      const int rb = __builtin_amdgcn_readlane(mt.y, 0);
+      const int re = __builtin_amdgcn_readlane(mt.z, nrows - 1);
+      for (int r = rb + lane; r < re; r += kWave) {
+        RunRaw<Real, kIdx16> raw;
+        loadRun<Real, kIdx16>(sw.runCol, rval, rbase + r, raw);
+        runPartial<Real, kIdx16>(raw, W, Pw + 3 * (r - rb));
+      }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      Real a0 = 0, a1 = 0, a2 = 0;
+      if (act) {
+        a0 = W[3 * k + 0];
+        a1 = W[3 * k + 1];
+        a2 = W[3 * k + 2];
+        if (!kFwd) {
+          a0 = a0 * dv;
+          a1 = a1 * dv;
+          a2 = a2 * dv;
+        }
+        for (int q = mt.y; q < mt.z; ++q) {
+          a0 = a0 - Pw[3 * (q - rb) + 0];
+          a1 = a1 - Pw[3 * (q - rb) + 1];
+          a2 = a2 - Pw[3 * (q - rb) + 2];
+        }
+      }
+      if (chain) {
+        const PDX_GLOBAL Real* Dt = gdense + dbase + (info >> 8);
+        Real dreg[kCap];
+#pragma unroll
+        for (int q = 0; q < kCap; ++q) dreg[q] = (act && q < nrows) ? Dt[q * nrows + lane] : Real(0);
+        if (kFwd) {
+#pragma unroll
+          for (int q = 0; q < kCap; ++q) {
+            if (q < nrows) {
+              const Real y0 = readlaneT(a0, q), y1 = readlaneT(a1, q), y2 = readlaneT(a2, q);
+              if (lane > q && dreg[q] != Real(0)) {
+                a0 = subMul<Real>(a0, dreg[q], y0);
+                a1 = subMul<Real>(a1, dreg[q], y1);
+                a2 = subMul<Real>(a2, dreg[q], y2);
+              }
+            }
+          }
+        } else {
+#pragma unroll
+          for (int q = kCap - 1; q >= 0; --q) {
+            if (q < nrows) {
+              const Real y0 = readlaneT(a0, q), y1 = readlaneT(a1, q), y2 = readlaneT(a2, q);
+              if (lane < q && dreg[q] != Real(0)) {
+                a0 = subMul<Real>(a0, dreg[q], y0);
+                a1 = subMul<Real>(a1, dreg[q], y1);
+                a2 = subMul<Real>(a2, dreg[q], y2);
+              }
+            }
+          }
+        }
+      }
+      if (act) {
+        W[3 * k + 0] = a0;
+        W[3 * k + 1] = a1;
+        W[3 * k + 2] = a2;
+        store(k, a0, a1, a2);
+      }
+      // the next task of this wavefront reuses the partial slice
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+    }
+    grid.sync();
+  }
+}
+
+template <typename Real, bool kExact, bool kIdx16>
+__global__ void __launch_bounds__(512) gridSubstepKernel(const InstDev* __restrict__ insts, int inst, int nsub,
+                                                          double* __restrict__ gslots, int waveRuns) {
+  extern __shared__ double smem[];
+  cg::grid_group grid = cg::this_grid();
+  const InstDev& I = insts[inst];
+  double* red = smem;  // 2 x 16 slots
+  Real* Pw = reinterpret_cast<Real*>(smem + 32) + (size_t)(threadIdx.x / kWave) * waveRuns * 3;
+  Real* W = static_cast<Real*>(I.Wg);
+  const int n = I.n;
+  const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  const int gs = gridDim.x * blockDim.x;
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  int buf = 0;
+  for (int sub = 0; sub < nsub; ++sub) {
+    // predict (Integrator.cpp:447-463), displacement scale, first right-hand side
+    double lscale = 0.0;
+    for (int v = gt; v < n; v += gs) {
+      for (int d = 0; d < 3; ++d) {
+        const double p = I.x[3 * v + d];
+        I.xbs[3 * v + d] = p;
+        const double xh = p + I.vel[3 * v + d] * I.h + I.gh2[d];
+        I.xhat[3 * v + d] = xh;
+        I.prev[3 * v + d] = p;
+        lscale = fmax(lscale, fabs(xh - p));
+      }
+    }
+    for (int v = gt; v < n; v += gs) {
+      const int r = I.vrank[v];
+      if (kExact) {
+        const Acc3 b = rhsExact(I, v);
+        putW(W, r, b.x, b.y, b.z);
+      } else if (I.pinned[v]) {
+        putW(W, r, I.pinpos[3 * v] - I.x[3 * v], I.pinpos[3 * v + 1] - I.x[3 * v + 1],
+             I.pinpos[3 * v + 2] - I.x[3 * v + 2]);
+      } else {
+        Acc3 rh;
+        (void)residualAndRhs(I, v, &rh);
+        putW(W, r, rh.x, rh.y, rh.z);
+      }
+    }
+    double scale = gridMax(grid, lscale, red, gslots, buf);
+    buf ^= 1;
+    scale = fmax(scale, I.gScaleH2);
+
+    int used = 0, conv = 0;
+    double lastRes = -1.0;
+    auto none = [](int, Real, Real, Real) {};
+    double* x = I.x;
+    const int* rankv = I.rankv;
+    auto put = [=](int k, Real a0, Real a1, Real a2) {
+      if (kExact) {
+        const int v = rankv[k];
+        x[3 * v] = (double)a0;
+        x[3 * v + 1] = (double)a1;
+        x[3 * v + 2] = (double)a2;
+      }
+    };
+    for (int k = 0; k < I.maxIter; ++k) {
+      gridSweep<Real, true, kIdx16>(grid, I.bfwd, W, Pw, none);
+      gridSweep<Real, false, kIdx16>(grid, I.bbwd, W, Pw, put);
+      if (!kExact) {
+        for (int v = gt; v < n; v += gs) {
+          const int r = I.vrank[v];
+          if (I.pinned[v]) {
+            I.x[3 * v] = I.pinpos[3 * v];
+            I.x[3 * v + 1] = I.pinpos[3 * v + 1];
+            I.x[3 * v + 2] = I.pinpos[3 * v + 2];
+          } else {
+            I.x[3 * v] += (double)W[3 * r];
+            I.x[3 * v + 1] += (double)W[3 * r + 1];
+            I.x[3 * v + 2] += (double)W[3 * r + 2];
+          }
+        }
+        grid.sync();
+      }
+      const bool more = (k + 1 < I.maxIter);
+      double ldiff = 0.0, lres = 0.0;
+      for (int v = gt; v < n; v += gs) {
+        const double x0 = I.x[3 * v], x1 = I.x[3 * v + 1], x2 = I.x[3 * v + 2];
+        const double d = fmax(fabs(x0 - I.prev[3 * v]),
+                              fmax(fabs(x1 - I.prev[3 * v + 1]), fabs(x2 - I.prev[3 * v + 2])));
+        ldiff = fmax(ldiff, d);
+        I.prev[3 * v] = x0;
+        I.prev[3 * v + 1] = x1;
+        I.prev[3 * v + 2] = x2;
+        const bool pinnedV = I.pinned[v] != 0;
+        const int r = I.vrank[v];
+        if (kExact) {
+          if (I.wantResidual && !pinnedV && I.mass[v] > 0.0) lres = fmax(lres, residualAndRhs(I, v, nullptr));
+          if (more) {
+            const Acc3 b = rhsExact(I, v);
+            putW(W, r, b.x, b.y, b.z);
+          }
+        } else {
+          if (pinnedV) {
+            if (more) putW(W, r, I.pinpos[3 * v] - x0, I.pinpos[3 * v + 1] - x1, I.pinpos[3 * v + 2] - x2);
+          } else if (more || I.wantResidual) {
+            Acc3 rh;
+            const double rv = residualAndRhs(I, v, more ? &rh : nullptr);
+            if (I.wantResidual && I.mass[v] > 0.0) lres = fmax(lres, rv);
+            if (more) putW(W, r, rh.x, rh.y, rh.z);
+          }
+        }
+      }
+      const double diff = gridMax(grid, ldiff, red, gslots, buf);
+      buf ^= 1;
+      double residual = -1.0;
+      if (I.wantResidual) {
+        residual = gridMax(grid, lres, red, gslots, buf);
+        buf ^= 1;
+      }
+      const double rel = (scale > 0.0) ? diff / scale : diff;
+      const bool absHit = (I.absTol > 0.0) && (diff <= I.absTol);
+      const bool relHit = (I.relTol > 0.0) && (rel <= I.relTol);
+      const bool resHit = (I.resTol > 0.0) && (residual <= I.resThresh);
+      const bool c = (I.resTol > 0.0) ? resHit : (absHit || relHit);
+      used = k + 1;
+      lastRes = residual;
+      if (c) {
+        conv = 1;
+        break;
+      }
+    }
+    // velocity + damping + re-pin (Integrator.cpp:774-791)
+    for (int v = gt; v < n; v += gs) {
+      const bool pinnedV = I.pinned[v] != 0;
+      for (int d = 0; d < 3; ++d) {
+        if (pinnedV) {
+          I.vel[3 * v + d] = 0.0;
+          I.x[3 * v + d] = I.pinpos[3 * v + d];
+        } else {
+          I.vel[3 * v + d] = (I.x[3 * v + d] - I.xbs[3 * v + d]) * I.dampInvH;
+        }
+      }
+    }
+    if (lead) {
+      I.istats[0] = used;
+      I.istats[1] = conv;
+      I.istats[2] += conv;
+      I.lstats[0] += used;
+      I.lstats[1] += 1;
+      I.dstats[0] = lastRes;
+    }
+    grid.sync();
+  }
+}
+
+// Global solve x = L^{-1} b on one instance with the whole grid.
+template <typename Real, bool kIdx16>
+__global__ void __launch_bounds__(512) gridSolveKernel(const InstDev* __restrict__ insts, int inst,
+                                                        const double* __restrict__ b, double* __restrict__ xout,
+                                                        int waveRuns) {
+  extern __shared__ double smem[];
+  cg::grid_group grid = cg::this_grid();
+  const InstDev& I = insts[inst];
+  Real* Pw = reinterpret_cast<Real*>(smem + 32) + (size_t)(threadIdx.x / kWave) * waveRuns * 3;
+  Real* W = static_cast<Real*>(I.Wg);
+  const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  const int gs = gridDim.x * blockDim.x;
+  for (int v = gt; v < I.n; v += gs) putW(W, I.vrank[v], b[3 * v], b[3 * v + 1], b[3 * v + 2]);
+  grid.sync();
+  auto none = [](int, Real, Real, Real) {};
+  const int* rankv = I.rankv;
+  auto put = [=](int k, Real a0, Real a1, Real a2) {
+    const int v = rankv[k];
+    xout[3 * v] = (double)a0;
+    xout[3 * v + 1] = (double)a1;
+    xout[3 * v + 2] = (double)a2;
+  };
+  gridSweep<Real, true, kIdx16>(grid, I.bfwd, W, Pw, none);
+  gridSweep<Real, false, kIdx16>(grid, I.bbwd, W, Pw, put);
+}
+
 // Stand-alone global solve x = L^{-1} b (EigenDirectSolver::solve) on one instance.
 template <typename Real, bool kBlocked, bool kIdx16>
 __global__ void __launch_bounds__(512) solveKernel(const InstDev* __restrict__ insts, int inst,
@@ -1215,6 +1504,44 @@ hipError_t launchAdjoint(const InstDev* d, const int* order, int ninst, int nsub
   return hipErrorInvalidValue;
 }
 
+hipError_t launchGrid(const InstDev* d, int inst, int nsub, double* gslots, int waveRuns, int blocks,
+                      int block, size_t lds, bool fp32, bool idx16, hipStream_t s) {
+  void* args[] = {(void*)&d, (void*)&inst, (void*)&nsub, (void*)&gslots, (void*)&waveRuns};
+#define PDX_G(R, E, X16)                                                                                 \
+  if (fp32 == (sizeof(R) == 4) && idx16 == X16)                                                          \
+    return hipLaunchCooperativeKernel((const void*)gridSubstepKernel<R, E, X16>, dim3(blocks), dim3(block), \
+                                      args, lds, s);
+  PDX_G(float, false, true) PDX_G(float, false, false) PDX_G(double, true, true) PDX_G(double, true, false)
+#undef PDX_G
+  return hipErrorInvalidValue;
+}
+
+hipError_t launchGridSolve(const InstDev* d, int inst, const double* b, double* x, int waveRuns, int blocks,
+                           int block, size_t lds, bool fp32, bool idx16, hipStream_t s) {
+  void* args[] = {(void*)&d, (void*)&inst, (void*)&b, (void*)&x, (void*)&waveRuns};
+#define PDX_G(R, X16)                                                                                   \
+  if (fp32 == (sizeof(R) == 4) && idx16 == X16)                                                         \
+    return hipLaunchCooperativeKernel((const void*)gridSolveKernel<R, X16>, dim3(blocks), dim3(block), args, \
+                                      lds, s);
+  PDX_G(float, true) PDX_G(float, false) PDX_G(double, true) PDX_G(double, false)
+#undef PDX_G
+  return hipErrorInvalidValue;
+}
+
+int gridBlocksMax(int block, size_t lds, bool fp32, bool idx16) {
+  int perCu = 0, dev = 0, cus = 0;
+  hipGetDevice(&dev);
+  hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+#define PDX_G(R, E, X16)                                                                                        \
+  if (fp32 == (sizeof(R) == 4) && idx16 == X16)                                                                 \
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, (const void*)gridSubstepKernel<R, E, X16>, block, \
+                                                     lds) != hipSuccess)                                        \
+      return 0;
+  PDX_G(float, false, true) PDX_G(float, false, false) PDX_G(double, true, true) PDX_G(double, true, false)
+#undef PDX_G
+  return perCu * cus;
+}
+
 hipError_t launchSolve(const InstDev* d, int inst, const double* b, double* x, int block,
                        size_t lds, bool fp32, bool blocked, bool idx16, hipStream_t s) {
 #define PDX_L(R, E, B, X16)                                                                       \
@@ -1244,6 +1571,18 @@ hipError_t setLdsLimit(size_t bytes) {
   if (e != hipSuccess) return e;
   PDX_KERNEL_TABLE(PDX_L)
 #undef PDX_L
+#define PDX_G(R, E, X16)                                                                                  \
+  e = hipFuncSetAttribute((const void*)gridSubstepKernel<R, E, X16>, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                          (int)bytes);                                                                    \
+  if (e != hipSuccess) return e;
+  PDX_G(float, false, true) PDX_G(float, false, false) PDX_G(double, true, true) PDX_G(double, true, false)
+#undef PDX_G
+#define PDX_G(R, X16)                                                                                   \
+  e = hipFuncSetAttribute((const void*)gridSolveKernel<R, X16>, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                          (int)bytes);                                                                  \
+  if (e != hipSuccess) return e;
+  PDX_G(float, true) PDX_G(float, false) PDX_G(double, true) PDX_G(double, false)
+#undef PDX_G
   return e;
 }
 

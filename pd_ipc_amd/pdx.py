@@ -99,6 +99,7 @@ def load_library(path=LIB_PATH):
         "pdx_rollout_timing": (I, [P, C.POINTER(D), C.POINTER(D)]),
         "pdx_last_kernel_ms": (D, [P]),
         "pdx_autotune": (I, [P, I, C.POINTER(I), C.POINTER(I), C.POINTER(D)]),
+        "pdx_set_launch": (I, [P, I, I]),
         "pdx_set_schedule": (I, [P, I, I]),
         "pdx_bytes_per_iteration": (D, [P]),
         "pdx_bytes_per_substep": (D, [P]),
@@ -388,6 +389,12 @@ class Engine:
 
     def set_schedule(self, block_threads, chain_cap=32):
         self._check(self.lib.pdx_set_schedule(self.h, block_threads, chain_cap), "set_schedule")
+
+    def set_launch(self, mode, grid_blocks=0):
+        """mode: 'batch' (one workgroup per instance) or 'grid' (the whole GPU on
+        one instance, fast solve only)."""
+        m = {"batch": 0, "grid": 1}[mode]
+        self._check(self.lib.pdx_set_launch(self.h, m, grid_blocks), "set_launch")
 
     def autotune(self, probe_substeps=1):
         """Picks the fastest launch shape for this context; returns (block, chain cap, ms)."""

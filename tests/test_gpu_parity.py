@@ -156,3 +156,42 @@ def test_autotune_leaves_no_trace():
         assert a.stats(i)["substeps"] == 5
     a.close()
     b.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision,nx,ny,shear,bend,iters,nsub,tol", [
+    ("fp64", 40, 40, 0.0, 0.0, 10, 20, 1e-10),
+    ("fp64", 100, 100, 500.0, 50.0, 40, 3, 1e-10),
+    ("mixed32", 100, 100, 500.0, 50.0, 40, 3, None),
+])
+def test_grid_form_matches_oracle(precision, nx, ny, shear, bend, iters, nsub, tol):
+    """The cooperative grid form (every CU on one cloth) against the FP64 oracle:
+    FP64 within 1e-10 relative, mixed32 within 1e-5 m."""
+    print(f"[case] grid form {precision} {nx}x{ny} iters {iters} nsub {nsub}", flush=True)
+    c = scenes.grid_cloth(nx, ny, 2300.0, shear=shear, bend=bend)
+    c.positions = c.positions + 1e-3 * scenes.SplitMix64(2).normals(3 * c.n)
+    c.pin_positions = c.positions.copy()
+    cfg = SimConfig.fixed_iterations(iters, 1.0 / 600.0)
+    e = Engine(precision, "fast")
+    e.add(c, cfg)
+    e.set_launch("grid")
+    perm = e.factor_arrays(0)["perm3"]
+    e.step(nsub)
+    x, v = e.state(0)
+    st = e.stats(0)
+    assert st["substeps"] == nsub and st["factorize_count"] == 1
+    o = Oracle(c, cfg, perm, threads=4)
+    o.step(nsub)
+    xo = o.positions()
+    err = float(np.max(np.abs(x - xo)))
+    print("max position difference", err, "relative", err / np.max(np.abs(xo)))
+    if tol is not None:
+        assert err <= tol * np.max(np.abs(xo))
+    else:
+        assert err <= 1e-5
+    # the grid solve against the oracle's solve
+    b = np.random.default_rng(1).standard_normal(3 * c.n)
+    xs = e.solve(0, b)
+    xr = o.solve(b)
+    assert np.max(np.abs(xs - xr)) <= (1e-12 if precision == "fp64" else 1e-4) * np.max(np.abs(xr))
+    e.close()
