@@ -19,12 +19,12 @@
 
 namespace pdx {
 hipError_t launchSubstep(const InstDev* d, const int* order, int ninst, int nsub, int block,
-                         size_t lds, bool fp32, bool blocked, bool idx16, hipStream_t s, int subBase,
+                         size_t lds, bool fp32, bool blocked, bool idx16, bool wl, hipStream_t s, int subBase,
                          int flags);
 hipError_t launchAdjoint(const InstDev* d, const int* order, int ninst, int nsub, int subBase, int block,
-                         size_t lds, bool fp32, bool blocked, bool idx16, hipStream_t s);
+                         size_t lds, bool fp32, bool blocked, bool idx16, bool wl, hipStream_t s);
 hipError_t launchSolve(const InstDev* d, int inst, const double* b, double* x, int block,
-                       size_t lds, bool fp32, bool blocked, bool idx16, hipStream_t s);
+                       size_t lds, bool fp32, bool blocked, bool idx16, bool wl, hipStream_t s);
 hipError_t setLdsLimit(size_t bytes);
 hipError_t launchGrid(const InstDev* d, int inst, int nsub, double* gslots, int waveRuns, int blocks,
                       int block, size_t lds, bool fp32, bool idx16, hipStream_t s);
@@ -955,7 +955,7 @@ int launch(pdx_ctx* c, int nsub) {
   }
   HIPCHECK(c, hipEventRecord(c->ev0, c->stream));
   HIPCHECK(c, launchSubstep(c->dInst, c->dOrder, (int)c->inst.size(), nsub, c->block, c->lds,
-                            c->prec == PDX_PREC_MIXED32, c->mode == PDX_SOLVE_FAST, c->idx16, c->stream, 0, 0));
+                            c->prec == PDX_PREC_MIXED32, c->mode == PDX_SOLVE_FAST, c->idx16, c->ldsW, c->stream, 0, 0));
   HIPCHECK(c, hipEventRecord(c->ev1, c->stream));
   c->lastLaunches = 1;
   return PDX_OK;
@@ -1083,7 +1083,7 @@ int pdx_step_graph(pdx_ctx* c, int nsub, int replays) {
   hipGraphExec_t ge = nullptr;
   HIPCHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   const hipError_t le = launchSubstep(c->dInst, c->dOrder, (int)c->inst.size(), nsub, c->block, c->lds,
-                                      c->prec == PDX_PREC_MIXED32, c->mode == PDX_SOLVE_FAST, c->idx16, c->stream,
+                                      c->prec == PDX_PREC_MIXED32, c->mode == PDX_SOLVE_FAST, c->idx16, c->ldsW, c->stream,
                                       0, 0);
   HIPCHECK(c, hipStreamEndCapture(c->stream, &g));
   if (le != hipSuccess) {
@@ -1229,7 +1229,7 @@ int pdx_solve3(pdx_ctx* c, int i, const double* b, double* x) {
                                 c->prec == PDX_PREC_MIXED32, c->idx16, c->stream));
   } else {
     HIPCHECK(c, launchSolve(c->dInst, i, c->dScratchB, c->dScratchX, c->block, c->lds,
-                            c->prec == PDX_PREC_MIXED32, c->mode == PDX_SOLVE_FAST, c->idx16, c->stream));
+                            c->prec == PDX_PREC_MIXED32, c->mode == PDX_SOLVE_FAST, c->idx16, c->ldsW, c->stream));
   }
   HIPCHECK(c, hipMemcpyAsync(x, c->dScratchX, (size_t)n3 * 8, hipMemcpyDeviceToHost, c->stream));
   HIPCHECK(c, hipStreamSynchronize(c->stream));
@@ -1293,7 +1293,7 @@ size_t alignUp(size_t b) { return (b + 255) / 256 * 256; }
 
 int rollLaunch(pdx_ctx* c, int nsub, int subBase, int flags) {
   HIPCHECK(c, launchSubstep(c->dInst, c->dOrder, (int)c->inst.size(), nsub, c->block, c->lds,
-                            c->prec == PDX_PREC_MIXED32, c->mode == PDX_SOLVE_FAST, c->idx16, c->stream,
+                            c->prec == PDX_PREC_MIXED32, c->mode == PDX_SOLVE_FAST, c->idx16, c->ldsW, c->stream,
                             subBase, flags));
   return PDX_OK;
 }
@@ -1506,7 +1506,7 @@ int pdx_adjoint(pdx_ctx* c) {
     if ((rc = rollCheckpoint(c, seg, false))) return rc;
     if ((rc = rollLaunch(c, len, s0, kRollTraj | kRollTape | kRollFixed))) return rc;
     HIPCHECK(c, launchAdjoint(c->dInst, c->dOrder, (int)ni, len, s0, c->block, c->lds,
-                              c->prec == PDX_PREC_MIXED32, c->mode == PDX_SOLVE_FAST, c->idx16, c->stream));
+                              c->prec == PDX_PREC_MIXED32, c->mode == PDX_SOLVE_FAST, c->idx16, c->ldsW, c->stream));
   }
   HIPCHECK(c, hipEventRecord(c->ev1, c->stream));
   // back to the state after the rollout, pins where the trajectory ended

@@ -48,10 +48,17 @@ __device__ inline double blockMax(double v, double* red) {
   return r;
 }
 
+// Address-space-qualified pointers: LDS accesses through them compile to ds_*
+// (tracked by the LDS counter only) and global ones to global_* (vector-memory
+// counter only).  Generic pointers would compile to flat accesses, which count on
+// both and make every LDS gather wait for the factor prefetch in flight.
+#define PDX_GLOBAL __attribute__((address_space(1)))
+#define PDX_LDS __attribute__((address_space(3)))
+
 template <typename Real>
-__device__ inline Real* ldsAt(int off) {
+__device__ inline PDX_LDS Real* ldsAt(int off) {
   extern __shared__ double smem[];
-  return reinterpret_cast<Real*>(reinterpret_cast<char*>(smem) + off);
+  return (PDX_LDS Real*)((PDX_LDS char*)smem + off);
 }
 
 // Barrier that orders LDS only.  __syncthreads() also waits for every
@@ -74,11 +81,13 @@ __device__ inline float subMul<float>(float acc, float l, float w) {
   return __builtin_fmaf(-l, w, acc);
 }
 
+constexpr int kU = 8;  // entries per load group of the level sweep
+
 // One level-scheduled sweep.  Forward: row k gathers L(k,j) y_j over ascending j.
 // Backward: column k gathers L(i,k) x_i over ascending i after scaling by 1/D_k.
 // `store(k, a0, a1, a2)` is called once per row with its final value.
-template <typename Real, bool kFwd, typename Store>
-__device__ void sweep(const SweepDev& sw, Real* W, const Real* dinv, Store store) {
+template <typename Real, bool kFwd, typename WP, typename Store>
+__device__ void sweep(const SweepDev& sw, WP W, const Real* dinv, Store store) {
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = threadIdx.x / kWave;
   const int nw = blockDim.x / kWave;
@@ -107,18 +116,36 @@ __device__ void sweep(const SweepDev& sw, Real* W, const Real* dinv, Store store
           a2 = a2 * dv;
         }
       }
+      // Entries are taken kU at a time: every load of a group (column, value,
+      // the three W values) is issued before the group's subtractions, which
+      // then run in ascending entry order as before.
       int off = ebase;
-      for (int j = 0; j < maxlen; ++j) {
-        const unsigned long long msk = __ballot(j < len);
-        if (j < len) {
-          const int e = off + lane;
-          const int c = sw.col[e];
-          const Real lv = val[e];
-          a0 = subMul<Real>(a0, lv, W[3 * c + 0]);
-          a1 = subMul<Real>(a1, lv, W[3 * c + 1]);
-          a2 = subMul<Real>(a2, lv, W[3 * c + 2]);
+      for (int j = 0; j < maxlen; j += kU) {
+        int cc[kU];
+        Real lv[kU], w0[kU], w1[kU], w2[kU];
+        bool ok[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+          ok[u] = (j + u) < len;
+          const unsigned long long msk = __ballot(ok[u]);
+          const int e = ok[u] ? off + lane : ebase;  // ebase holds an entry whenever maxlen > 0
+          off += __popcll(msk);
+          cc[u] = sw.col[e];
+          lv[u] = val[e];
         }
-        off += __popcll(msk);
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+          w0[u] = W[3 * cc[u] + 0];
+          w1[u] = W[3 * cc[u] + 1];
+          w2[u] = W[3 * cc[u] + 2];
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u)
+          if (ok[u]) {
+            a0 = subMul<Real>(a0, lv[u], w0[u]);
+            a1 = subMul<Real>(a1, lv[u], w1[u]);
+            a2 = subMul<Real>(a2, lv[u], w2[u]);
+          }
       }
       if (lane < nrows) {
         W[3 * k + 0] = a0;
@@ -160,7 +187,6 @@ struct RunRaw {
 // Global-address-space views: loads through them compile to global_load, which
 // only the vector-memory counter tracks, so an LDS barrier does not drain them
 // (flat loads also count on the LDS counter).
-#define PDX_GLOBAL __attribute__((address_space(1)))
 template <typename T>
 __device__ inline const PDX_GLOBAL T* gview(const void* p) {
   return (const PDX_GLOBAL T*)(p);
@@ -193,8 +219,8 @@ __device__ inline void loadRun(const void* colv, const Real* val, int r, RunRaw<
   }
 }
 
-template <typename Real, bool kIdx16>
-__device__ inline void runPartial(const RunRaw<Real, kIdx16>& b, const Real* W, Real* P) {
+template <typename Real, bool kIdx16, typename WP, typename PP>
+__device__ inline void runPartial(const RunRaw<Real, kIdx16>& b, WP W, PP P) {
   Real p0 = 0, p1 = 0, p2 = 0;
 #pragma unroll
   for (int q = 0; q < kRun; ++q) {
@@ -229,9 +255,10 @@ struct SweepTimes {
   long long stream = 0, wait1 = 0, fin = 0, wait2 = 0;
 };
 
-template <typename Real, bool kFwd, bool kIdx16, typename Store>
-__device__ void blockSweep(const BlockSweepDev& sw, Real* W, Real* P, Real* D, const int* steps,
-                           const Real* dinvUnused, Store store, SweepTimes& tstream, bool wInLds) {
+template <typename Real, bool kFwd, bool kIdx16, typename WP, typename Store>
+__device__ void blockSweep(const BlockSweepDev& sw, WP W, PDX_LDS Real* P, PDX_LDS Real* D,
+                           const PDX_LDS int* steps, const Real* dinvUnused, Store store, SweepTimes& tstream,
+                           bool wInLds) {
   (void)dinvUnused;
   auto barrier = [wInLds]() {
     if (wInLds)
@@ -245,9 +272,9 @@ __device__ void blockSweep(const BlockSweepDev& sw, Real* W, Real* P, Real* D, c
   const int nw = bs / kWave;
   const int ns = sw.nstep;
   if (ns <= 0) return;
-  const int* sRun = steps;
-  const int* sDense = steps + (ns + 1);
-  const int* sTask = steps + 2 * (ns + 1);
+  const PDX_LDS int* sRun = steps;
+  const PDX_LDS int* sDense = steps + (ns + 1);
+  const PDX_LDS int* sTask = steps + 2 * (ns + 1);
   const Real* rval = static_cast<const Real*>(sw.runVal);
   const Real* dense = static_cast<const Real*>(sw.dense);
   const v4i* meta = reinterpret_cast<const v4i*>(sw.laneMeta);
@@ -351,7 +378,7 @@ __device__ void blockSweep(const BlockSweepDev& sw, Real* W, Real* P, Real* D, c
         }
       }
       if (chain) {
-        const Real* Dt = D + (info >> 8);
+        const PDX_LDS Real* Dt = D + (info >> 8);
         Real dreg[kCap];
 #pragma unroll
         for (int q = 0; q < kCap; ++q) dreg[q] = (act && q < nrows) ? Dt[q * nrows + lane] : Real(0);
@@ -401,7 +428,7 @@ __device__ void blockSweep(const BlockSweepDev& sw, Real* W, Real* P, Real* D, c
 
 // Copies the step tables (run, dense and task offsets) of both sweeps into LDS.
 __device__ inline void loadStepTables(const InstDev& I) {
-  int* dst = ldsAt<int>(I.stepOff);
+  PDX_LDS int* dst = ldsAt<int>(I.stepOff);
   const BlockSweepDev* sws[2] = {&I.bfwd, &I.bbwd};
   for (int q = 0; q < 2; ++q) {
     const int n1 = sws[q]->nstep + 1;
@@ -634,35 +661,35 @@ __device__ inline double residualAndRhs(const InstDev& I, int v, Acc3* rhs) {
   return rhs ? residualAndRhsT<true>(I, v, *rhs) : residualAndRhsT<false>(I, v, dummy);
 }
 
-template <typename Real>
-__device__ inline void putW(Real* W, int r, double a, double b, double c) {
+template <typename Real, typename WP>
+__device__ inline void putW(WP W, int r, double a, double b, double c) {
   W[3 * r + 0] = (Real)a;
   W[3 * r + 1] = (Real)b;
   W[3 * r + 2] = (Real)c;
 }
 
 // Fills W with the first right-hand side of a substep (iteration 0 at x_n).
-template <typename Real, bool kExact>
-__device__ void gatherFirst(const InstDev& I, Real* W) {
+template <typename Real, bool kExact, typename WP>
+__device__ void gatherFirst(const InstDev& I, WP W) {
   for (int v = threadIdx.x; v < I.n; v += blockDim.x) {
     const int r = I.vrank[v];
     if (kExact) {
       const Acc3 b = rhsExact(I, v);
-      putW(W, r, b.x, b.y, b.z);
+      putW<Real>(W, r, b.x, b.y, b.z);
     } else if (I.pinned[v]) {
-      putW(W, r, I.pinpos[3 * v] - I.x[3 * v], I.pinpos[3 * v + 1] - I.x[3 * v + 1],
+      putW<Real>(W, r, I.pinpos[3 * v] - I.x[3 * v], I.pinpos[3 * v + 1] - I.x[3 * v + 1],
            I.pinpos[3 * v + 2] - I.x[3 * v + 2]);
     } else {
       Acc3 rh;
       (void)residualAndRhs(I, v, &rh);
-      putW(W, r, rh.x, rh.y, rh.z);
+      putW<Real>(W, r, rh.x, rh.y, rh.z);
     }
   }
 }
 
 
-template <typename Real, bool kBlocked, bool kIdx16>
-__device__ void runForward(const InstDev& I, Real* W, SweepTimes& tst) {
+template <typename Real, bool kBlocked, bool kIdx16, typename WP>
+__device__ void runForward(const InstDev& I, WP W, SweepTimes& tst) {
   const Real* dinv = static_cast<const Real*>(I.dinv);
   auto none = [](int, Real, Real, Real) {};
   if (kBlocked)
@@ -671,8 +698,8 @@ __device__ void runForward(const InstDev& I, Real* W, SweepTimes& tst) {
     sweep<Real, true>(I.fwd, W, dinv, none);
 }
 
-template <typename Real, bool kExact, bool kBlocked, bool kIdx16>
-__device__ void runBackward(const InstDev& I, Real* W, SweepTimes& tst) {
+template <typename Real, bool kExact, bool kBlocked, bool kIdx16, typename WP>
+__device__ void runBackward(const InstDev& I, WP W, SweepTimes& tst) {
   const Real* dinv = static_cast<const Real*>(I.dinv);
   double* x = I.x;
   const int* rankv = I.rankv;
@@ -751,15 +778,10 @@ constexpr int kRollTraj = 1;   // pin targets of each substep from the pin traje
 constexpr int kRollTape = 2;   // record every iterate of every substep (FP64)
 constexpr int kRollFixed = 4;  // fixed iteration count: no residual, no stop test
 
-template <typename Real, bool kExact, bool kBlocked, bool kIdx16, bool kRoll>
-__global__ void __launch_bounds__(512) substepKernel(const InstDev* __restrict__ insts,
-                                                      const int* __restrict__ order, int nsub,
-                                                      int subBase, int flags) {
-  if (!kRoll) flags = 0;
+template <typename Real, bool kExact, bool kBlocked, bool kIdx16, bool kRoll, typename WP>
+__device__ void substepBody(const InstDev& I, WP W, int nsub, int subBase, int flags) {
   extern __shared__ double smem[];
-  const InstDev& I = insts[order[blockIdx.x]];
   double* red = smem;  // 2 x 16 slots
-  Real* W = I.ldsW ? reinterpret_cast<Real*>(smem + 32) : static_cast<Real*>(I.Wg);
   const int n = I.n;
   const int tid = threadIdx.x, bs = blockDim.x;
   const bool fixedIter = kRoll && (flags & kRollFixed) != 0;
@@ -845,17 +867,17 @@ __global__ void __launch_bounds__(512) substepKernel(const InstDev* __restrict__
           if (wantRes && !pinnedV && I.mass[v] > 0.0) lres = fmax(lres, residualAndRhs(I, v, nullptr));
           if (more) {
             const Acc3 b = rhsExact(I, v);
-            putW(W, r, b.x, b.y, b.z);
+            putW<Real>(W, r, b.x, b.y, b.z);
           }
         } else {
           if (pinnedV) {
-            if (more) putW(W, r, I.pinpos[3 * v] - x0, I.pinpos[3 * v + 1] - x1, I.pinpos[3 * v + 2] - x2);
+            if (more) putW<Real>(W, r, I.pinpos[3 * v] - x0, I.pinpos[3 * v + 1] - x1, I.pinpos[3 * v + 2] - x2);
           } else {
             if (more || wantRes) {
               Acc3 rh;
               const double rv = residualAndRhs(I, v, more ? &rh : nullptr);
               if (wantRes && I.mass[v] > 0.0) lres = fmax(lres, rv);
-              if (more) putW(W, r, rh.x, rh.y, rh.z);
+              if (more) putW<Real>(W, r, rh.x, rh.y, rh.z);
             }
           }
         }
@@ -926,6 +948,20 @@ __global__ void __launch_bounds__(512) substepKernel(const InstDev* __restrict__
   }
 }
 
+// W in LDS (kWL) or in HBM is a separate kernel: one kernel holding both bodies
+// spills registers that neither body needs on its own.
+template <typename Real, bool kExact, bool kBlocked, bool kIdx16, bool kRoll, bool kWL>
+__global__ void __launch_bounds__(512) substepKernel(const InstDev* __restrict__ insts,
+                                                      const int* __restrict__ order, int nsub,
+                                                      int subBase, int flags) {
+  if (!kRoll) flags = 0;
+  const InstDev& I = insts[order[blockIdx.x]];
+  if (kWL)
+    substepBody<Real, kExact, kBlocked, kIdx16, kRoll>(I, ldsAt<Real>(32 * sizeof(double)), nsub, subBase, flags);
+  else
+    substepBody<Real, kExact, kBlocked, kIdx16, kRoll>(I, (PDX_GLOBAL Real*)I.Wg, nsub, subBase, flags);
+}
+
 // Reverse mode of `nsub` substeps (global indices subBase ..), one workgroup per
 // instance, from the iterates recorded by the substep kernel (kRollTape) and the
 // adjoint (xbar, vbar) of the state after the last of them; leaves the adjoint of
@@ -939,13 +975,8 @@ __global__ void __launch_bounds__(512) substepKernel(const InstDev* __restrict__
 //   gbend_s += -mu^T (dL/dk_s) x^{k+1}                        (one thread per stencil)
 // then bbar feeds x_n, v_n (prediction), the pins and the stiffnesses.  Every
 // sum has a fixed order; nothing is accumulated atomically.
-template <typename Real, bool kBlocked, bool kIdx16>
-__global__ void __launch_bounds__(512) adjointKernel(const InstDev* __restrict__ insts,
-                                                      const int* __restrict__ order, int nsub,
-                                                      int subBase) {
-  extern __shared__ double smem[];
-  const InstDev& I = insts[order[blockIdx.x]];
-  Real* W = I.ldsW ? reinterpret_cast<Real*>(smem + 32) : static_cast<Real*>(I.Wg);
+template <typename Real, bool kBlocked, bool kIdx16, typename WP>
+__device__ void adjointBody(const InstDev& I, WP W, int nsub, int subBase) {
   const int n = I.n, K = I.maxIter;
   const int tid = threadIdx.x, bs = blockDim.x;
   const size_t n3 = (size_t)3 * n;
@@ -982,7 +1013,7 @@ __global__ void __launch_bounds__(512) adjointKernel(const InstDev* __restrict__
     const double* tp = I.tape + (size_t)sub * (K + 1) * n3;
     for (int k = K - 1; k >= 0; --k) {
       for (int v = tid; v < n; v += bs)
-        putW(W, I.vrank[v], I.cur[3 * v], I.cur[3 * v + 1], I.cur[3 * v + 2]);
+        putW<Real>(W, I.vrank[v], I.cur[3 * v], I.cur[3 * v + 1], I.cur[3 * v + 2]);
       __syncthreads();
       if (kBlocked) {
         blockSweep<Real, true, kIdx16>(I.bfwd, W, ldsAt<Real>(I.partOff), ldsAt<Real>(I.denseOff),
@@ -1145,6 +1176,17 @@ __global__ void __launch_bounds__(512) adjointKernel(const InstDev* __restrict__
   }
 }
 
+template <typename Real, bool kBlocked, bool kIdx16, bool kWL>
+__global__ void __launch_bounds__(512) adjointKernel(const InstDev* __restrict__ insts,
+                                                      const int* __restrict__ order, int nsub,
+                                                      int subBase) {
+  const InstDev& I = insts[order[blockIdx.x]];
+  if (kWL)
+    adjointBody<Real, kBlocked, kIdx16>(I, ldsAt<Real>(32 * sizeof(double)), nsub, subBase);
+  else
+    adjointBody<Real, kBlocked, kIdx16>(I, (PDX_GLOBAL Real*)I.Wg, nsub, subBase);
+}
+
 // ---------------------------------------------------------------------------
 // Cooperative form for one large instance: the whole grid (one workgroup per CU,
 // all co-resident) advances a single cloth.  Vertex phases are grid-strided; a
@@ -1169,7 +1211,8 @@ __device__ inline double gridMax(cg::grid_group& grid, double v, double* red, do
 }
 
 template <typename Real, bool kFwd, bool kIdx16, typename Store>
-__device__ void gridSweep(cg::grid_group& grid, const BlockSweepDev& sw, Real* W, Real* Pw, Store store) {
+__device__ void gridSweep(cg::grid_group& grid, const BlockSweepDev& sw, PDX_GLOBAL Real* W, PDX_LDS Real* Pw,
+                          Store store) {
   const int lane = threadIdx.x & (kWave - 1);
   const int nw = blockDim.x / kWave;
   const int gw = blockIdx.x * nw + (int)(threadIdx.x / kWave);
@@ -1266,8 +1309,8 @@ __global__ void __launch_bounds__(512) gridSubstepKernel(const InstDev* __restri
   cg::grid_group grid = cg::this_grid();
   const InstDev& I = insts[inst];
   double* red = smem;  // 2 x 16 slots
-  Real* Pw = reinterpret_cast<Real*>(smem + 32) + (size_t)(threadIdx.x / kWave) * waveRuns * 3;
-  Real* W = static_cast<Real*>(I.Wg);
+  PDX_LDS Real* Pw = ldsAt<Real>(32 * sizeof(double)) + (size_t)(threadIdx.x / kWave) * waveRuns * 3;
+  PDX_GLOBAL Real* W = (PDX_GLOBAL Real*)I.Wg;
   const int n = I.n;
   const int gt = blockIdx.x * blockDim.x + threadIdx.x;
   const int gs = gridDim.x * blockDim.x;
@@ -1290,14 +1333,14 @@ __global__ void __launch_bounds__(512) gridSubstepKernel(const InstDev* __restri
       const int r = I.vrank[v];
       if (kExact) {
         const Acc3 b = rhsExact(I, v);
-        putW(W, r, b.x, b.y, b.z);
+        putW<Real>(W, r, b.x, b.y, b.z);
       } else if (I.pinned[v]) {
-        putW(W, r, I.pinpos[3 * v] - I.x[3 * v], I.pinpos[3 * v + 1] - I.x[3 * v + 1],
+        putW<Real>(W, r, I.pinpos[3 * v] - I.x[3 * v], I.pinpos[3 * v + 1] - I.x[3 * v + 1],
              I.pinpos[3 * v + 2] - I.x[3 * v + 2]);
       } else {
         Acc3 rh;
         (void)residualAndRhs(I, v, &rh);
-        putW(W, r, rh.x, rh.y, rh.z);
+        putW<Real>(W, r, rh.x, rh.y, rh.z);
       }
     }
     double scale = gridMax(grid, lscale, red, gslots, buf);
@@ -1351,16 +1394,16 @@ __global__ void __launch_bounds__(512) gridSubstepKernel(const InstDev* __restri
           if (I.wantResidual && !pinnedV && I.mass[v] > 0.0) lres = fmax(lres, residualAndRhs(I, v, nullptr));
           if (more) {
             const Acc3 b = rhsExact(I, v);
-            putW(W, r, b.x, b.y, b.z);
+            putW<Real>(W, r, b.x, b.y, b.z);
           }
         } else {
           if (pinnedV) {
-            if (more) putW(W, r, I.pinpos[3 * v] - x0, I.pinpos[3 * v + 1] - x1, I.pinpos[3 * v + 2] - x2);
+            if (more) putW<Real>(W, r, I.pinpos[3 * v] - x0, I.pinpos[3 * v + 1] - x1, I.pinpos[3 * v + 2] - x2);
           } else if (more || I.wantResidual) {
             Acc3 rh;
             const double rv = residualAndRhs(I, v, more ? &rh : nullptr);
             if (I.wantResidual && I.mass[v] > 0.0) lres = fmax(lres, rv);
-            if (more) putW(W, r, rh.x, rh.y, rh.z);
+            if (more) putW<Real>(W, r, rh.x, rh.y, rh.z);
           }
         }
       }
@@ -1415,11 +1458,11 @@ __global__ void __launch_bounds__(512) gridSolveKernel(const InstDev* __restrict
   extern __shared__ double smem[];
   cg::grid_group grid = cg::this_grid();
   const InstDev& I = insts[inst];
-  Real* Pw = reinterpret_cast<Real*>(smem + 32) + (size_t)(threadIdx.x / kWave) * waveRuns * 3;
-  Real* W = static_cast<Real*>(I.Wg);
+  PDX_LDS Real* Pw = ldsAt<Real>(32 * sizeof(double)) + (size_t)(threadIdx.x / kWave) * waveRuns * 3;
+  PDX_GLOBAL Real* W = (PDX_GLOBAL Real*)I.Wg;
   const int gt = blockIdx.x * blockDim.x + threadIdx.x;
   const int gs = gridDim.x * blockDim.x;
-  for (int v = gt; v < I.n; v += gs) putW(W, I.vrank[v], b[3 * v], b[3 * v + 1], b[3 * v + 2]);
+  for (int v = gt; v < I.n; v += gs) putW<Real>(W, I.vrank[v], b[3 * v], b[3 * v + 1], b[3 * v + 2]);
   grid.sync();
   auto none = [](int, Real, Real, Real) {};
   const int* rankv = I.rankv;
@@ -1433,18 +1476,27 @@ __global__ void __launch_bounds__(512) gridSolveKernel(const InstDev* __restrict
   gridSweep<Real, false, kIdx16>(grid, I.bbwd, W, Pw, put);
 }
 
+template <typename Real, bool kBlocked, bool kIdx16, typename WP>
+__device__ void solveBody(const InstDev& I, WP W, const double* __restrict__ b, double* __restrict__ xout);
+
 // Stand-alone global solve x = L^{-1} b (EigenDirectSolver::solve) on one instance.
-template <typename Real, bool kBlocked, bool kIdx16>
+template <typename Real, bool kBlocked, bool kIdx16, bool kWL>
 __global__ void __launch_bounds__(512) solveKernel(const InstDev* __restrict__ insts, int inst,
                                                     const double* __restrict__ b,
                                                     double* __restrict__ xout) {
-  extern __shared__ double smem[];
   const InstDev& I = insts[inst];
-  Real* W = I.ldsW ? reinterpret_cast<Real*>(smem + 32) : static_cast<Real*>(I.Wg);
+  if (kWL)
+    solveBody<Real, kBlocked, kIdx16>(I, ldsAt<Real>(32 * sizeof(double)), b, xout);
+  else
+    solveBody<Real, kBlocked, kIdx16>(I, (PDX_GLOBAL Real*)I.Wg, b, xout);
+}
+
+template <typename Real, bool kBlocked, bool kIdx16, typename WP>
+__device__ void solveBody(const InstDev& I, WP W, const double* __restrict__ b, double* __restrict__ xout) {
   if (kBlocked) loadStepTables(I);
   for (int v = threadIdx.x; v < I.n; v += blockDim.x) {
     const int r = I.vrank[v];
-    putW(W, r, b[3 * v], b[3 * v + 1], b[3 * v + 2]);
+    putW<Real>(W, r, b[3 * v], b[3 * v + 1], b[3 * v + 2]);
   }
   __syncthreads();
   const Real* dinv = static_cast<const Real*>(I.dinv);
@@ -1474,17 +1526,23 @@ __global__ void __launch_bounds__(512) solveKernel(const InstDev* __restrict__ i
   X(double, true, true, true) X(double, true, true, false) X(double, true, false, false)
 
 hipError_t launchSubstep(const InstDev* d, const int* order, int ninst, int nsub, int block,
-                         size_t lds, bool fp32, bool blocked, bool idx16, hipStream_t s, int subBase,
+                         size_t lds, bool fp32, bool blocked, bool idx16, bool wl, hipStream_t s, int subBase,
                          int flags) {
-#define PDX_L(R, E, B, X16)                                                                            \
-  if (fp32 == (sizeof(R) == 4) && blocked == B && (idx16 && blocked) == X16) {                         \
-    if (flags)                                                                                         \
-      hipLaunchKernelGGL((substepKernel<R, E, B, X16, true>), dim3(ninst), dim3(block), lds, s, d,     \
-                         order, nsub, subBase, flags);                                                 \
-    else                                                                                               \
-      hipLaunchKernelGGL((substepKernel<R, E, B, X16, false>), dim3(ninst), dim3(block), lds, s, d,    \
-                         order, nsub, subBase, 0);                                                     \
-    return hipGetLastError();                                                                          \
+#define PDX_L(R, E, B, X16)                                                                              \
+  if (fp32 == (sizeof(R) == 4) && blocked == B && (idx16 && blocked) == X16) {                           \
+    if (flags && wl)                                                                                     \
+      hipLaunchKernelGGL((substepKernel<R, E, B, X16, true, true>), dim3(ninst), dim3(block), lds, s, d, \
+                         order, nsub, subBase, flags);                                                   \
+    else if (flags)                                                                                      \
+      hipLaunchKernelGGL((substepKernel<R, E, B, X16, true, false>), dim3(ninst), dim3(block), lds, s, d, \
+                         order, nsub, subBase, flags);                                                   \
+    else if (wl)                                                                                         \
+      hipLaunchKernelGGL((substepKernel<R, E, B, X16, false, true>), dim3(ninst), dim3(block), lds, s, d, \
+                         order, nsub, subBase, 0);                                                       \
+    else                                                                                                 \
+      hipLaunchKernelGGL((substepKernel<R, E, B, X16, false, false>), dim3(ninst), dim3(block), lds, s,  \
+                         d, order, nsub, subBase, 0);                                                    \
+    return hipGetLastError();                                                                            \
   }
   PDX_KERNEL_TABLE(PDX_L)
 #undef PDX_L
@@ -1492,12 +1550,16 @@ hipError_t launchSubstep(const InstDev* d, const int* order, int ninst, int nsub
 }
 
 hipError_t launchAdjoint(const InstDev* d, const int* order, int ninst, int nsub, int subBase, int block,
-                         size_t lds, bool fp32, bool blocked, bool idx16, hipStream_t s) {
-#define PDX_L(R, E, B, X16)                                                                           \
-  if (fp32 == (sizeof(R) == 4) && blocked == B && (idx16 && blocked) == X16) {                        \
-    hipLaunchKernelGGL((adjointKernel<R, B, X16>), dim3(ninst), dim3(block), lds, s, d, order, nsub, \
-                       subBase);                                                                      \
-    return hipGetLastError();                                                                         \
+                         size_t lds, bool fp32, bool blocked, bool idx16, bool wl, hipStream_t s) {
+#define PDX_L(R, E, B, X16)                                                                              \
+  if (fp32 == (sizeof(R) == 4) && blocked == B && (idx16 && blocked) == X16) {                           \
+    if (wl)                                                                                              \
+      hipLaunchKernelGGL((adjointKernel<R, B, X16, true>), dim3(ninst), dim3(block), lds, s, d, order,   \
+                         nsub, subBase);                                                                 \
+    else                                                                                                 \
+      hipLaunchKernelGGL((adjointKernel<R, B, X16, false>), dim3(ninst), dim3(block), lds, s, d, order,  \
+                         nsub, subBase);                                                                 \
+    return hipGetLastError();                                                                            \
   }
   PDX_KERNEL_TABLE(PDX_L)
 #undef PDX_L
@@ -1543,11 +1605,14 @@ int gridBlocksMax(int block, size_t lds, bool fp32, bool idx16) {
 }
 
 hipError_t launchSolve(const InstDev* d, int inst, const double* b, double* x, int block,
-                       size_t lds, bool fp32, bool blocked, bool idx16, hipStream_t s) {
-#define PDX_L(R, E, B, X16)                                                                       \
-  if (fp32 == (sizeof(R) == 4) && blocked == B && (idx16 && blocked) == X16) {                    \
-    hipLaunchKernelGGL((solveKernel<R, B, X16>), dim3(1), dim3(block), lds, s, d, inst, b, x);    \
-    return hipGetLastError();                                                                     \
+                       size_t lds, bool fp32, bool blocked, bool idx16, bool wl, hipStream_t s) {
+#define PDX_L(R, E, B, X16)                                                                            \
+  if (fp32 == (sizeof(R) == 4) && blocked == B && (idx16 && blocked) == X16) {                         \
+    if (wl)                                                                                            \
+      hipLaunchKernelGGL((solveKernel<R, B, X16, true>), dim3(1), dim3(block), lds, s, d, inst, b, x);  \
+    else                                                                                               \
+      hipLaunchKernelGGL((solveKernel<R, B, X16, false>), dim3(1), dim3(block), lds, s, d, inst, b, x); \
+    return hipGetLastError();                                                                          \
   }
   PDX_KERNEL_TABLE(PDX_L)
 #undef PDX_L
@@ -1556,21 +1621,22 @@ hipError_t launchSolve(const InstDev* d, int inst, const double* b, double* x, i
 
 hipError_t setLdsLimit(size_t bytes) {
   hipError_t e = hipSuccess;
-#define PDX_L(R, E, B, X16)                                                                                 \
-  e = hipFuncSetAttribute((const void*)substepKernel<R, E, B, X16, false>,                                \
-                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);                         \
-  if (e != hipSuccess) return e;                                                                           \
-  e = hipFuncSetAttribute((const void*)substepKernel<R, E, B, X16, true>,                                 \
-                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);                         \
-  if (e != hipSuccess) return e;                                                                           \
-  e = hipFuncSetAttribute((const void*)solveKernel<R, B, X16>, hipFuncAttributeMaxDynamicSharedMemorySize,     \
-                          (int)bytes);                                                                     \
-  if (e != hipSuccess) return e;                                                                           \
-  e = hipFuncSetAttribute((const void*)adjointKernel<R, B, X16>, hipFuncAttributeMaxDynamicSharedMemorySize,   \
-                          (int)bytes);                                                                     \
+  const hipFuncAttribute A = hipFuncAttributeMaxDynamicSharedMemorySize;
+#define PDX_A(F)                                                 \
+  e = hipFuncSetAttribute((const void*)(F), A, (int)bytes);       \
   if (e != hipSuccess) return e;
+#define PDX_L(R, E, B, X16)                                     \
+  PDX_A((substepKernel<R, E, B, X16, false, false>))            \
+  PDX_A((substepKernel<R, E, B, X16, false, true>))             \
+  PDX_A((substepKernel<R, E, B, X16, true, false>))             \
+  PDX_A((substepKernel<R, E, B, X16, true, true>))              \
+  PDX_A((solveKernel<R, B, X16, false>))                        \
+  PDX_A((solveKernel<R, B, X16, true>))                         \
+  PDX_A((adjointKernel<R, B, X16, false>))                      \
+  PDX_A((adjointKernel<R, B, X16, true>))
   PDX_KERNEL_TABLE(PDX_L)
 #undef PDX_L
+#undef PDX_A
 #define PDX_G(R, E, X16)                                                                                  \
   e = hipFuncSetAttribute((const void*)gridSubstepKernel<R, E, X16>, hipFuncAttributeMaxDynamicSharedMemorySize, \
                           (int)bytes);                                                                    \
