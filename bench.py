@@ -218,6 +218,85 @@ def adjoint_workload(args):
     return out
 
 
+def dropin_workload(args):
+    """Plain FP64 drop-in on the reference-method configurations, one cloth per
+    context, against the FP64 oracle (the reference CPU path restated) on the
+    same inputs: 40x40 distance only (10 iters x 600 substeps) and 100x100 with
+    shear and bend, sigma 1e-3 m position noise (40 iters x 300 substeps)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_ffi import Oracle
+    out = {}
+    for name, (c, cfg, nsub) in [("config1_40x40", scenes.config1()), ("config2_100x100", scenes.config2())]:
+        if args.dropin_substeps:
+            nsub = min(nsub, args.dropin_substeps)
+        res = {"substeps": nsub, "pd_iterations": cfg.max_iterations}
+        for solve in ("fast", "exact"):
+            e = Engine("fp64", solve)
+            e.add(c, cfg)
+            t0 = time.perf_counter()
+            e.factor()
+            tf = time.perf_counter() - t0
+            perm = e.factor_arrays(0)["perm3"]
+            ns = nsub if solve == "fast" else min(nsub, 600 if c.n < 5000 else 2)
+            t0 = time.perf_counter()
+            e.step(ns)
+            dt = time.perf_counter() - t0
+            x, _ = e.state(0)
+            st = e.stats(0)
+            o = Oracle(c, cfg, perm, threads=4)
+            o.factor()
+            t1 = time.perf_counter()
+            o.step(ns)
+            tc = time.perf_counter() - t1
+            xo = o.positions()
+            rel = float(np.max(np.abs(x - xo)) / np.max(np.abs(xo)))
+            res[solve] = {"gpu_ms_per_substep": round(1e3 * dt / ns, 4), "substeps_timed": ns,
+                          "max_rel_pos_diff_vs_oracle": rel, "bitwise_equal": bool(np.array_equal(x, xo)),
+                          "factorize_count": st["factorize_count"], "factor_and_upload_s": round(tf, 3),
+                          "cpu_oracle_ms_per_substep_4threads": round(1e3 * tc / ns, 4)}
+            log(f"{name} {solve}: GPU {1e3 * dt / ns:.3f} ms/substep, CPU oracle {1e3 * tc / ns:.3f} ms/substep, "
+                f"rel diff {rel:.3e}")
+            e.close()
+        out[name] = res
+    return out
+
+
+def large_workload(args):
+    """512x512 cloth (262,144 vertices, distance + shear + bend, four corners
+    pinned), 40 PD iterations: FP64 in the one-workgroup form (W in HBM) and
+    FP32-solve / FP64-residual in the cooperative grid form (every CU on the one
+    cloth), max position error of the FP32 form against FP64 after the run."""
+    c, cfg, nsub = scenes.config3()
+    nsub = min(nsub, args.large_substeps)
+    res = {"vertices": c.n, "substeps": nsub, "pd_iterations": cfg.max_iterations}
+    xs = {}
+    for prec, mode, g in (("fp64", "batch", 0), ("mixed32", "grid", 32)):
+        e = Engine(prec, "fast")
+        e.add(c, cfg)
+        if mode == "grid":
+            e.set_launch("grid", g)
+        t0 = time.perf_counter()
+        e.factor()
+        tf = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        e.step(nsub)
+        dt = time.perf_counter() - t0
+        xs[prec] = e.state(0)[0]
+        st = e.stats(0)
+        bps = e.bytes_per_substep()
+        res[prec] = {"form": mode if mode == "batch" else f"grid ({g} workgroups)",
+                     "ms_per_substep": round(1e3 * dt / nsub, 2),
+                     "cloth_substeps_per_s": round(nsub / dt, 4),
+                     "algorithmic_GB_per_substep": round(bps / 1e9, 3),
+                     "achieved_GB_s": round(bps / (dt / nsub) / 1e9, 2),
+                     "factor_nnz": st["factor_nnz"], "steps_per_sweep": [st["fwd_steps"], st["bwd_steps"]],
+                     "factor_and_upload_s": round(tf, 2)}
+        log(f"large {prec} {mode}: {1e3 * dt / nsub:.1f} ms/substep")
+        e.close()
+    res["max_pos_error_m_mixed32_vs_fp64"] = float(np.max(np.abs(xs["mixed32"] - xs["fp64"])))
+    return res
+
+
 def copy_calibration(mib, reps=4):
     """A plain device-to-device copy of known size (the runtime's copy kernel),
     used to scale the FETCH_SIZE / WRITE_SIZE counters to bytes."""
@@ -248,7 +327,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-dropin", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape (512 threads, chain cap 32)")
-    ap.add_argument("--workload", default="batch", choices=["batch", "adjoint"])
+    ap.add_argument("--workload", default="batch", choices=["batch", "adjoint", "dropin", "large"])
+    ap.add_argument("--dropin-substeps", type=int, default=0, help="cap on the drop-in configs' substeps")
+    ap.add_argument("--large-substeps", type=int, default=10)
     ap.add_argument("--adj-instances", type=int, default=64)
     args = ap.parse_args()
     if args.copy_calibration:
@@ -259,6 +340,14 @@ def main():
                                    "GPUs by design but no multi-GPU path is built (use --gpus 1)"}))
         sys.exit(2)
 
+    if args.workload == "dropin":
+        print(json.dumps({"metric": "plain FP64 drop-in vs the FP64 oracle (reference CPU path) per config",
+                          **dropin_workload(args)}), flush=True)
+        return
+    if args.workload == "large":
+        print(json.dumps({"metric": "512x512 single cloth, FP64 and FP32-solve", **large_workload(args)}),
+              flush=True)
+        return
     if args.workload == "adjoint":
         print(json.dumps({"metric": "differentiable batch gradient (adjoint) throughput", **adjoint_workload(args)}),
               flush=True)

@@ -19,6 +19,8 @@ SOURCES = ["kernels.hip", "context.cpp", "factor.cpp", "amd.cpp", "scene.cpp"]
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-Wall",
           "-Wno-unused-function", "-Wno-unused-result", "-Wno-unused-value"]
 DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+if os.environ.get("PDX_PHASE_TIMERS") == "1":
+    DEVICE.append("-DPDX_PHASE_TIMERS=1")  # per-phase device timers in pdx_get_stats (slower kernels)
 
 
 def _run(cmd):

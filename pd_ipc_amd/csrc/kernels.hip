@@ -27,6 +27,20 @@ namespace pdx {
 namespace {
 
 constexpr int kWave = 64;
+
+// Phase timers (device wall clock, 100 MHz) read per sweep step and per phase.
+// Reading the clock waits on the scalar-memory counter, which the LDS traffic
+// shares, so the timers are compiled in only on request (PDX_PHASE_TIMERS=1).
+#ifndef PDX_PHASE_TIMERS
+#define PDX_PHASE_TIMERS 0
+#endif
+__device__ inline long long tick() {
+#if PDX_PHASE_TIMERS
+  return wall_clock64();
+#else
+  return 0;
+#endif
+}
 constexpr double kMinLength = 1e-12;  // DistanceTerm.h:31
 
 __device__ inline double waveMax(double v) {
@@ -318,7 +332,7 @@ __device__ void blockSweep(const BlockSweepDev& sw, WP W, PDX_LDS Real* P, PDX_L
   };
   prefetch(0);
   for (int st = 0; st < ns; ++st) {
-    const long long ts0 = wall_clock64();
+    const long long ts0 = tick();
     const int base = sRun[st], end = sRun[st + 1];
     const int dbase = sDense[st], dend = sDense[st + 1];
     const int tb = sTask[st], te = sTask[st + 1];
@@ -339,9 +353,9 @@ __device__ void blockSweep(const BlockSweepDev& sw, WP W, PDX_LDS Real* P, PDX_L
     const v4i m0 = mpre[0], m1 = mpre[1];
     const Real dv0 = dvpre[0], dv1 = dvpre[1];
     if (st + 1 < ns) prefetch(st + 1);
-    const long long ts1 = wall_clock64();
+    const long long ts1 = tick();
     barrier();
-    const long long ts2 = wall_clock64();
+    const long long ts2 = tick();
     int slot = 0;
     for (int t = tb + wave; t < te; t += nw, ++slot) {
       v4i mt;
@@ -415,13 +429,13 @@ __device__ void blockSweep(const BlockSweepDev& sw, WP W, PDX_LDS Real* P, PDX_L
         store(k, a0, a1, a2);
       }
     }
-    const long long ts3 = wall_clock64();
+    const long long ts3 = tick();
     barrier();
     if (tid == 0) {
       tstream.stream += ts1 - ts0;
       tstream.wait1 += ts2 - ts1;
       tstream.fin += ts3 - ts2;
-      tstream.wait2 += wall_clock64() - ts3;
+      tstream.wait2 += tick() - ts3;
     }
   }
 }
@@ -795,7 +809,7 @@ __device__ void substepBody(const InstDev& I, WP W, int nsub, int subBase, int f
   for (int sub = 0; sub < nsub; ++sub) {
     if (kRoll && (flags & kRollTraj)) applyPinTrajectory(I, subBase + sub);
     double* tape = (kRoll && (flags & kRollTape)) ? I.tape + (size_t)sub * (I.maxIter + 1) * n3 : nullptr;
-    const long long tp0 = wall_clock64();
+    const long long tp0 = tick();
     // predict (Integrator.cpp:447-463) and the displacement scale
     double lscale = 0.0;
     for (int v = tid; v < n; v += bs) {
@@ -811,20 +825,20 @@ __device__ void substepBody(const InstDev& I, WP W, int nsub, int subBase, int f
     }
     double scale = blockMax(lscale, red);
     scale = fmax(scale, I.gScaleH2);
-    const long long tp1 = wall_clock64();
+    const long long tp1 = tick();
     gatherFirst<Real, kExact>(I, W);
     __syncthreads();
     if (tid == 0) {
       tacc[3] += tp1 - tp0;
-      tacc[0] += wall_clock64() - tp1;
+      tacc[0] += tick() - tp1;
     }
 
     int used = 0, conv = 0;
     double lastRes = -1.0;
     for (int k = 0; k < I.maxIter; ++k) {
-      if (tid == 0) tk[0] = wall_clock64();
+      if (tid == 0) tk[0] = tick();
       runForward<Real, kBlocked, kIdx16>(I, W, swt);
-      if (tid == 0) tk[1] = wall_clock64();
+      if (tid == 0) tk[1] = tick();
       runBackward<Real, kExact, kBlocked, kIdx16>(I, W, swt);
       if (kExact && kBlocked) __syncthreads();  // the sweep stored x; the gather reads it
       if (!kExact) {
@@ -844,7 +858,7 @@ __device__ void substepBody(const InstDev& I, WP W, int nsub, int subBase, int f
         }
         __syncthreads();
       }
-      if (tid == 0) tk[2] = wall_clock64();
+      if (tid == 0) tk[2] = tick();
       const bool more = (k + 1 < I.maxIter);
       double* tapeK = tape ? tape + (size_t)(k + 1) * n3 : nullptr;
       double ldiff = 0.0, lres = 0.0;
@@ -891,7 +905,7 @@ __device__ void substepBody(const InstDev& I, WP W, int nsub, int subBase, int f
       // criterion (Integrator.cpp:673-751); two slot sets alternate so one barrier suffices
       double* rs = red + 16 * ((k & 1) ^ 1);
       const double diff = blockMax(ldiff, rs);
-      if (tid == 0) tk[3] = wall_clock64();
+      if (tid == 0) tk[3] = tick();
       double residual = -1.0;
       if (I.wantResidual) {
         __syncthreads();
@@ -901,7 +915,7 @@ __device__ void substepBody(const InstDev& I, WP W, int nsub, int subBase, int f
         tacc[1] += tk[1] - tk[0];
         tacc[2] += tk[2] - tk[1];
         tacc[0] += tk[3] - tk[2];
-        tacc[4] += wall_clock64() - tk[3];
+        tacc[4] += tick() - tk[3];
       }
       const double rel = (scale > 0.0) ? diff / scale : diff;
       const bool absHit = (I.absTol > 0.0) && (diff <= I.absTol);
@@ -916,7 +930,7 @@ __device__ void substepBody(const InstDev& I, WP W, int nsub, int subBase, int f
       }
     }
     // velocity + damping + re-pin (Integrator.cpp:774-791)
-    const long long tv0 = wall_clock64();
+    const long long tv0 = tick();
     for (int v = tid; v < n; v += bs) {
       const bool pinnedV = I.pinned[v] != 0;
       for (int d = 0; d < 3; ++d) {
@@ -929,7 +943,7 @@ __device__ void substepBody(const InstDev& I, WP W, int nsub, int subBase, int f
       }
     }
     if (tid == 0) {
-      tacc[3] += wall_clock64() - tv0;
+      tacc[3] += tick() - tv0;
       I.istats[0] = used;
       I.istats[1] = conv;
       I.istats[2] += conv;
