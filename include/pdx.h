@@ -103,7 +103,8 @@ typedef struct {
   int bwd_steps;
   double analyze_seconds;
   double factorize_seconds;
-  /* device time of workgroup 0 of the instance per phase, summed over substeps:
+  /* device time of the instance's workgroup per phase, summed over substeps (zero
+   * unless the library was built with PDX_PHASE_TIMERS=1; batch form only):
    * gather (rhs + residual), forward sweep, backward sweep, predict + velocity,
    * criterion reductions; fast mode, both sweeps: run streaming of thread 0, its wait at
    * the step barrier, finalisation of wavefront 0, its wait at the second barrier */
