@@ -67,6 +67,9 @@ __device__ inline double blockMax(double v, double* red) {
 // counter only).  Generic pointers would compile to flat accesses, which count on
 // both and make every LDS gather wait for the factor prefetch in flight.
 #define PDX_GLOBAL __attribute__((address_space(1)))
+// The sweeps and gathers are inlined into their kernels: called as functions
+// they keep caller registers across the call boundary and spill them.
+#define PDX_INLINE __attribute__((always_inline))
 #define PDX_LDS __attribute__((address_space(3)))
 
 template <typename Real>
@@ -101,7 +104,7 @@ constexpr int kU = 8;  // entries per load group of the level sweep
 // Backward: column k gathers L(i,k) x_i over ascending i after scaling by 1/D_k.
 // `store(k, a0, a1, a2)` is called once per row with its final value.
 template <typename Real, bool kFwd, typename WP, typename Store>
-__device__ void sweep(const SweepDev& sw, WP W, const Real* dinv, Store store) {
+__device__ PDX_INLINE void sweep(const SweepDev& sw, WP W, const Real* dinv, Store store) {
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = threadIdx.x / kWave;
   const int nw = blockDim.x / kWave;
@@ -270,7 +273,7 @@ struct SweepTimes {
 };
 
 template <typename Real, bool kFwd, bool kIdx16, typename WP, typename Store>
-__device__ void blockSweep(const BlockSweepDev& sw, WP W, PDX_LDS Real* P, PDX_LDS Real* D,
+__device__ PDX_INLINE void blockSweep(const BlockSweepDev& sw, WP W, PDX_LDS Real* P, PDX_LDS Real* D,
                            const PDX_LDS int* steps, const Real* dinvUnused, Store store, SweepTimes& tstream,
                            bool wInLds) {
   (void)dinvUnused;
@@ -308,24 +311,30 @@ __device__ void blockSweep(const BlockSweepDev& sw, WP W, PDX_LDS Real* P, PDX_L
   const PDX_GLOBAL Real* gdense = gview<Real>(dense);
   const PDX_GLOBAL v4i* gmeta = gview<v4i>(meta);
   const PDX_GLOBAL Real* gldinv = gview<Real>(ldinv);
+  // Lanes past the step's end load the step's first item instead: all such
+  // lanes of a wavefront then hit one address and coalesce into one request,
+  // rather than fetching unrelated runs.
   auto prefetch = [&](int st) {
     const int rb = sRun[st], re = sRun[st + 1];
+    const int rfirst = min(rb, lastRun);
 #pragma unroll
     for (int q = 0; q < kPreRuns; ++q) {
       const int r = rb + tid + q * bs;
       havePre[q] = r < re;
-      loadRun<Real, kIdx16>(sw.runCol, rval, min(r, lastRun), pre[q]);
+      loadRun<Real, kIdx16>(sw.runCol, rval, havePre[q] ? r : rfirst, pre[q]);
     }
-    const int d = sDense[st] + 4 * tid;
+    const int db = sDense[st];
+    const int d = db + 4 * tid;
     haveD = d < sDense[st + 1];
-    const int dc = min(d, lastDense);
+    const int dc = haveD ? d : min(db, lastDense);
 #pragma unroll
     for (int q = 0; q < 4; ++q) dpre[q] = gdense[dc + q];
+    const int tb = sTask[st];
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
-      const int t = sTask[st] + wave + q * nw;
+      const int t = tb + wave + q * nw;
       haveM[q] = t < sTask[st + 1];
-      const size_t ti = (size_t)min(t, lastTask) * kWave + lane;
+      const size_t ti = (size_t)(haveM[q] ? t : min(tb, lastTask)) * kWave + lane;
       mpre[q] = gmeta[ti];
       if (!kFwd) dvpre[q] = gldinv[ti];
     }
@@ -684,7 +693,7 @@ __device__ inline void putW(WP W, int r, double a, double b, double c) {
 
 // Fills W with the first right-hand side of a substep (iteration 0 at x_n).
 template <typename Real, bool kExact, typename WP>
-__device__ void gatherFirst(const InstDev& I, WP W) {
+__device__ PDX_INLINE void gatherFirst(const InstDev& I, WP W) {
   for (int v = threadIdx.x; v < I.n; v += blockDim.x) {
     const int r = I.vrank[v];
     if (kExact) {
@@ -703,7 +712,7 @@ __device__ void gatherFirst(const InstDev& I, WP W) {
 
 
 template <typename Real, bool kBlocked, bool kIdx16, typename WP>
-__device__ void runForward(const InstDev& I, WP W, SweepTimes& tst) {
+__device__ PDX_INLINE void runForward(const InstDev& I, WP W, SweepTimes& tst) {
   const Real* dinv = static_cast<const Real*>(I.dinv);
   auto none = [](int, Real, Real, Real) {};
   if (kBlocked)
@@ -713,7 +722,7 @@ __device__ void runForward(const InstDev& I, WP W, SweepTimes& tst) {
 }
 
 template <typename Real, bool kExact, bool kBlocked, bool kIdx16, typename WP>
-__device__ void runBackward(const InstDev& I, WP W, SweepTimes& tst) {
+__device__ PDX_INLINE void runBackward(const InstDev& I, WP W, SweepTimes& tst) {
   const Real* dinv = static_cast<const Real*>(I.dinv);
   double* x = I.x;
   const int* rankv = I.rankv;
