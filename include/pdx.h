@@ -199,6 +199,11 @@ int pdx_get_gradients(pdx_ctx* ctx, int instance, double* g_x0, double* g_v0, do
 /* Device time (ms) of the last pdx_rollout and of the last pdx_adjoint. */
 int pdx_rollout_timing(const pdx_ctx* ctx, double* forward_ms, double* adjoint_ms);
 
+/* Fast-mode schedule of one instance, 8 values per sweep (forward, then backward):
+ * steps, tasks, chain tasks, rows in chains, runs, dense values, most tasks in
+ * one step, most runs in one step. */
+int pdx_get_schedule_stats(pdx_ctx* ctx, int instance, int64_t* out16);
+
 /* Launch form (PDX_LAUNCH_*); grid_blocks = workgroups of the grid form, 0 = all
  * that can be co-resident.  Re-packs the schedules for the form. */
 int pdx_set_launch(pdx_ctx* ctx, int mode, int grid_blocks);

@@ -101,6 +101,7 @@ struct HostBlockSweep {
   std::vector<int> slotRank, slotRunBeg, slotRunEnd;
   std::vector<int> laneMeta;                      // 64 lanes x {rank, run begin, run end, info} per task
   std::vector<double> laneDinv;                   // 64 lanes x 1/D of the lane's rank per task
+  std::vector<int> laneDense;                     // 64 lanes x offset of the lane's inverse block
   std::vector<int> runCol;                        // kRun per run
   std::vector<double> runVal, dense;
 };
@@ -181,50 +182,42 @@ void packBlocked(const Factor& f, bool fwd, int cap, int runCap, int stepRunCap,
     extRange(k, b, p0, p1);
     return p1 - p0;
   };
-  struct Task {
-    std::vector<int> ranks, blocks;
-    bool chain;
-    double cost;
+  // A task is one wavefront: up to 64 lanes holding whole blocks (elimination
+  // tree chains of <= cap rows, or single rows) in contiguous lane segments.
+  struct Item {
+    int block, len, runs;
   };
   out.stepTask.push_back(0);
   out.stepRun.push_back(0);
   out.stepDense.push_back(0);
   int steps = 0;
   for (int l = 0; l < out.nstep; ++l) {
-    std::vector<Task> tasks;
-    std::vector<std::pair<int, int>> singles;
+    std::vector<Item> items;
     for (int b : byLevel[(size_t)l]) {
-      const int len = bs[(size_t)b + 1] - bs[(size_t)b];
-      if (len == 1) {
-        singles.push_back({extLen(bs[(size_t)b], b), b});
-      } else {
-        Task t;
-        t.chain = true;
-        for (int k = bs[(size_t)b]; k < bs[(size_t)b + 1]; ++k) t.ranks.push_back(k);
-        t.blocks.assign((size_t)len, b);
-        t.cost = 4.0 * len;
-        tasks.push_back(t);
-      }
+      Item it{b, bs[(size_t)b + 1] - bs[(size_t)b], 0};
+      for (int k = bs[(size_t)b]; k < bs[(size_t)b + 1]; ++k) it.runs += (extLen(k, b) + kRun - 1) / kRun;
+      items.push_back(it);
     }
-    std::stable_sort(singles.begin(), singles.end(), [](auto& a, auto& b) { return a.first > b.first; });
-    // batches of <= 64 rows (one per lane) and <= runCap runs
-    for (size_t c0 = 0; c0 < singles.size();) {
-      Task t;
-      t.chain = false;
-      int tr = 0;
-      size_t q = c0;
-      for (; q < singles.size() && q < c0 + 64; ++q) {
-        const int r = (singles[q].first + kRun - 1) / kRun;
-        if (q > c0 && tr + r > runCap) break;
-        tr += r;
-        t.ranks.push_back(bs[(size_t)singles[q].second]);
-        t.blocks.push_back(singles[q].second);
+    // longest blocks first, then most runs: tasks fill with similar segments
+    std::stable_sort(items.begin(), items.end(), [](const Item& a, const Item& b) {
+      return a.len != b.len ? a.len > b.len : a.runs > b.runs;
+    });
+    std::vector<std::vector<Item>> tasks;
+    {
+      std::vector<Item> cur;
+      int lanes = 0, truns = 0;
+      for (const Item& it : items) {
+        if (!cur.empty() && (lanes + it.len > 64 || truns + it.runs > runCap)) {
+          tasks.push_back(cur);
+          cur.clear();
+          lanes = truns = 0;
+        }
+        cur.push_back(it);
+        lanes += it.len;
+        truns += it.runs;
       }
-      c0 = q;
-      t.cost = 1.0;
-      tasks.push_back(t);
+      if (!cur.empty()) tasks.push_back(cur);
     }
-    std::stable_sort(tasks.begin(), tasks.end(), [](const Task& a, const Task& b) { return a.cost > b.cost; });
     int runBase = out.stepRun.back();
     int denseBase = out.stepDense.back();
     int runs = 0;
@@ -243,69 +236,92 @@ void packBlocked(const Factor& f, bool fwd, int cap, int runCap, int stepRunCap,
       denseBase = out.stepDense.back();
       runs = 0;
     };
-    for (const Task& t : tasks) {
-      const int nr = (int)t.ranks.size();
-      {
-        int tr = 0;
-        for (int r = 0; r < nr; ++r) tr += (extLen(t.ranks[(size_t)r], t.blocks[(size_t)r]) + kRun - 1) / kRun;
-        const int td = t.chain ? nr * nr + 3 : 0;
-        out.maxTaskRuns = std::max(out.maxTaskRuns, tr);
-        const bool nonEmpty = (int)out.taskInfo.size() > out.stepTask.back();
-        if (nonEmpty && (runs + tr > stepRunCap || (int)(out.dense.size() - (size_t)denseBase) + td > denseCap))
-          closeStep();
+    for (const auto& t : tasks) {
+      int used = 0, tr = 0, td = 0, maxLen = 1;
+      for (const Item& it : t) {
+        used += it.len;
+        tr += it.runs;
+        if (it.len > 1) td += it.len * it.len;
+        maxLen = std::max(maxLen, it.len);
       }
-      out.taskInfo.push_back(nr | (t.chain ? (1 << 16) : 0));
+      out.maxTaskRuns = std::max(out.maxTaskRuns, tr);
+      const bool nonEmpty = (int)out.taskInfo.size() > out.stepTask.back();
+      if (nonEmpty && (runs + tr > stepRunCap || (int)(out.dense.size() - (size_t)denseBase) + td > denseCap))
+        closeStep();
+      out.taskInfo.push_back(used | (maxLen > 1 ? (1 << 16) : 0));
       out.taskSlot.push_back((int)out.slotRank.size());
-      out.taskDense.push_back(t.chain ? (int)out.dense.size() - denseBase : 0);
+      out.taskDense.push_back((int)out.dense.size() - denseBase);
+      // per lane: {rank or -1, run begin, run end, segment length | lane offset in
+      // the segment << 7 | lanes used << 14 | longest segment << 21}, 1/D, and the
+      // offset of the segment's inverse block in the step's dense range
       const size_t meta0 = out.laneMeta.size();
-      const int info = nr | (t.chain ? (1 << 7) : 0) | ((t.chain ? (int)out.dense.size() - denseBase : 0) << 8);
+      const int common = (used << 14) | (maxLen << 21);
       for (int ln = 0; ln < 64; ++ln) {
         out.laneMeta.push_back(-1);
         out.laneMeta.push_back(0);
         out.laneMeta.push_back(0);
-        out.laneMeta.push_back(info);
+        out.laneMeta.push_back(1 | common);
         out.laneDinv.push_back(0.0);
+        out.laneDense.push_back(0);
       }
-      for (int r = 0; r < nr; ++r) {
-        const int k = t.ranks[(size_t)r];
-        int p0, p1;
-        extRange(k, t.blocks[(size_t)r], p0, p1);
-        out.slotRank.push_back(k);
-        out.slotRunBeg.push_back(runs);
-        for (int p = p0; p < p1; p += kRun) {
-          for (int q = 0; q < kRun; ++q) {
-            if (p + q < p1) {
-              out.runCol.push_back(fwd ? f.rc[(size_t)(p + q)] : f.ci[(size_t)(p + q)]);
-              out.runVal.push_back(fwd ? f.rx[(size_t)(p + q)] : f.cx[(size_t)(p + q)]);
-            } else {
-              out.runCol.push_back(0);
-              out.runVal.push_back(0.0);
+      int lane = 0;
+      for (const Item& it : t) {
+        const int b = it.block, nr = it.len, s0 = bs[(size_t)b];
+        const int doff = (int)out.dense.size() - denseBase;
+        for (int r = 0; r < nr; ++r, ++lane) {
+          const int k = s0 + r;
+          int p0, p1;
+          extRange(k, b, p0, p1);
+          out.slotRank.push_back(k);
+          out.slotRunBeg.push_back(runs);
+          for (int p = p0; p < p1; p += kRun) {
+            for (int q = 0; q < kRun; ++q) {
+              if (p + q < p1) {
+                out.runCol.push_back(fwd ? f.rc[(size_t)(p + q)] : f.ci[(size_t)(p + q)]);
+                out.runVal.push_back(fwd ? f.rx[(size_t)(p + q)] : f.cx[(size_t)(p + q)]);
+              } else {
+                out.runCol.push_back(0);
+                out.runVal.push_back(0.0);
+              }
+            }
+            ++runs;
+          }
+          out.slotRunEnd.push_back(runs);
+          const size_t m = meta0 + 4 * (size_t)lane;
+          out.laneMeta[m + 0] = k;
+          out.laneMeta[m + 1] = out.slotRunBeg.back();
+          out.laneMeta[m + 2] = runs;
+          out.laneMeta[m + 3] = nr | (r << 7) | common;
+          out.laneDinv[meta0 / 4 + (size_t)lane] = f.dinv[(size_t)k];
+          out.laneDense[meta0 / 4 + (size_t)lane] = nr > 1 ? doff : 0;
+        }
+        if (nr > 1) {
+          // The chain's own unit lower triangle B (ranks s0 .. s0+nr-1) is stored
+          // inverted, X = B^-1 (FP64 on the host, unit diagonal included), so the
+          // segment finishes with one product per lane instead of nr dependent
+          // substitution rounds:  forward y = X r,  backward y = X^T r.
+          // Layout [s * nr + t] (lane offset t, term s):
+          //   forward  X(t, s)    (terms s <= t)
+          //   backward X(s, t)    (terms s >= t)
+          std::vector<double> blkL((size_t)nr * nr, 0.0), X((size_t)nr * nr, 0.0);
+          for (int r = 1; r < nr; ++r) {
+            const int k = s0 + r;
+            for (int p = f.rp[(size_t)k]; p < f.rp[(size_t)k + 1]; ++p) {
+              const int c = f.rc[(size_t)p] - s0;
+              if (c >= 0 && c < r) blkL[(size_t)r * nr + c] = f.rx[(size_t)p];
             }
           }
-          ++runs;
-        }
-        out.slotRunEnd.push_back(runs);
-        out.laneMeta[meta0 + 4 * (size_t)r + 0] = k;
-        out.laneMeta[meta0 + 4 * (size_t)r + 1] = out.slotRunBeg.back();
-        out.laneMeta[meta0 + 4 * (size_t)r + 2] = runs;
-        out.laneDinv[meta0 / 4 + (size_t)r] = f.dinv[(size_t)k];
-      }
-      if (t.chain) {
-        const int s0 = t.ranks[0];
-        std::vector<double> blkL((size_t)nr * nr, 0.0);
-        for (int r = 1; r < nr; ++r) {
-          const int k = s0 + r;
-          for (int p = f.rp[(size_t)k]; p < f.rp[(size_t)k + 1]; ++p) {
-            const int c = f.rc[(size_t)p] - s0;
-            if (c >= 0 && c < r) blkL[(size_t)r * nr + c] = f.rx[(size_t)p];
+          for (int c = 0; c < nr; ++c) {
+            X[(size_t)c * nr + c] = 1.0;
+            for (int r = c + 1; r < nr; ++r) {
+              double acc = 0.0;
+              for (int q = c; q < r; ++q) acc += blkL[(size_t)r * nr + q] * X[(size_t)q * nr + c];
+              X[(size_t)r * nr + c] = -acc;
+            }
           }
-        }
-        if (fwd) {
           for (int sc = 0; sc < nr; ++sc)
-            for (int tt = 0; tt < nr; ++tt) out.dense.push_back(blkL[(size_t)tt * nr + sc]);
-        } else {
-          for (int r = 0; r < nr; ++r)
-            for (int tt = 0; tt < nr; ++tt) out.dense.push_back(blkL[(size_t)r * nr + tt]);
+            for (int tt = 0; tt < nr; ++tt)
+              out.dense.push_back(fwd ? X[(size_t)tt * nr + sc] : X[(size_t)sc * nr + tt]);
         }
       }
     }
@@ -600,7 +616,7 @@ int upload(pdx_ctx* c) {
         bstart, bst, bw, sa, sb, sc, bk, vrank, rankv, dinv, W, ist, lst, dst, recE, restE, kapE, recC,
         restC, kapC, brA, brB, brC, brK;
     size_t f[9], b[9];
-    size_t bf[14], bb[14];
+    size_t bf[15], bb[15];
   };
   std::vector<Off> offs(c->inst.size());
   c->bytesIter = 0;
@@ -710,6 +726,7 @@ int upload(pdx_ctx* c) {
       bo[q][11] = fp32 ? ar.vec(castVec<float>(h.dense)) : ar.vec(h.dense);
       bo[q][12] = ar.vec(h.laneMeta);
       bo[q][13] = fp32 ? ar.vec(castVec<float>(h.laneDinv)) : ar.vec(h.laneDinv);
+      bo[q][14] = ar.vec(h.laneDense);
       if (c->idx16) {
         std::vector<uint16_t> c16(h.runCol.begin(), h.runCol.end());
         bo[q][9] = ar.vec(c16);
@@ -855,6 +872,7 @@ int upload(pdx_ctx* c) {
       bd[q]->dense = B + bo[q][11];
       bd[q]->laneMeta = (const int*)(B + bo[q][12]);
       bd[q]->laneDinv = B + bo[q][13];
+      bd[q]->laneDense = (const int*)(B + bo[q][14]);
     }
     d.partOff = (int)(kRedBytes + (c->ldsW ? (size_t)3 * I.fac.m * w : 0));
     d.denseOff = d.partOff + (int)(partBytes(I, w));
@@ -1572,6 +1590,38 @@ int pdx_get_topology(pdx_ctx* c, int i, int* color_count, uint32_t* color_of, ui
   put(t.vedges, vertex_edges);
   put(t.bstart, bend_vertex_start);
   put(t.bstencil, bend_stencils);
+  return PDX_OK;
+}
+
+int pdx_get_schedule_stats(pdx_ctx* c, int i, int64_t* out) {
+  if (!c || !out || i < 0 || i >= (int)c->inst.size()) return PDX_EINVAL;
+  Instance& I = c->inst[(size_t)i];
+  if (!I.factored || !I.packed) {
+    bool changed = false;
+    const int rc = refreshFactors(c, changed);
+    if (rc) return rc;
+  }
+  const HostBlockSweep* sw[2] = {&I.bf, &I.bb};
+  for (int q = 0; q < 2; ++q) {
+    const HostBlockSweep& h = *sw[q];
+    int64_t chains = 0, chainRows = 0, maxTasks = 0;
+    for (size_t t = 0; t < h.taskInfo.size(); ++t)
+      if (h.taskInfo[t] >> 16) {
+        ++chains;
+        chainRows += h.taskInfo[t] & 0xffff;
+      }
+    for (int st = 0; st < h.nstep; ++st)
+      maxTasks = std::max<int64_t>(maxTasks, h.stepTask[(size_t)st + 1] - h.stepTask[(size_t)st]);
+    int64_t* o = out + 8 * q;
+    o[0] = h.nstep;
+    o[1] = (int64_t)h.taskInfo.size();
+    o[2] = chains;
+    o[3] = chainRows;
+    o[4] = (int64_t)h.runCol.size() / kRun;
+    o[5] = (int64_t)h.dense.size();
+    o[6] = maxTasks;
+    o[7] = h.maxRuns;
+  }
   return PDX_OK;
 }
 

@@ -47,10 +47,12 @@ struct BlockSweepDev {
   const void* runCol;     // int32, or uint16 when every rank fits
   const void* runVal;
   const int* laneMeta;    // per task, 64 lanes x {rank or -1, run begin, run end,
-                          //   nrows | chain << 7 | dense offset << 8}
+                          //   segment length | offset in segment << 7 | lanes used << 14
+                          //   | longest segment << 21}
   const void* laneDinv;   // per task, 64 lanes x 1/D of the lane's rank (Real)
-  const void* dense;      // forward: [s * nrows + t] = L(start+t, start+s)
-                          // backward: [r * nrows + t] = L(start+r, start+t)
+  const int* laneDense;   // per task, 64 lanes x offset of the segment's inverse block
+  const void* dense;      // per segment of length L > 1, its inverse X = B^-1:
+                          // forward [s * L + t] = X(t, s), backward [s * L + t] = X(s, t)
 };
 
 struct InstDev {

@@ -186,6 +186,36 @@ __device__ inline float readlaneT(float v, int l) {
 }
 
 constexpr int kCap = 32;
+
+// Finishes the lanes of one task: lane t holds row o of a segment (a chain of
+// L <= kCap rows in lanes t-o .. t-o+L-1, or a single row, L = 1).  With the
+// segment's stored inverse X (forward X, backward X^T; term s of the lane at
+// Xs[s * L + o]):  y_t = sum over s < L, ascending, of X-term * r(segment row s).
+// The broadcasts read the rows' values before the segment is finished, so no
+// term waits for another.
+template <typename Real, typename DP>
+__device__ inline void segmentFinish(DP Xs, int w, int lane, bool act, Real& a0, Real& a1, Real& a2) {
+  const int L = w & 0x7f, o = (w >> 7) & 0x7f, maxL = (w >> 21) & 0x7f;
+  if (maxL <= 1) return;  // single rows only (uniform over the task)
+  const int base = lane - o;
+  Real y0 = 0, y1 = 0, y2 = 0;
+#pragma unroll
+  for (int q = 0; q < kCap; ++q) {
+    if (q < maxL) {
+      const int src = base + min(q, L - 1);
+      const Real r0 = __shfl(a0, src), r1 = __shfl(a1, src), r2 = __shfl(a2, src);
+      if (q < L) {
+        const Real c = (L == 1) ? Real(1) : (act ? Real(Xs[q * L + o]) : Real(0));
+        y0 = y0 + c * r0;
+        y1 = y1 + c * r1;
+        y2 = y2 + c * r2;
+      }
+    }
+  }
+  a0 = y0;
+  a1 = y1;
+  a2 = y2;
+}
 constexpr int kRun = 16;
 
 typedef int v4i __attribute__((ext_vector_type(4)));
@@ -302,6 +332,7 @@ __device__ PDX_INLINE void blockSweep(const BlockSweepDev& sw, WP W, PDX_LDS Rea
   bool haveD = false;
   v4i mpre[2];
   Real dvpre[2] = {0, 0};
+  int dopre[2] = {0, 0};
   bool haveM[2] = {false, false};
   // Every thread issues its loads unconditionally (indices clamped to valid
   // entries) so that no branch or register copy waits for them before use.
@@ -311,6 +342,7 @@ __device__ PDX_INLINE void blockSweep(const BlockSweepDev& sw, WP W, PDX_LDS Rea
   const PDX_GLOBAL Real* gdense = gview<Real>(dense);
   const PDX_GLOBAL v4i* gmeta = gview<v4i>(meta);
   const PDX_GLOBAL Real* gldinv = gview<Real>(ldinv);
+  const PDX_GLOBAL int* gldense = gview<int>(sw.laneDense);
   // Lanes past the step's end load the step's first item instead: all such
   // lanes of a wavefront then hit one address and coalesce into one request,
   // rather than fetching unrelated runs.
@@ -336,6 +368,7 @@ __device__ PDX_INLINE void blockSweep(const BlockSweepDev& sw, WP W, PDX_LDS Rea
       haveM[q] = t < sTask[st + 1];
       const size_t ti = (size_t)(haveM[q] ? t : min(tb, lastTask)) * kWave + lane;
       mpre[q] = gmeta[ti];
+      dopre[q] = gldense[ti];
       if (!kFwd) dvpre[q] = gldinv[ti];
     }
   };
@@ -361,6 +394,7 @@ __device__ PDX_INLINE void blockSweep(const BlockSweepDev& sw, WP W, PDX_LDS Rea
       for (int q = 0; q < 4; ++q) D[d - dbase + q] = gdense[d + q];
     const v4i m0 = mpre[0], m1 = mpre[1];
     const Real dv0 = dvpre[0], dv1 = dvpre[1];
+    const int do0 = dopre[0], do1 = dopre[1];
     if (st + 1 < ns) prefetch(st + 1);
     const long long ts1 = tick();
     barrier();
@@ -369,20 +403,22 @@ __device__ PDX_INLINE void blockSweep(const BlockSweepDev& sw, WP W, PDX_LDS Rea
     for (int t = tb + wave; t < te; t += nw, ++slot) {
       v4i mt;
       Real dv = 0;
+      int doff = 0;
       if (slot == 0) {
         mt = m0;
         dv = dv0;
+        doff = do0;
       } else if (slot == 1) {
         mt = m1;
         dv = dv1;
+        doff = do1;
       } else {
         mt = gmeta[(size_t)t * kWave + lane];
+        doff = gldense[(size_t)t * kWave + lane];
         if (!kFwd) dv = gldinv[(size_t)t * kWave + lane];
       }
       const int k = mt.x;
       const int info = mt.w;
-      const int nrows = info & 0x7f;
-      const bool chain = (info & 0x80) != 0;
       const bool act = k >= 0;
       Real a0 = 0, a1 = 0, a2 = 0;
       if (act) {
@@ -400,37 +436,7 @@ __device__ PDX_INLINE void blockSweep(const BlockSweepDev& sw, WP W, PDX_LDS Rea
           a2 = a2 - P[3 * q + 2];
         }
       }
-      if (chain) {
-        const PDX_LDS Real* Dt = D + (info >> 8);
-        Real dreg[kCap];
-#pragma unroll
-        for (int q = 0; q < kCap; ++q) dreg[q] = (act && q < nrows) ? Dt[q * nrows + lane] : Real(0);
-        if (kFwd) {
-#pragma unroll
-          for (int q = 0; q < kCap; ++q) {
-            if (q < nrows) {
-              const Real y0 = readlaneT(a0, q), y1 = readlaneT(a1, q), y2 = readlaneT(a2, q);
-              if (lane > q && dreg[q] != Real(0)) {
-                a0 = subMul<Real>(a0, dreg[q], y0);
-                a1 = subMul<Real>(a1, dreg[q], y1);
-                a2 = subMul<Real>(a2, dreg[q], y2);
-              }
-            }
-          }
-        } else {
-#pragma unroll
-          for (int q = kCap - 1; q >= 0; --q) {
-            if (q < nrows) {
-              const Real y0 = readlaneT(a0, q), y1 = readlaneT(a1, q), y2 = readlaneT(a2, q);
-              if (lane < q && dreg[q] != Real(0)) {
-                a0 = subMul<Real>(a0, dreg[q], y0);
-                a1 = subMul<Real>(a1, dreg[q], y1);
-                a2 = subMul<Real>(a2, dreg[q], y2);
-              }
-            }
-          }
-        }
-      }
+      segmentFinish<Real>(D + doff, info, lane, act, a0, a1, a2);
       if (act) {
         W[3 * k + 0] = a0;
         W[3 * k + 1] = a1;
@@ -1250,13 +1256,13 @@ __device__ void gridSweep(cg::grid_group& grid, const BlockSweepDev& sw, PDX_GLO
     for (int t = tb + gw; t < te; t += GW) {
       const v4i mt = gmeta[(size_t)t * kWave + lane];
       const Real dv = kFwd ? Real(0) : gldinv[(size_t)t * kWave + lane];
+      const int doff = gview<int>(sw.laneDense)[(size_t)t * kWave + lane];
       const int k = mt.x;
       const int info = mt.w;
-      const int nrows = info & 0x7f;
-      const bool chain = (info & 0x80) != 0;
+      const int used = (info >> 14) & 0x7f;
       const bool act = k >= 0;
       const int rb = __builtin_amdgcn_readlane(mt.y, 0);
-      const int re = __builtin_amdgcn_readlane(mt.z, nrows - 1);
+      const int re = __builtin_amdgcn_readlane(mt.z, used - 1);
       for (int r = rb + lane; r < re; r += kWave) {
         RunRaw<Real, kIdx16> raw;
         loadRun<Real, kIdx16>(sw.runCol, rval, rbase + r, raw);
@@ -1280,37 +1286,7 @@ __device__ void gridSweep(cg::grid_group& grid, const BlockSweepDev& sw, PDX_GLO
           a2 = a2 - Pw[3 * (q - rb) + 2];
         }
       }
-      if (chain) {
-        const PDX_GLOBAL Real* Dt = gdense + dbase + (info >> 8);
-        Real dreg[kCap];
-#pragma unroll
-        for (int q = 0; q < kCap; ++q) dreg[q] = (act && q < nrows) ? Dt[q * nrows + lane] : Real(0);
-        if (kFwd) {
-#pragma unroll
-          for (int q = 0; q < kCap; ++q) {
-            if (q < nrows) {
-              const Real y0 = readlaneT(a0, q), y1 = readlaneT(a1, q), y2 = readlaneT(a2, q);
-              if (lane > q && dreg[q] != Real(0)) {
-                a0 = subMul<Real>(a0, dreg[q], y0);
-                a1 = subMul<Real>(a1, dreg[q], y1);
-                a2 = subMul<Real>(a2, dreg[q], y2);
-              }
-            }
-          }
-        } else {
-#pragma unroll
-          for (int q = kCap - 1; q >= 0; --q) {
-            if (q < nrows) {
-              const Real y0 = readlaneT(a0, q), y1 = readlaneT(a1, q), y2 = readlaneT(a2, q);
-              if (lane < q && dreg[q] != Real(0)) {
-                a0 = subMul<Real>(a0, dreg[q], y0);
-                a1 = subMul<Real>(a1, dreg[q], y1);
-                a2 = subMul<Real>(a2, dreg[q], y2);
-              }
-            }
-          }
-        }
-      }
+      segmentFinish<Real>(gdense + dbase + doff, info, lane, act, a0, a1, a2);
       if (act) {
         W[3 * k + 0] = a0;
         W[3 * k + 1] = a1;

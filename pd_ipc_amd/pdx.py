@@ -100,6 +100,7 @@ def load_library(path=LIB_PATH):
         "pdx_last_kernel_ms": (D, [P]),
         "pdx_autotune": (I, [P, I, C.POINTER(I), C.POINTER(I), C.POINTER(D)]),
         "pdx_set_launch": (I, [P, I, I]),
+        "pdx_get_schedule_stats": (I, [P, I, P]),
         "pdx_set_schedule": (I, [P, I, I]),
         "pdx_bytes_per_iteration": (D, [P]),
         "pdx_bytes_per_substep": (D, [P]),
@@ -395,6 +396,13 @@ class Engine:
         one instance, fast solve only)."""
         m = {"batch": 0, "grid": 1}[mode]
         self._check(self.lib.pdx_set_launch(self.h, m, grid_blocks), "set_launch")
+
+    def schedule_stats(self, i=0):
+        o = np.zeros(16, np.int64)
+        self._check(self.lib.pdx_get_schedule_stats(self.h, i, _ptr(o)), "get_schedule_stats")
+        keys = ["steps", "tasks", "chain_tasks", "chain_rows", "runs", "dense", "max_tasks_per_step",
+                "max_runs_per_step"]
+        return {"fwd": dict(zip(keys, o[:8].tolist())), "bwd": dict(zip(keys, o[8:].tolist()))}
 
     def autotune(self, probe_substeps=1):
         """Picks the fastest launch shape for this context; returns (block, chain cap, ms)."""
