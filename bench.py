@@ -236,6 +236,7 @@ def dropin_workload(args):
             t0 = time.perf_counter()
             e.factor()
             tf = time.perf_counter() - t0
+            tune = e.autotune(2) if (solve == "fast" and not args.no_tune) else None
             perm = e.factor_arrays(0)["perm3"]
             ns = nsub if solve == "fast" else min(nsub, 600 if c.n < 5000 else 2)
             t0 = time.perf_counter()
@@ -253,6 +254,7 @@ def dropin_workload(args):
             res[solve] = {"gpu_ms_per_substep": round(1e3 * dt / ns, 4), "substeps_timed": ns,
                           "max_rel_pos_diff_vs_oracle": rel, "bitwise_equal": bool(np.array_equal(x, xo)),
                           "factorize_count": st["factorize_count"], "factor_and_upload_s": round(tf, 3),
+                          "schedule": None if tune is None else {"threads": tune[0], "chain_cap": tune[1]},
                           "cpu_oracle_ms_per_substep_4threads": round(1e3 * tc / ns, 4)}
             log(f"{name} {solve}: GPU {1e3 * dt / ns:.3f} ms/substep, CPU oracle {1e3 * tc / ns:.3f} ms/substep, "
                 f"rel diff {rel:.3e}")

@@ -193,6 +193,33 @@ constexpr int kCap = 32;
 // Xs[s * L + o]):  y_t = sum over s < L, ascending, of X-term * r(segment row s).
 // The broadcasts read the rows' values before the segment is finished, so no
 // term waits for another.
+// a -= P[q] for the row's runs q in [qb, qe), in ascending q; the partials are
+// read four at a time so their LDS latencies overlap.
+template <typename Real, typename PP>
+__device__ inline void subtractPartials(PP P, int qb, int qe, Real& a0, Real& a1, Real& a2) {
+  int q = qb;
+  for (; q + 4 <= qe; q += 4) {
+    Real p[4][3];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      p[u][0] = P[3 * (q + u) + 0];
+      p[u][1] = P[3 * (q + u) + 1];
+      p[u][2] = P[3 * (q + u) + 2];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      a0 = a0 - p[u][0];
+      a1 = a1 - p[u][1];
+      a2 = a2 - p[u][2];
+    }
+  }
+  for (; q < qe; ++q) {
+    a0 = a0 - P[3 * q + 0];
+    a1 = a1 - P[3 * q + 1];
+    a2 = a2 - P[3 * q + 2];
+  }
+}
+
 template <typename Real, typename DP>
 __device__ inline void segmentFinish(DP Xs, int w, int lane, bool act, Real& a0, Real& a1, Real& a2) {
   const int L = w & 0x7f, o = (w >> 7) & 0x7f, maxL = (w >> 21) & 0x7f;
@@ -430,11 +457,7 @@ __device__ PDX_INLINE void blockSweep(const BlockSweepDev& sw, WP W, PDX_LDS Rea
           a1 = a1 * dv;
           a2 = a2 * dv;
         }
-        for (int q = mt.y; q < mt.z; ++q) {
-          a0 = a0 - P[3 * q + 0];
-          a1 = a1 - P[3 * q + 1];
-          a2 = a2 - P[3 * q + 2];
-        }
+        subtractPartials<Real>(P, mt.y, mt.z, a0, a1, a2);
       }
       segmentFinish<Real>(D + doff, info, lane, act, a0, a1, a2);
       if (act) {
@@ -1280,11 +1303,7 @@ __device__ void gridSweep(cg::grid_group& grid, const BlockSweepDev& sw, PDX_GLO
           a1 = a1 * dv;
           a2 = a2 * dv;
         }
-        for (int q = mt.y; q < mt.z; ++q) {
-          a0 = a0 - Pw[3 * (q - rb) + 0];
-          a1 = a1 - Pw[3 * (q - rb) + 1];
-          a2 = a2 - Pw[3 * (q - rb) + 2];
-        }
+        subtractPartials<Real>(Pw - 3 * rb, mt.y, mt.z, a0, a1, a2);
       }
       segmentFinish<Real>(gdense + dbase + doff, info, lane, act, a0, a1, a2);
       if (act) {
