@@ -62,6 +62,13 @@ typedef struct {
   double rel_tol;          /* relTolerance (<=0 disables) */
   double abs_tol;          /* absTolerance (<=0 disables) */
   double residual_tol;     /* residualTolerance (<=0 disables; then the displacement test decides) */
+  /* Chebyshev blend (Integrator.cpp:474-617; the reference reads PD_CHEB, PD_CHEB_RHO,
+   * PD_CHEB_START from the environment): chebyshev_rho > 0 turns it on; from
+   * iteration chebyshev_start+1 the solve's output is blended with the previous
+   * iterate, x = prev + (x_sol - prev) w, w = 2/(2 - rho^2) then 4/(4 - rho^2 w).
+   * The residual stays that of the unblended solve, as in the reference. */
+  double chebyshev_rho;
+  int chebyshev_start;
 } pdx_config;
 
 /* One cloth: mirrors Mesh (core/mesh/Mesh.h:96-217).  velocities/pin_positions may

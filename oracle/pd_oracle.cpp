@@ -773,6 +773,7 @@ int stepOnce(orc_ctx& c) {
         c.bBase[(size_t)v * 3 + 2] = mOverH2 * p.z;
       }
     }
+    double chebOmega = 1.0;
     for (int k = 0; k < c.cfg.max_iterations; ++k) {
       // DistanceTerm.cpp:396-434 fused projection + colour-ordered scatter
 #pragma omp for
@@ -804,13 +805,29 @@ int stepOnce(orc_ctx& c) {
       double localDiff = 0.0, localScale = 0.0, localRes = 0.0;
       int localResV = -1;
       const double* xSol = c.xSolution.data();
+      // Integrator.cpp:597-617, default (non-interval) schedule
+      const bool chebBlend = c.cfg.cheb != 0 && k > c.cfg.cheb_start;
+      if (chebBlend) {
+        const double rho = c.cfg.cheb_rho;
+        if (k == c.cfg.cheb_start + 1) chebOmega = 2.0 / (2.0 - rho * rho);
+        else chebOmega = 4.0 / (4.0 - rho * rho * chebOmega);
+      }
 #pragma omp for nowait
       for (long long v = 0; v < n; ++v) {
         const size_t i = (size_t)v;
         const V3 xNew{xSol[3 * i + 0], xSol[3 * i + 1], xSol[3 * i + 2]};
-        c.positions[i] = xNew;
-        localDiff = std::max(localDiff, maxAbsComponent(xNew - previous[i]));
-        previous[i] = xNew;
+        if (chebBlend) {
+          const V3 xPrev = previous[i];
+          const V3 xPut{xPrev.x + (xNew.x - xPrev.x) * chebOmega, xPrev.y + (xNew.y - xPrev.y) * chebOmega,
+                        xPrev.z + (xNew.z - xPrev.z) * chebOmega};
+          c.positions[i] = xPut;
+          localDiff = std::max(localDiff, maxAbsComponent(xPut - xPrev));
+          previous[i] = xPut;
+        } else {
+          c.positions[i] = xNew;
+          localDiff = std::max(localDiff, maxAbsComponent(xNew - previous[i]));
+          previous[i] = xNew;
+        }
         localScale = std::max(localScale, maxAbsComponent(c.predicted[i] - c.positionsBeforeStep[i]));
         if (gatherResidual) {
           const double mass = c.masses[i];

@@ -31,6 +31,12 @@ typedef struct {
   double rel_tol;
   double abs_tol;
   double residual_tol;
+  /* Chebyshev blend (Integrator.cpp:474-617, the reference's PD_CHEB / PD_CHEB_RHO /
+   * PD_CHEB_START): 0 = off, else from iteration cheb_start+1 on the solve's
+   * output is blended with the previous iterate, x = prev + (x_sol - prev) w. */
+  int cheb;
+  double cheb_rho;
+  int cheb_start;
 } orc_config;
 
 /* Scene arrays are copied.  pin_positions may be NULL (then = positions). */

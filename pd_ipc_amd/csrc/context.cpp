@@ -442,6 +442,8 @@ void fillConfig(Config& c, const pdx_config* p) {
   c.relTol = p->rel_tol;
   c.absTol = p->abs_tol;
   c.resTol = p->residual_tol;
+  c.chebRho = p->chebyshev_rho;
+  c.chebStart = p->chebyshev_start;
 }
 
 int validateScene(const pdx_scene* s) {
@@ -602,6 +604,8 @@ int upload(pdx_ctx* c) {
   c->ldsW = kRedBytes + maxW + maxExtra <= kLdsLimit;
   c->lds = kRedBytes + (c->ldsW ? maxW : 0) + maxExtra;
   if (c->launchMode == 1) {
+    for (auto& I : c->inst)
+      if (I.cfg.chebRho > 0.0) return fail(c, PDX_EUNSUPPORTED, "the grid form has no Chebyshev blend");
     c->ldsW = false;
     c->lds = kRedBytes + (size_t)(c->block / 64) * c->waveRuns * 3 * w;
     for (auto& I : c->inst)
@@ -802,6 +806,8 @@ int upload(pdx_ctx* c) {
     d.absTol = I.cfg.absTol;
     d.resTol = I.cfg.resTol;
     d.resThresh = I.cfg.resTol * std::max(gl, 1.0e-12);
+    d.chebRho = I.cfg.chebRho;
+    d.chebStart = I.cfg.chebStart;
     d.x = (double*)(B + o.x);
     d.vel = (double*)(B + o.v);
     d.xbs = (double*)(B + o.xbs);
@@ -1361,6 +1367,9 @@ int pdx_set_pin_trajectory(pdx_ctx* c, int i, int nsub, const double* traj) {
 int pdx_rollout(pdx_ctx* c, int nsub, int every) {
   if (!c || nsub <= 0 || every <= 0) return PDX_EINVAL;
   if (c->launchMode == 1) return fail(c, PDX_EUNSUPPORTED, "the differentiable rollout runs in the batch form");
+  for (auto& I : c->inst)
+    if (I.cfg.chebRho > 0.0)
+      return fail(c, PDX_EUNSUPPORTED, "the differentiable rollout differentiates the plain iteration (no Chebyshev blend)");
   int rc = ensureReady(c);
   if (rc) return rc;
   const size_t ni = c->inst.size();

@@ -64,6 +64,8 @@ struct InstDev {
   double gh2[3];
   double gScaleH2;   // length(g) * h * h
   double relTol, absTol, resTol, resThresh;
+  double chebRho;     // > 0: Chebyshev blend from iteration chebStart + 1 on
+  int chebStart, pad3;
   // state (FP64)
   double* x;
   double* vel;

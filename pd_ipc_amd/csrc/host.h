@@ -23,6 +23,8 @@ struct Config {
   double damping = 0.02;
   int maxIter = 10;
   double relTol = 1e-3, absTol = 1e-12, resTol = 1e-3;
+  double chebRho = 0.0;  // Chebyshev blend (0 = off)
+  int chebStart = 10;
 };
 
 // Topology-only data (core/mesh/ConstraintColoring.h:34-81).

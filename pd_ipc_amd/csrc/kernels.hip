@@ -871,6 +871,7 @@ __device__ void substepBody(const InstDev& I, WP W, int nsub, int subBase, int f
       tacc[0] += tick() - tp1;
     }
 
+    double chebOmega = 1.0;
     int used = 0, conv = 0;
     double lastRes = -1.0;
     for (int k = 0; k < I.maxIter; ++k) {
@@ -900,7 +901,45 @@ __device__ void substepBody(const InstDev& I, WP W, int nsub, int subBase, int f
       const bool more = (k + 1 < I.maxIter);
       double* tapeK = tape ? tape + (size_t)(k + 1) * n3 : nullptr;
       double ldiff = 0.0, lres = 0.0;
-      for (int v = tid; v < n; v += bs) {
+      // Chebyshev blend (Integrator.cpp:597-617, default schedule)
+      const bool cheb = !kRoll && I.chebRho > 0.0 && k > I.chebStart;
+      if (cheb) {
+        const double rho = I.chebRho;
+        chebOmega = (k == I.chebStart + 1) ? 2.0 / (2.0 - rho * rho) : 4.0 / (4.0 - rho * rho * chebOmega);
+        // residual at the solve's output; the blended iterate becomes the previous one
+        for (int v = tid; v < n; v += bs) {
+          const double x0 = I.x[3 * v], x1 = I.x[3 * v + 1], x2 = I.x[3 * v + 2];
+          if (wantRes && !I.pinned[v] && I.mass[v] > 0.0) lres = fmax(lres, residualAndRhs(I, v, nullptr));
+          const double p0 = I.prev[3 * v], p1 = I.prev[3 * v + 1], p2 = I.prev[3 * v + 2];
+          const double q0 = p0 + (x0 - p0) * chebOmega, q1 = p1 + (x1 - p1) * chebOmega,
+                       q2 = p2 + (x2 - p2) * chebOmega;
+          ldiff = fmax(ldiff, fmax(fabs(q0 - p0), fmax(fabs(q1 - p1), fabs(q2 - p2))));
+          I.prev[3 * v] = q0;
+          I.prev[3 * v + 1] = q1;
+          I.prev[3 * v + 2] = q2;
+        }
+        __syncthreads();
+        for (int v = tid; v < n; v += bs)
+          for (int d = 0; d < 3; ++d) I.x[3 * v + d] = I.prev[3 * v + d];
+        __syncthreads();
+        // the next right-hand side at the blended positions
+        if (more)
+          for (int v = tid; v < n; v += bs) {
+            const int r = I.vrank[v];
+            if (kExact) {
+              const Acc3 b = rhsExact(I, v);
+              putW<Real>(W, r, b.x, b.y, b.z);
+            } else if (I.pinned[v]) {
+              putW<Real>(W, r, I.pinpos[3 * v] - I.x[3 * v], I.pinpos[3 * v + 1] - I.x[3 * v + 1],
+                         I.pinpos[3 * v + 2] - I.x[3 * v + 2]);
+            } else {
+              Acc3 rh;
+              (void)residualAndRhs(I, v, &rh);
+              putW<Real>(W, r, rh.x, rh.y, rh.z);
+            }
+          }
+      }
+      for (int v = cheb ? n : tid; v < n; v += bs) {
         const double x0 = I.x[3 * v], x1 = I.x[3 * v + 1], x2 = I.x[3 * v + 2];
         if (tapeK) {
           tapeK[3 * v] = x0;

@@ -35,7 +35,8 @@ class PdxError(RuntimeError):
 class _Config(C.Structure):
     _fields_ = [("dt", C.c_double), ("gravity", C.c_double * 3), ("damping", C.c_double),
                 ("max_iterations", C.c_int), ("rel_tol", C.c_double), ("abs_tol", C.c_double),
-                ("residual_tol", C.c_double)]
+                ("residual_tol", C.c_double), ("chebyshev_rho", C.c_double),
+                ("chebyshev_start", C.c_int)]
 
 
 class _Scene(C.Structure):
@@ -129,6 +130,8 @@ class SimConfig:
     rel_tol: float = 1e-3
     abs_tol: float = 1e-12
     residual_tol: float = 1e-3
+    chebyshev_rho: float = 0.0   # > 0 turns the Chebyshev blend on (reference PD_CHEB_RHO, 0.98)
+    chebyshev_start: int = 10    # first blended iteration is start + 1 (reference PD_CHEB_START)
 
     @staticmethod
     def fixed_iterations(iters, dt, damping=0.02, gravity=(0.0, -9.81, 0.0)):
@@ -147,6 +150,8 @@ class SimConfig:
         c.rel_tol = self.rel_tol
         c.abs_tol = self.abs_tol
         c.residual_tol = self.residual_tol
+        c.chebyshev_rho = self.chebyshev_rho
+        c.chebyshev_start = self.chebyshev_start
         return c
 
 

@@ -18,7 +18,8 @@ REF_LIB = os.path.join(ORACLE_DIR, "_ref", "libref_mesh.so")
 class _Cfg(C.Structure):
     _fields_ = [("dt", C.c_double), ("gravity", C.c_double * 3), ("damping", C.c_double),
                 ("max_iterations", C.c_int), ("rel_tol", C.c_double), ("abs_tol", C.c_double),
-                ("residual_tol", C.c_double)]
+                ("residual_tol", C.c_double), ("cheb", C.c_int), ("cheb_rho", C.c_double),
+                ("cheb_start", C.c_int)]
 
 
 _lib = None
@@ -67,6 +68,8 @@ def _cfg(cfg):
     c.damping = cfg.damping
     c.max_iterations = cfg.max_iterations
     c.rel_tol, c.abs_tol, c.residual_tol = cfg.rel_tol, cfg.abs_tol, cfg.residual_tol
+    rho = getattr(cfg, "chebyshev_rho", 0.0)
+    c.cheb, c.cheb_rho, c.cheb_start = int(rho > 0.0), rho, getattr(cfg, "chebyshev_start", 10)
     return c
 
 

@@ -195,3 +195,46 @@ def test_grid_form_matches_oracle(precision, nx, ny, shear, bend, iters, nsub, t
     xr = o.solve(b)
     assert np.max(np.abs(xs - xr)) <= (1e-12 if precision == "fp64" else 1e-4) * np.max(np.abs(xr))
     e.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision,solve", [("fp64", "exact"), ("fp64", "fast"), ("mixed32", "fast")])
+def test_chebyshev_blend_matches_oracle(precision, solve):
+    """Chebyshev blend (rho 0.98, from iteration 4): exact FP64 bitwise against
+    the oracle (positions, velocities, residual), fast FP64 within 1e-10, mixed32
+    within 1e-5 m."""
+    print(f"[case] chebyshev {precision} {solve}", flush=True)
+    c = scenes.grid_cloth(40, 40, 2300.0, shear=500.0, bend=50.0)
+    cfg = SimConfig(dt=1.0 / 600.0, max_iterations=10, rel_tol=0.0, abs_tol=0.0, residual_tol=1e-30,
+                    chebyshev_rho=0.98, chebyshev_start=3)
+    e = Engine(precision, solve)
+    e.add(c, cfg)
+    perm = e.factor_arrays(0)["perm3"]
+    e.step(20)
+    x, v = e.state(0)
+    o = Oracle(c, cfg, perm, threads=4)
+    o.step(20)
+    xo, vo = o.positions(), o.velocities()
+    if precision == "fp64" and solve == "exact":
+        assert np.array_equal(x, xo) and np.array_equal(v, vo)
+        assert e.stats(0)["last_residual"] == o.last_residual()
+    elif precision == "fp64":
+        assert np.max(np.abs(x - xo)) <= 1e-10 * np.max(np.abs(xo))
+    else:
+        assert np.max(np.abs(x - xo)) <= 1e-5
+    e.close()
+
+
+@pytest.mark.gpu
+def test_bench60_published_chebyshev_on_gpu():
+    """The published 60x60 Chebyshev run (20 iterations, 300 substeps) on the
+    GPU fast FP64 path: final residual 9.37677 (docs/chebyshev.md:285)."""
+    print("[case] bench60 chebyshev gpu", flush=True)
+    c = scenes.grid_cloth(60, 60, 1.0e4, sx=0.025, sz=0.025, mass="edges")
+    cfg = SimConfig(dt=1.0 / 120.0, max_iterations=20, residual_tol=0.3, chebyshev_rho=0.98,
+                    chebyshev_start=10)
+    e = Engine("fp64", "fast")
+    e.add(c, cfg)
+    e.step(300)
+    assert f"{e.stats(0)['last_residual']:.6g}" == "9.37677"
+    e.close()

@@ -226,3 +226,24 @@ def test_thread_count_independence():
         outs.append(o.positions())
     for x in outs[1:]:
         assert np.array_equal(x.view(np.uint64), outs[0].view(np.uint64))
+
+
+def _bench60(iters, rho):
+    c = scenes.grid_cloth(60, 60, 1.0e4, sx=0.025, sz=0.025, mass="edges")
+    cfg = SimConfig(dt=1.0 / 120.0, max_iterations=iters, residual_tol=0.3, chebyshev_rho=rho,
+                    chebyshev_start=10)
+    _, f = _perm(c, cfg)
+    return c, cfg, f["perm3"]
+
+
+@pytest.mark.parametrize("iters,plain,cheb", [(10, "27.3033", "27.3033"), (20, "12.3328", "9.37677"),
+                                              (40, "8.59041", "7.40925"), (80, "6.14024", "3.51819")])
+def test_bench60_published_residuals_plain_and_chebyshev(iters, plain, cheb):
+    """pd_bench --grid 60 60 --steps 300 (docs/chebyshev.md:284-287): final
+    residual of the plain iteration and of the Chebyshev blend (PD_CHEB=1,
+    rho 0.98, start 10) at 10/20/40/80 iterations, 4 threads."""
+    for rho, want in ((0.0, plain), (0.98, cheb)):
+        c, cfg, perm = _bench60(iters, rho)
+        o = Oracle(c, cfg, perm, threads=4)
+        o.step(300)
+        assert f"{o.last_residual():.6g}" == want, (iters, rho)
