@@ -244,6 +244,76 @@ int pdx_grid_fill(int nx, int ny, double sx, double sz, double kappa, double she
                   double bend_k, double* positions, int32_t* ea, int32_t* eb, double* rest,
                   double* kap, int32_t* sa, int32_t* sb, int32_t* sc, double* bk);
 
+/* --- unstructured input (Mesh::loadObj, Mesh.cpp:56-125) -------------------- */
+/* Reads a Wavefront OBJ triangle mesh the way Mesh::loadObj does: "v" lines are
+ * vertices, "f" lines give 1-based indices (text before the first '/'; the first
+ * three corners of a polygon), unique undirected edges in face order with rest
+ * length |x_b - x_a|.  PDX_EINVAL when the file cannot be opened, a face index
+ * does not parse, there are no vertices or no triangles, or an index is outside
+ * [1, n] (the reference reads such an index out of bounds).
+ * pdx_obj_counts returns the sizes; pdx_obj_fill fills 3n positions, 3T
+ * triangle corners (0-based), E edges with kappa on every edge. */
+int pdx_obj_counts(const char* path, int* n, int* n_edges, int* n_tris);
+int pdx_obj_fill(const char* path, double kappa, double* positions, int32_t* tris,
+                 int32_t* ea, int32_t* eb, double* rest, double* kap);
+
+/* Lumped masses.  Surfaces (Mesh::computeMassesFromSurfaces, Mesh.cpp:127-153):
+ * density * triangle area / 3 to each corner.  Edges (computeMassesFromEdges,
+ * :155-181): density * rest^2 / 2 to each end.  Both then give every mass not
+ * above 1e-12 the smallest positive mass (1 if there is none). */
+int pdx_masses_from_surfaces(int n, const double* rest_positions, int n_tris,
+                             const int32_t* tris, double density, double* masses);
+int pdx_masses_from_edges(int n, int n_edges, const int32_t* ea, const int32_t* eb,
+                          const double* rest, double density, double* masses);
+
+/* --- physics diagnostics ----------------------------------------------------- */
+/* Computed on the host in FP64 from the instance's current state, in the
+ * reference's summation order (edge order, then stencil order, then vertex order). */
+typedef struct {
+  double max_strain;       /* Mesh::maxRelativeStrain (Mesh.cpp:351-360) */
+  double mean_strain;      /* Mesh::meanRelativeStrain (Mesh.cpp:362-372) */
+  double elastic_energy;   /* surrogateEnergy (Integrator.cpp:804-826): edges + bend stencils */
+  double total_energy;     /* totalEnergy (Integrator.cpp:828-840): kinetic + elastic + gravity */
+  double max_speed;        /* maxSpeed (Integrator.cpp:842-846) */
+  double total_mass;       /* Mesh::totalMass (Mesh.cpp:345-349) */
+} pdx_diag;
+int pdx_diagnostics(pdx_ctx* ctx, int instance, pdx_diag* out);
+/* Structural nonzeros of the 3n x 3n matrix L (IGlobalSolver::Stats::nnz); needs
+ * pdx_factor (or a step) first. */
+int pdx_matrix_nnz(const pdx_ctx* ctx, int instance, long long* nnz);
+
+/* --- headless report (pd_bench, app/bench/main.cpp:53-264) ------------------ */
+/* Runs one cloth the way pd_bench does and writes its report.  Options as pd_bench:
+ *   --grid NX NY, --mesh PATH (OBJ; falls back to the grid when it does not load),
+ *   --spacing S, --steps N, --substeps N, --dt H, --stiffness K (default 1e4),
+ *   --density RHO, --damping KD, --iters N, --preset preview|realtime|accurate,
+ *   --residual-tol T, --tol T, --threads N (accepted, no effect), --pin-top,
+ *   --pin-all, --report N, --help
+ * plus --shear K and --bend K (the reference reads PD_SHEAR / PD_BEND from the
+ * environment; both are honoured too), --precision fp64|mixed32,
+ * --solve exact|fast and --device D.  A preset sets the iteration cap and the
+ * residual tolerance; explicit --iters / --residual-tol override it whatever the
+ * order.  Unknown options are ignored, as in pd_bench.
+ * The report text (NUL-terminated, truncated to text_cap - 1 bytes) goes to
+ * `text` when it is not NULL; the figures go to `summary` when it is not NULL.
+ * Returns PDX_OK, PDX_EINVAL for an unknown preset, or the failing call's status. */
+typedef struct {
+  int n_vertices, n_edges, n_bends, n_pinned;
+  int substeps, converged_substeps, max_iterations_used;
+  long long total_iterations;
+  double last_residual;
+  double max_strain, mean_strain, run_max_strain;
+  double elastic_energy, total_energy, max_speed, total_mass;
+  double wall_seconds;             /* whole stepping loop, diagnostics included */
+  double device_ms_per_substep;    /* substep kernel, HIP events */
+  int analyze_count, factorize_count, color_count;
+  long long matrix_nnz, factor_nnz;   /* 3n x 3n structure; 3 x one component's strictly-lower factor */
+  double analyze_seconds, factorize_seconds;
+  int loaded_mesh;                 /* 1 when --mesh loaded, 0 for the grid */
+} pdx_bench_summary;
+int pdx_bench_run(int argc, const char* const* argv, char* text, long long text_cap,
+                  pdx_bench_summary* summary);
+
 #ifdef __cplusplus
 }
 #endif

@@ -81,6 +81,30 @@ void ref_coloring(void* h, uint32_t* color_of, uint32_t* order, uint32_t* start,
   std::memcpy(vedges, c.vertexEdges.data(), c.vertexEdges.size() * 4);
 }
 
+// An OBJ cloth as makeScene builds it for a mesh path (Scene.cpp:44-68):
+// Mesh::loadObj with the stiffness, then masses from the surfaces with the
+// density.  No pins.  Returns NULL when loadObj refuses the file.
+void* ref_load_obj(const char* path, double stiffness, double density) {
+  Mesh* m = new Mesh();
+  if (!Mesh::loadObj(path, *m, stiffness)) {
+    delete m;
+    return nullptr;
+  }
+  for (auto& e : m->edges) e.stiffness = stiffness;
+  m->computeMassesFromSurfaces(density > 0.0 ? density : 1.0);
+  return m;
+}
+
+int64_t ref_triangle_count(void* h) {
+  return static_cast<int64_t>(static_cast<Mesh*>(h)->triangleIndices.size());
+}
+
+void ref_triangles(void* h, int32_t* out) {
+  Mesh* m = static_cast<Mesh*>(h);
+  for (size_t t = 0; t < m->triangleIndices.size(); ++t)
+    for (int k = 0; k < 3; ++k) out[3 * t + k] = m->triangleIndices[t][static_cast<size_t>(k)];
+}
+
 void ref_bend_adjacency(void* h, uint32_t* vstart, uint32_t* stencils, double* weight) {
   Mesh* m = static_cast<Mesh*>(h);
   const auto& b = m->bendAdjacency();

@@ -1578,6 +1578,31 @@ int pdx_rollout_timing(const pdx_ctx* c, double* forward_ms, double* adjoint_ms)
 
 extern "C" {
 
+int pdx_diagnostics(pdx_ctx* c, int i, pdx_diag* out) {
+  if (!c || !out || i < 0 || i >= (int)c->inst.size()) return PDX_EINVAL;
+  Instance& I = c->inst[(size_t)i];
+  std::vector<double> x(I.scene.x.size()), v(I.scene.v.size());
+  const int rc = pdx_get_state(c, i, x.data(), v.data());
+  if (rc) return rc;
+  double d[6];
+  physicsDiagnostics(I.scene, x.data(), v.data(), I.cfg.g, d);
+  out->max_strain = d[0];
+  out->mean_strain = d[1];
+  out->elastic_energy = d[2];
+  out->total_energy = d[3];
+  out->max_speed = d[4];
+  out->total_mass = d[5];
+  return PDX_OK;
+}
+
+int pdx_matrix_nnz(const pdx_ctx* c, int i, long long* nnz) {
+  if (!c || !nnz || i < 0 || i >= (int)c->inst.size()) return PDX_EINVAL;
+  const Instance& I = c->inst[(size_t)i];
+  if (!I.factored) return PDX_ENOTREADY;
+  *nnz = 3 * (long long)I.A.rows.size();  // every block is a scalar times I3
+  return PDX_OK;
+}
+
 int pdx_get_topology(pdx_ctx* c, int i, int* color_count, uint32_t* color_of, uint32_t* order,
                      uint32_t* start, uint32_t* vertex_start, uint32_t* vertex_edges,
                      uint32_t* bend_vertex_start, uint32_t* bend_stencils) {

@@ -84,4 +84,10 @@ int refactorNumeric(const Matrix& A, Factor& f);
 uint64_t numericStamp(const Scene& s, double dt);
 uint64_t structureStamp(const Scene& s);
 
+// Strain, energy and speed figures of a state (x, v: 3n) of the scene's cloth,
+// FP64 in the reference's summation order (core/mesh/Mesh.cpp:345-372,
+// core/sim/Integrator.cpp:804-846).  Fills pdx_diag's fields in order:
+// max strain, mean strain, elastic energy, total energy, max speed, total mass.
+void physicsDiagnostics(const Scene& s, const double* x, const double* v, const double g[3], double out[6]);
+
 }  // namespace pdx

@@ -58,6 +58,24 @@ class _Stats(C.Structure):
                 ("factorize_seconds", C.c_double), ("phase_seconds", C.c_double * 9)]
 
 
+class _Diag(C.Structure):
+    _fields_ = [("max_strain", C.c_double), ("mean_strain", C.c_double), ("elastic_energy", C.c_double),
+                ("total_energy", C.c_double), ("max_speed", C.c_double), ("total_mass", C.c_double)]
+
+
+class _BenchSummary(C.Structure):
+    _fields_ = [("n_vertices", C.c_int), ("n_edges", C.c_int), ("n_bends", C.c_int), ("n_pinned", C.c_int),
+                ("substeps", C.c_int), ("converged_substeps", C.c_int), ("max_iterations_used", C.c_int),
+                ("total_iterations", C.c_longlong), ("last_residual", C.c_double),
+                ("max_strain", C.c_double), ("mean_strain", C.c_double), ("run_max_strain", C.c_double),
+                ("elastic_energy", C.c_double), ("total_energy", C.c_double), ("max_speed", C.c_double),
+                ("total_mass", C.c_double), ("wall_seconds", C.c_double),
+                ("device_ms_per_substep", C.c_double), ("analyze_count", C.c_int),
+                ("factorize_count", C.c_int), ("color_count", C.c_int), ("matrix_nnz", C.c_longlong),
+                ("factor_nnz", C.c_longlong), ("analyze_seconds", C.c_double),
+                ("factorize_seconds", C.c_double), ("loaded_mesh", C.c_int)]
+
+
 _lib = None
 
 
@@ -107,6 +125,13 @@ def load_library(path=LIB_PATH):
         "pdx_bytes_per_substep": (D, [P]),
         "pdx_grid_counts": (I, [I, I, I, I, C.POINTER(I), C.POINTER(I), C.POINTER(I)]),
         "pdx_grid_fill": (I, [I, I, D, D, D, D, D, P, P, P, P, P, P, P, P, P]),
+        "pdx_obj_counts": (I, [C.c_char_p, C.POINTER(I), C.POINTER(I), C.POINTER(I)]),
+        "pdx_obj_fill": (I, [C.c_char_p, D, P, P, P, P, P, P]),
+        "pdx_masses_from_surfaces": (I, [I, P, I, P, D, P]),
+        "pdx_masses_from_edges": (I, [I, I, P, P, P, D, P]),
+        "pdx_diagnostics": (I, [P, I, C.POINTER(_Diag)]),
+        "pdx_matrix_nnz": (I, [P, I, C.POINTER(C.c_longlong)]),
+        "pdx_bench_run": (I, [I, C.POINTER(C.c_char_p), C.c_char_p, C.c_longlong, C.POINTER(_BenchSummary)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -306,6 +331,21 @@ class Engine:
         out = {k: getattr(s, k) for k, _ in _Stats._fields_}
         out["phase_seconds"] = list(s.phase_seconds)
         return out
+
+    def diagnostics(self, i=0):
+        """Strain, energy and speed of the instance's current state, in the reference's
+        summation order: Mesh::maxRelativeStrain / meanRelativeStrain / totalMass
+        (core/mesh/Mesh.cpp:345-372), surrogateEnergy / totalEnergy / maxSpeed
+        (core/sim/Integrator.cpp:804-846)."""
+        d = _Diag()
+        self._check(self.lib.pdx_diagnostics(self.h, i, C.byref(d)), "diagnostics")
+        return {k: getattr(d, k) for k, _ in _Diag._fields_}
+
+    def matrix_nnz(self, i=0):
+        """Structural nonzeros of the 3n x 3n matrix L (IGlobalSolver::Stats::nnz)."""
+        v = C.c_longlong()
+        self._check(self.lib.pdx_matrix_nnz(self.h, i, C.byref(v)), "matrix_nnz")
+        return v.value
 
     def solve(self, i, b):
         b = np.ascontiguousarray(b, np.float64)

@@ -1,20 +1,23 @@
-"""Benchmark and test scenes: regular-grid cloths of the five workloads.
+"""Benchmark and test scenes: regular-grid cloths of the five workloads, and
+triangle-mesh cloths read from OBJ files.
 
 Grid geometry, edge order, shear diagonals and bend stencils come from the
 library's own builder (``pdx_grid_fill``, restating core/mesh/Mesh.cpp:12-54,
 :183-207, :219-285).  The cloth is 1 m x 1 m in the xz-plane at y = 0 with
 spacing 1/(nx-1) along x and 1/(ny-1) along z, total mass 1 kg lumped uniformly
-(m_v = 1/n), gravity -9.81 y, damping 0.02 (core/sim/Scene.h:37).
+(m_v = 1/n), gravity -9.81 y, damping 0.02 (core/sim/Scene.h:37).  OBJ meshes
+go through ``pdx_obj_fill`` and ``pdx_masses_from_surfaces`` (Mesh.cpp:56-153).
 
 Random draws use a seeded splitmix64 stream (uniforms = top 53 bits / 2^53,
 normals by Box-Muller), so every instance is reproducible from (seed, index).
 """
 import ctypes as C
 import math
+import os
 
 import numpy as np
 
-from .pdx import Cloth, SimConfig, load_library
+from .pdx import Cloth, PdxError, SimConfig, load_library
 
 H600 = 1.0 / 600.0
 _MASK = (1 << 64) - 1
@@ -89,12 +92,10 @@ def grid_cloth(nx, ny, kappa, shear=0.0, bend=0.0, sx=None, sz=None, pins="top",
     else:
         # edge-lumped masses over the structural edges, before shear is added
         # (core/sim/Scene.cpp:62-68 then :80-92)
-        masses = np.zeros(n)
+        masses = np.empty(n)
         ns = 2 * nx * ny - nx - ny
-        for e in range(ns):
-            share = 1.0 * rest[e] * rest[e] * 0.5
-            masses[ea[e]] += share
-            masses[eb[e]] += share
+        rc = lib.pdx_masses_from_edges(n, ns, p(ea), p(eb), p(rest), 1.0, p(masses))
+        assert rc == 0
     pinned = np.zeros(n, np.uint8)
     if isinstance(pins, str):
         top = (ny - 1) * nx
@@ -105,6 +106,36 @@ def grid_cloth(nx, ny, kappa, shear=0.0, bend=0.0, sx=None, sz=None, pins="top",
     return Cloth(positions=pos, masses=masses, pinned=pinned, edge_a=ea, edge_b=eb, edge_rest=rest,
                  edge_kappa=kap, bend_a=sa, bend_b=sb, bend_c=sc, bend_k=bk,
                  meta=dict(nx=nx, ny=ny, kappa=kappa, shear=shear, bend=bend))
+
+
+def obj_cloth(path, kappa, density=1.0, pins=()):
+    """Triangle-mesh cloth from a Wavefront OBJ file, as makeScene builds it for a
+    mesh path (core/sim/Scene.cpp:44-68): Mesh::loadObj (unique triangle edges,
+    kappa on every edge), masses from the triangle areas with the given density,
+    no shear diagonals or bend stencils, and no pins unless given.  The triangle
+    corners (0-based) are kept in meta['triangles']."""
+    lib = load_library()
+    path_b = os.fsencode(path)
+    n, E, T = C.c_int(), C.c_int(), C.c_int()
+    rc = lib.pdx_obj_counts(path_b, C.byref(n), C.byref(E), C.byref(T))
+    if rc != 0:
+        raise PdxError(rc, f"{path}: not a loadable OBJ triangle mesh")
+    n, E, T = n.value, E.value, T.value
+    pos, tris = np.empty(3 * n), np.empty(3 * T, np.int32)
+    ea, eb, rest, kap = np.empty(E, np.int32), np.empty(E, np.int32), np.empty(E), np.empty(E)
+    p = lambda a: a.ctypes.data_as(C.c_void_p)
+    rc = lib.pdx_obj_fill(path_b, kappa, p(pos), p(tris), p(ea), p(eb), p(rest), p(kap))
+    if rc != 0:
+        raise PdxError(rc, f"{path}: not a loadable OBJ triangle mesh")
+    masses = np.empty(n)
+    rc = lib.pdx_masses_from_surfaces(n, p(pos), T, p(tris), density, p(masses))
+    if rc != 0:
+        raise PdxError(rc, "masses_from_surfaces")
+    pinned = np.zeros(n, np.uint8)
+    pinned[list(pins)] = 1
+    return Cloth(positions=pos, masses=masses, pinned=pinned, edge_a=ea, edge_b=eb, edge_rest=rest,
+                 edge_kappa=kap, meta=dict(kappa=kappa, density=density, triangles=tris.reshape(-1, 3),
+                                           path=str(path)))
 
 
 def bench_scene_40():

@@ -40,6 +40,86 @@ def name_of(nx, ny, shear, bend):
 
 def extract(L, nx, ny, spacing, kappa, shear, bend):
     h = L.ref_grid_scene(nx, ny, spacing, kappa, 1.0, shear, bend)
+    d = read_mesh(L, h)
+    L.ref_free(h)
+    d.update(nx=nx, ny=ny, spacing=spacing, kappa=kappa, shear=shear, bend=bend)
+    return d
+
+
+# OBJ inputs for Mesh::loadObj (core/mesh/Mesh.cpp:56-125), each with every face
+# index inside [1, n] so that the reference reads nothing out of bounds.
+def _tri_grid(nx, ny):
+    lines = ["# triangulated grid, alternating diagonals, wavy heights"]
+    for j in range(ny):
+        for i in range(nx):
+            y = 0.01 * ((i * 7 + j * 3) % 5) - 0.02
+            lines.append(f"v {i * 0.05:.6f} {y:.6f} {j * 0.04:.6f}")
+    for j in range(ny - 1):
+        for i in range(nx - 1):
+            a, b = j * nx + i + 1, j * nx + i + 2
+            c, d = a + nx, b + nx
+            if (i + j) % 2 == 0:
+                lines += [f"f {a} {b} {d}", f"f {a} {d} {c}"]
+            else:
+                lines += [f"f {a} {b} {c}", f"f {b} {d} {c}"]
+    return "\n".join(lines) + "\n"
+
+
+OBJ_CASES = {
+    "tri_grid": (_tri_grid(9, 6), 2300.0, 1.0),
+    # slash forms, vt/vn lines, a quad (first three corners), a face with two
+    # corners (skipped), a repeated corner, a short vertex line (z = 0), signs and
+    # trailing text on indices, CRLF endings, an unreferenced vertex (mass filled)
+    "forms": ("# forms\r\n"
+              "v 0 0 0\r\n"
+              "v 1.0 0.0 0.0\r\n"
+              "  v 1 0.5 1\r\n"
+              "v 0 0.25 1.5\r\n"
+              "v 2.0 1e-1\r\n"
+              "v 5 5 5\r\n"
+              "vt 0.5 0.5\r\n"
+              "vn 0 1 0\r\n"
+              "\r\n"
+              "f 1/1/1 2/1/1 3/1/1\r\n"
+              "f 1//1 3//1 4//1\r\n"
+              "f 2 5 3 4\r\n"
+              "f 4 1\r\n"
+              "f 3 3 5\r\n"
+              "f +2 5abc 1/x\r\n"
+              "o name\r\n"
+              "g group\r\n", 1000.0, 0.7),
+    # every triangle has zero area: every mass is filled with 1
+    "degenerate": ("v 0 0 0\nv 1 0 0\nv 2 0 0\nv 3 0 0\nf 1 2 3\nf 2 3 4\n", 500.0, 1.0),
+    # refused by the reference: an index without digits, no faces, no vertices
+    "bad_index": ("v 0 0 0\nv 1 0 0\nv 0 1 0\nf 1 x 3\n", 100.0, 1.0),
+    "no_faces": ("v 0 0 0\nv 1 0 0\nv 0 1 0\n", 100.0, 1.0),
+    "no_vertices": ("# nothing\n", 100.0, 1.0),
+}
+
+
+def extract_obj(L, name, text, kappa, density):
+    import tempfile
+    with tempfile.NamedTemporaryFile("wb", suffix=".obj", delete=False) as f:
+        f.write(text.encode())
+        path = f.name
+    try:
+        h = L.ref_load_obj(path.encode(), kappa, density)
+    finally:
+        os.unlink(path)
+    out = dict(name=name, obj_text=np.frombuffer(text.encode(), np.uint8), kappa=kappa, density=density,
+               loads=h is not None)
+    if h is None:
+        return out
+    T = int(L.ref_triangle_count(h))
+    tris = np.zeros(3 * T, np.int32)
+    L.ref_triangles(h, tris.ctypes.data_as(C.c_void_p))
+    out.update(read_mesh(L, h))
+    out["triangles"] = tris
+    L.ref_free(h)
+    return out
+
+
+def read_mesh(L, h):
     cnt = np.zeros(5, np.int64)
     p = lambda a: a.ctypes.data_as(C.c_void_p)
     L.ref_counts(h, p(cnt))
@@ -58,9 +138,7 @@ def extract(L, nx, ny, spacing, kappa, shear, bend):
     bvstart, bsten, bw = np.zeros(n + 1, np.uint32), np.zeros(3 * B, np.uint32), np.zeros(3 * B)
     if B:
         L.ref_bend_adjacency(h, p(bvstart), p(bsten), p(bw))
-    L.ref_free(h)
-    return dict(nx=nx, ny=ny, spacing=spacing, kappa=kappa, shear=shear, bend=bend,
-                positions=pos, masses=mass, pinned=pinned, edge_a=ea, edge_b=eb, rest=rest,
+    return dict(positions=pos, masses=mass, pinned=pinned, edge_a=ea, edge_b=eb, rest=rest,
                 edge_kappa=kap, bend_a=sa, bend_b=sb, bend_c=sc, bend_k=sk,
                 color_of=color_of, order=order, start=start, vertex_start=vstart,
                 vertex_edges=vedges, bend_vertex_start=bvstart, bend_stencils=bsten,
@@ -78,6 +156,11 @@ def main():
             path = path.replace(".npz", "_k1e4.npz")
         np.savez_compressed(path, **d)
         print(path, "colours", d["color_count"], "edges", len(d["edge_a"]), "bends", len(d["bend_a"]))
+    for name, (text, kappa, density) in OBJ_CASES.items():
+        d = extract_obj(L, name, text, kappa, density)
+        path = os.path.join(HERE, f"ref_obj_{name}.npz")
+        np.savez_compressed(path, **d)
+        print(path, "loads", d["loads"], "edges", len(d.get("edge_a", [])))
 
 
 if __name__ == "__main__":

@@ -225,4 +225,11 @@ def ref_mesh_lib():
         fn = getattr(L, name)
         fn.restype = None
         fn.argtypes = args
+    if hasattr(L, "ref_load_obj"):
+        L.ref_load_obj.restype = P
+        L.ref_load_obj.argtypes = [C.c_char_p, C.c_double, C.c_double]
+        L.ref_triangle_count.restype = C.c_int64
+        L.ref_triangle_count.argtypes = [P]
+        L.ref_triangles.restype = None
+        L.ref_triangles.argtypes = [P, P]
     return L

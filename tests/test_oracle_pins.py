@@ -39,6 +39,69 @@ def test_bench_scene_published_physics():
     assert s["compact_ready"] == 1 and s["compact_nnz"] == 22009
 
 
+# pd_bench 40x40 / kappa 1e4 / 300 substeps variants as published, each figure at
+# the digits it was printed with: (iterations, residual tol, bend k) ->
+# residual, final max strain, mean strain, elastic energy
+PUBLISHED_VARIANTS = [
+    # presets preview / accurate (docs/perf.md:1422, :1424)
+    ((20, 0.3, 0.0), ("13.3816", "0.0581053", None, "0.320141"), 6),
+    ((80, 1e-3, 0.0), ("6.40876", "0.0586015", None, "0.322604"), 6),
+    # iteration sweep (docs/solver-feasibility.md:141-144): residual printed as 24.10
+    ((10, 1e-3, 0.0), ("24.10", "0.057035", "0.00134129", "0.318593"), 6),
+    ((20, 1e-3, 0.0), ("13.38", "0.0581053", "0.00134602", "0.320141"), 6),
+    ((80, 1e-3, 0.0), ("6.41", "0.0586015", "0.00135108", "0.322604"), 6),
+    # bend stiffness sweep, PD_BEND=k (docs/perf.md:856-862)
+    ((40, 1e-3, 0.001), ("8.62434", "0.0585737", "0.0013502", "0.322056"), 6),
+    ((40, 1e-3, 1.0), ("50.33", "0.05872", "0.00135012", "0.326154"), 6),
+    ((40, 1e-3, 100.0), ("2153.4", "0.06213", "0.00143839", "0.37633"), 6),
+    ((40, 1e-3, 1000.0), ("5235.6", "0.04993", "0.00139981", "0.42291"), 6),
+]
+
+
+def _printed(value, text):
+    """value printed with as many significant digits (or decimals) as text has."""
+    if "." in text and text.endswith("0"):
+        return f"{value:.{len(text.split('.')[1])}f}" == text
+    digits = len(text.replace("-", "").replace(".", "").lstrip("0"))
+    return f"{value:.{digits}g}" == text
+
+
+@pytest.mark.parametrize("setup,expect,threads", PUBLISHED_VARIANTS,
+                         ids=[f"it{s[0]}_tol{s[1]}_bend{s[2]}" for s, _, _ in PUBLISHED_VARIANTS])
+def test_bench_scene_published_variants(setup, expect, threads):
+    iters, rtol, bend = setup
+    c = scenes.grid_cloth(40, 40, 1.0e4, bend=bend, sx=0.025, sz=0.025, mass="edges")
+    cfg = SimConfig(dt=1.0 / 120.0, max_iterations=iters, residual_tol=rtol)
+    _, f = _perm(c, cfg)
+    o = Oracle(c, cfg, f["perm3"], threads=threads)
+    o.step(300)
+    res, smax, smean, el = expect
+    assert _printed(o.last_residual(), res), o.last_residual()
+    assert _printed(o.max_strain(), smax), o.max_strain()
+    if smean is not None:
+        assert _printed(o.mean_strain(), smean), o.mean_strain()
+    assert _printed(o.surrogate_energy(), el), o.surrogate_energy()
+    assert o.stats()["early_exit_count"] == 0  # 0/300 converged in every published run
+
+
+def test_bench_200_published_realtime():
+    """pd_bench --grid 200 200 --steps 100 --preset realtime (docs/perf.md:1431)."""
+    c = scenes.grid_cloth(200, 200, 1.0e4, sx=0.025, sz=0.025, mass="edges")
+    cfg = SimConfig(dt=1.0 / 120.0, max_iterations=40, residual_tol=0.3)
+    _, f = _perm(c, cfg)
+    o = Oracle(c, cfg, f["perm3"], threads=8)
+    o.step(100)
+    assert f"{o.last_residual():.6g}" == "58.3972"
+    assert f"{o.max_strain():.6g}" == "0.529022"
+    assert f"{o.surrogate_energy():.5g}" == "38.989"
+
+
+def test_printed_digit_helper():
+    assert _printed(24.0984, "24.10") and not _printed(24.0884, "24.10")
+    assert _printed(5235.55, "5235.6") and _printed(0.0499347, "0.04993")
+    assert not _printed(0.0499347, "0.04994")
+
+
 @pytest.mark.parametrize("n,bend,nnz3,levels", [(40, 0.0, 66027, 155), (100, 0.0, 637233, 521),
                                                 (200, 0.0, 3255033, 1125), (40, 1000.0, 182613, None)])
 def test_factor_nnz_and_levels_match_published(n, bend, nnz3, levels):
