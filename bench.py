@@ -56,24 +56,30 @@ def time_batch(e, steps, warmup):
 
 
 def dropin_fp64(cloths, cfg, nsub, solve, max_seconds):
-    """Plain FP64 drop-in: one context per cloth, one instance per call."""
-    tot_sub, tot_t, done = 0, 0.0, 0
+    """Plain FP64 drop-in: one context per cloth, one instance per call, cloths in
+    a seeded random order until `max_seconds` of wall time (set-up included) are
+    spent.  Returns (cloths, substeps, stepping seconds, set-up seconds)."""
+    tot_sub, tot_t, done, setup = 0, 0.0, 0, 0.0
     order = np.random.default_rng(0).permutation(len(cloths))
+    w0 = time.perf_counter()
     for i in order:
+        t0 = time.perf_counter()
         e = Engine("fp64", solve)
         e.add(cloths[i], cfg)
         e.factor()
+        if solve == "dense":
+            e.autotune(1)
         e.step(1)
+        setup += time.perf_counter() - t0
         t0 = time.perf_counter()
         e.step(nsub)
         tot_t += time.perf_counter() - t0
         tot_sub += nsub
         done += 1
         e.close()
-        if tot_t > max_seconds and done >= 8:
+        if time.perf_counter() - w0 > max_seconds and done >= 8:
             break
-    # scale the sample's per-instance-substep cost to the whole mix by vertex count
-    return done, tot_sub, tot_t
+    return done, tot_sub, tot_t, setup
 
 
 def cpu_baseline(cloths, cfg, perms, budget_s, threads):
@@ -230,15 +236,15 @@ def dropin_workload(args):
         if args.dropin_substeps:
             nsub = min(nsub, args.dropin_substeps)
         res = {"substeps": nsub, "pd_iterations": cfg.max_iterations}
-        for solve in ("fast", "exact"):
+        for solve in ("dense", "fast", "exact"):
             e = Engine("fp64", solve)
             e.add(c, cfg)
             t0 = time.perf_counter()
             e.factor()
             tf = time.perf_counter() - t0
-            tune = e.autotune(2) if (solve == "fast" and not args.no_tune) else None
+            tune = e.autotune(2) if (solve != "exact" and not args.no_tune) else None
             perm = e.factor_arrays(0)["perm3"]
-            ns = nsub if solve == "fast" else min(nsub, 600 if c.n < 5000 else 2)
+            ns = nsub if solve != "exact" else min(nsub, 600 if c.n < 5000 else 2)
             t0 = time.perf_counter()
             e.step(ns)
             dt = time.perf_counter() - t0
@@ -251,10 +257,15 @@ def dropin_workload(args):
             tc = time.perf_counter() - t1
             xo = o.positions()
             rel = float(np.max(np.abs(x - xo)) / np.max(np.abs(xo)))
+            sched = None
+            if tune is not None:
+                sched = ({"grid_workgroups": tune[0]} if solve == "dense"
+                         else {"threads": tune[0], "chain_cap": tune[1]})
             res[solve] = {"gpu_ms_per_substep": round(1e3 * dt / ns, 4), "substeps_timed": ns,
                           "max_rel_pos_diff_vs_oracle": rel, "bitwise_equal": bool(np.array_equal(x, xo)),
                           "factorize_count": st["factorize_count"], "factor_and_upload_s": round(tf, 3),
-                          "schedule": None if tune is None else {"threads": tune[0], "chain_cap": tune[1]},
+                          "schedule": sched,
+                          "algorithmic_GB_s": round(e.bytes_per_substep() / (dt / ns) / 1e9, 1),
                           "cpu_oracle_ms_per_substep_4threads": round(1e3 * tc / ns, 4)}
             log(f"{name} {solve}: GPU {1e3 * dt / ns:.3f} ms/substep, CPU oracle {1e3 * tc / ns:.3f} ms/substep, "
                 f"rel diff {rel:.3e}")
@@ -417,15 +428,18 @@ def main():
     }
     del fp64
     if not args.no_dropin:
-        done, nsub, t = dropin_fp64(cloths, cfg, 4, args.solve, args.dropin_seconds)
-        # per-instance cost grows with size: weight the sample by vertices
-        sample_v = None
-        rate = nsub / t
-        out["fp64_dropin"] = {"value": round(rate, 3), "unit": "cloth-substeps/s",
-                              "sample": f"{done} random instances x 4 substeps, one context each",
-                              "speedup_batched_fp32": round(value / rate, 1)}
-        log(f"fp64 drop-in sample: {done} instances, {rate:.2f} cloth-substeps/s")
-        del sample_v
+        # the plain FP64 drop-in, one context per cloth: the dense solve form (the
+        # fastest single-cloth form up to 32,768 vertices) and the chain-blocked sweeps
+        for solve, key, budget in (("dense", "fp64_dropin", args.dropin_seconds),
+                                   ("fast", "fp64_dropin_sweeps", args.dropin_seconds / 2)):
+            done, nsub, t, setup = dropin_fp64(cloths, cfg, 4, solve, budget)
+            rate = nsub / t
+            out[key] = {"value": round(rate, 3), "unit": "cloth-substeps/s", "solve": solve,
+                        "sample": f"{done} random instances x 4 substeps, one context each",
+                        "setup_s_per_instance": round(setup / done, 3),
+                        "speedup_batched_fp32": round(value / rate, 1)}
+            log(f"fp64 drop-in ({solve}) sample: {done} instances, {rate:.2f} cloth-substeps/s, "
+                f"set-up {setup / done:.3f} s per instance")
     if not args.no_cpu:
         perms = []
         sample = cloths[:64]

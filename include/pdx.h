@@ -42,7 +42,13 @@ enum {
 
 enum {
   PDX_SOLVE_EXACT = 0,  /* level-scheduled gather sweeps in the reference's sum order */
-  PDX_SOLVE_FAST = 1    /* chain-blocked sweeps (forward order kept, backward reordered) */
+  PDX_SOLVE_FAST = 1,   /* chain-blocked sweeps (forward order kept, backward reordered) */
+  PDX_SOLVE_DENSE = 2   /* one instance: the inverse of the component matrix is formed once
+                           from the factor (host, m x m, resident in HBM) and each PD
+                           iteration's solve is one dense product on the whole GPU
+                           (cooperative launch); m <= 32768.  Same arithmetic as the fast
+                           mode otherwise (FP64 state and residual; mixed32 solves the
+                           update in FP32).  No differentiable rollout, no Chebyshev blend. */
 };
 
 enum {
@@ -212,14 +218,16 @@ int pdx_rollout_timing(const pdx_ctx* ctx, double* forward_ms, double* adjoint_m
 int pdx_get_schedule_stats(pdx_ctx* ctx, int instance, int64_t* out16);
 
 /* Launch form (PDX_LAUNCH_*); grid_blocks = workgroups of the grid form, 0 = all
- * that can be co-resident.  Re-packs the schedules for the form. */
+ * that can be co-resident.  Re-packs the schedules for the form.  In the dense
+ * solve mode (always on the grid) only grid_blocks applies. */
 int pdx_set_launch(pdx_ctx* ctx, int mode, int grid_blocks);
 
 /* Autotuner: times `probe_substeps` substeps of the whole context under each
  * launch shape (threads per workgroup 256 / 512 x chain cap 8 / 16 / 24 / 32 in
- * fast mode) and keeps the fastest.  State and statistics are restored, so the
- * probes leave no trace.  No counterpart in the reference (its CPU path has no
- * launch schedule). */
+ * fast mode; in the dense solve mode, 16, 32, 64, ... workgroups, reported in
+ * best_block with best_cap 0) and keeps the fastest.  State and statistics are
+ * restored, so the probes leave no trace.  No counterpart in the reference (its
+ * CPU path has no launch schedule). */
 int pdx_autotune(pdx_ctx* ctx, int probe_substeps, int* best_block, int* best_cap, double* best_ms);
 
 /* --- timing ---------------------------------------------------------------- */

@@ -31,6 +31,11 @@ hipError_t launchGrid(const InstDev* d, int inst, int nsub, double* gslots, int 
 hipError_t launchGridSolve(const InstDev* d, int inst, const double* b, double* x, int waveRuns, int blocks,
                            int block, size_t lds, bool fp32, bool idx16, hipStream_t s);
 int gridBlocksMax(int block, size_t lds, bool fp32, bool idx16);
+hipError_t launchDense(const InstDev* d, int inst, int nsub, double* gslots, int blocks, int block, size_t lds,
+                       bool fp32, hipStream_t s);
+int denseBlocksMax(int block, size_t lds, bool fp32);
+hipError_t launchDenseSolve(const InstDev* d, int inst, const double* b, double* x, int n, int m, bool fp32,
+                            hipStream_t s);
 }  // namespace pdx
 
 using namespace pdx;
@@ -39,6 +44,10 @@ namespace {
 
 constexpr size_t kLdsLimit = 160 * 1024;
 constexpr size_t kRedBytes = 32 * sizeof(double);
+// dense solve mode: largest component size (the FP64 inverse is 8 m^2 bytes, 8.6 GB here)
+constexpr int kDenseMaxM = 32768;
+// row stride of the device inverse: m rounded up to a multiple of 4
+int denseLd(int m) { return (m + 3) & ~3; }
 // launch flags of the differentiable rollout (the substep kernel's kRoll*)
 constexpr int kRollTraj = 1, kRollTape = 2, kRollFixed = 4;
 
@@ -343,6 +352,7 @@ struct Instance {
   double analyzeSec = 0, factorSec = 0;
   HostSweep hf, hb;
   HostBlockSweep bf, bb;
+  std::vector<double> inv;  // dense solve mode: inverse of the component matrix, rank space
   bool packed = false;
   // device addresses of the writable state (inside the arena)
   double *dx = nullptr, *dv = nullptr, *dpin = nullptr, *dbrhs = nullptr;
@@ -404,6 +414,12 @@ struct pdx_ctx {
   int gridLaunchBlocks = 0; // blocks of the last grid upload
   int waveRuns = 0;         // runs per wavefront LDS slice (grid form)
   double* dGridSlots = nullptr;
+  // dense solve mode: the device copy of the inverse (m x m Real), refreshed when
+  // the factor is
+  char* dLinv = nullptr;
+  size_t linvBytes = 0;
+  bool linvStale = true;
+  bool denseLds = false;  // W staged in each workgroup's LDS for the product
   // differentiable rollout: pin trajectories (host), device buffers, seeds
   struct Roll {
     int nsub = 0, every = 0, nseg = 0, epoch = -1;
@@ -506,6 +522,11 @@ int refreshFactors(pdx_ctx* c, bool& changed) {
     if (!I.factored || ns != I.numStamp || !I.packed) todo.push_back(i);
   }
   if (todo.empty()) return PDX_OK;
+  if (c->mode == PDX_SOLVE_DENSE) {
+    if (ni != 1) return fail(c, PDX_EUNSUPPORTED, "the dense solve mode takes exactly one instance");
+    if (c->inst[0].scene.n > kDenseMaxM)
+      return fail(c, PDX_EUNSUPPORTED, "the dense solve mode takes at most " + std::to_string(kDenseMaxM) + " vertices");
+  }
   changed = true;
   std::vector<int> rcs(todo.size(), 0);
   std::atomic<size_t> next{0};
@@ -516,7 +537,11 @@ int refreshFactors(pdx_ctx* c, bool& changed) {
       Instance& I = c->inst[(size_t)todo[t]];
       const double t0 = now();
       auto pack = [&]() {
-        if (c->mode == PDX_SOLVE_FAST) {
+        if (c->mode == PDX_SOLVE_DENSE) {
+          const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+          denseInverse(I.fac, I.inv, (int)std::min(16u, hw));
+          c->linvStale = true;
+        } else if (c->mode == PDX_SOLVE_FAST) {
           const int w = c->prec == PDX_PREC_MIXED32 ? 4 : 8;
           if (c->launchMode == 1) {
             // grid form: a task's runs fit one wavefront's LDS slice; steps are
@@ -611,6 +636,16 @@ int upload(pdx_ctx* c) {
     for (auto& I : c->inst)
       if (std::max(I.bf.maxTaskRuns, I.bb.maxTaskRuns) > c->waveRuns)
         return fail(c, PDX_EUNSUPPORTED, "a factor row streams more runs than one wavefront's LDS slice");
+  }
+  const bool dense = c->mode == PDX_SOLVE_DENSE;
+  if (dense) {
+    for (auto& I : c->inst)
+      if (I.cfg.chebRho > 0.0) return fail(c, PDX_EUNSUPPORTED, "the dense solve mode has no Chebyshev blend");
+    // W in HBM; each workgroup copies it into LDS for the product when it fits
+    c->ldsW = false;
+    const size_t wb = (size_t)3 * c->inst[0].fac.m * w;
+    c->denseLds = kRedBytes + wb <= kLdsLimit;
+    c->lds = kRedBytes + (c->denseLds ? wb : 0);
   }
   if (c->lds > kLdsLimit)
     return fail(c, PDX_EUNSUPPORTED, "sweep staging does not fit in LDS (" + std::to_string(c->lds) + " B)");
@@ -753,7 +788,9 @@ int upload(pdx_ctx* c) {
     const double nb = (double)s.n, Eb = (double)s.E, Bb = (double)s.B, Fb = (double)I.fac.nnz,
                  mb = (double)I.fac.m;
     const double iw = (c->mode == PDX_SOLVE_FAST && c->idx16) ? 2.0 : 4.0;
-    double bi = 2.0 * Fb * ((double)w + iw) + 2.0 * (mb + 1) * 4 + mb * ((double)w + 4);
+    // dense solve mode: the inverse (m^2 w) and W read once, the rank -> vertex map
+    double bi = dense ? mb * mb * (double)w + 3.0 * mb * (double)w + mb * 4
+                      : 2.0 * Fb * ((double)w + iw) + 2.0 * (mb + 1) * 4 + mb * ((double)w + 4);
     bi += 2.0 * (Eb * (2 * 4 + 2 * 8) + (nb + 1) * 4 + 2 * Eb * 4);
     bi += 24.0 * nb * 8 + nb * 8 + nb;
     if (Bb > 0) bi += Bb * (3 * 4 + 8) + (nb + 1) * 4 + 3 * Bb * (4 + 8) + 3 * nb * 8;
@@ -761,7 +798,8 @@ int upload(pdx_ctx* c) {
     I.bytesPerIter = bi;
     c->bytesIter += bi;
   }
-  const size_t slotOff = ar.add(nullptr, 2 * 1024 * sizeof(double));
+  // grid reductions: 2 alternating sets of 2 slots per workgroup, at most 1024 workgroups
+  const size_t slotOff = ar.add(nullptr, 4 * 1024 * sizeof(double));
   const size_t instOff = ar.add(nullptr, sizeof(InstDev) * c->inst.size());
   // LPT order: heaviest instance (factor entries) first
   std::vector<int> order(c->inst.size());
@@ -780,6 +818,38 @@ int upload(pdx_ctx* c) {
   HIPCHECK(c, hipMalloc(&c->dbuf, ar.host.size()));
   c->dbytes = ar.host.size();
   char* B = c->dbuf;
+  if (dense) {
+    const Instance& I = c->inst[0];
+    const int m = I.fac.m, ld = denseLd(m);
+    const size_t bytes = (size_t)m * ld * w;
+    if (c->dLinv && c->linvBytes != bytes) {
+      HIPCHECK(c, hipFree(c->dLinv));
+      c->dLinv = nullptr;
+      c->linvStale = true;
+    }
+    if (!c->dLinv) {
+      if (hipMalloc(&c->dLinv, bytes) != hipSuccess) {
+        c->dLinv = nullptr;
+        return fail(c, PDX_ENOMEM, "dense solve mode: no room for the " + std::to_string(bytes) + " B inverse");
+      }
+      c->linvBytes = bytes;
+      c->linvStale = true;
+    }
+    if (c->linvStale) {
+      // rows padded to ld with zeros (16-byte aligned rows for the product's loads)
+      auto put = [&](auto zero) -> int {
+        using T = decltype(zero);
+        std::vector<T> rows((size_t)m * ld, zero);
+        for (int r = 0; r < m; ++r)
+          for (int j = 0; j < m; ++j) rows[(size_t)r * ld + j] = (T)I.inv[(size_t)r * m + j];
+        HIPCHECK(c, hipMemcpy(c->dLinv, rows.data(), bytes, hipMemcpyHostToDevice));
+        return PDX_OK;
+      };
+      const int rc = fp32 ? put(0.0f) : put(0.0);
+      if (rc) return rc;
+      c->linvStale = false;
+    }
+  }
   c->hInst.assign(c->inst.size(), InstDev{});
   for (size_t i = 0; i < c->inst.size(); ++i) {
     Instance& I = c->inst[i];
@@ -884,6 +954,9 @@ int upload(pdx_ctx* c) {
     d.denseOff = d.partOff + (int)(partBytes(I, w));
     d.stepOff = d.denseOff + (int)(denseBytes(I, w));
     d.Wg = c->ldsW ? nullptr : (void*)(B + o.W);
+    d.Linv = dense ? (const void*)c->dLinv : nullptr;
+    d.denseLd = dense ? denseLd(I.fac.m) : 0;
+    d.denseLdsW = (dense && c->denseLds) ? 1 : 0;
     d.istats = (int*)(B + o.ist);
     d.lstats = (long long*)(B + o.lst);
     d.dstats = (double*)(B + o.dst);
@@ -967,11 +1040,34 @@ int launchGridForm(pdx_ctx* c, int nsub) {
   return PDX_OK;
 }
 
+// Dense solve mode: cooperative launch, every co-resident workgroup unless the
+// caller (pdx_set_launch) or the autotuner chose fewer.
+int denseLaunchBlocks(pdx_ctx* c) {
+  const int maxb = denseBlocksMax(c->block, c->lds, c->prec == PDX_PREC_MIXED32);
+  const int want = c->gridBlocks > 0 ? c->gridBlocks : maxb;
+  return std::min(std::min(want, maxb), 1024);
+}
+
+int launchDenseForm(pdx_ctx* c, int nsub) {
+  const int g = denseLaunchBlocks(c);
+  if (g <= 0) return fail(c, PDX_EHIP, "dense solve mode: no co-resident workgroup fits");
+  HIPCHECK(c, launchDense(c->dInst, 0, nsub, c->dGridSlots, g, c->block, c->lds, c->prec == PDX_PREC_MIXED32,
+                          c->stream));
+  return PDX_OK;
+}
+
+// Launch forms that are replayed directly rather than captured in a graph.
+bool cooperative(const pdx_ctx* c) { return c->launchMode == 1 || c->mode == PDX_SOLVE_DENSE; }
+
+int launchCooperative(pdx_ctx* c, int nsub) {
+  return c->mode == PDX_SOLVE_DENSE ? launchDenseForm(c, nsub) : launchGridForm(c, nsub);
+}
+
 int launch(pdx_ctx* c, int nsub) {
   HIPCHECK(c, hipSetDevice(c->device));
-  if (c->launchMode == 1) {
+  if (cooperative(c)) {
     HIPCHECK(c, hipEventRecord(c->ev0, c->stream));
-    const int rc = launchGridForm(c, nsub);
+    const int rc = launchCooperative(c, nsub);
     if (rc) return rc;
     HIPCHECK(c, hipEventRecord(c->ev1, c->stream));
     c->lastLaunches = 1;
@@ -993,7 +1089,8 @@ int pdx_create(pdx_ctx** out, int device, int precision, int solve_mode) {
   if (!out) return PDX_EINVAL;
   *out = nullptr;
   if (precision != PDX_PREC_FP64 && precision != PDX_PREC_MIXED32) return PDX_EINVAL;
-  if (solve_mode != PDX_SOLVE_EXACT && solve_mode != PDX_SOLVE_FAST) return PDX_EINVAL;
+  if (solve_mode != PDX_SOLVE_EXACT && solve_mode != PDX_SOLVE_FAST && solve_mode != PDX_SOLVE_DENSE)
+    return PDX_EINVAL;
   pdx_ctx* c = new pdx_ctx();
   c->device = device;
   c->prec = precision;
@@ -1009,6 +1106,7 @@ int pdx_destroy(pdx_ctx* c) {
     hipStreamSynchronize(c->stream);
   }
   if (c->dbuf) hipFree(c->dbuf);
+  if (c->dLinv) hipFree(c->dLinv);
   if (c->roll.dbuf) hipFree(c->roll.dbuf);
   if (c->dScratchB) hipFree(c->dScratchB);
   if (c->dScratchX) hipFree(c->dScratchX);
@@ -1094,11 +1192,11 @@ int pdx_step_graph(pdx_ctx* c, int nsub, int replays) {
   if (!c || nsub <= 0 || replays < 0) return PDX_EINVAL;
   int rc = ensureReady(c);
   if (rc) return rc;
-  if (c->launchMode == 1) {
+  if (cooperative(c)) {
     // cooperative launches are replayed directly (not captured)
     HIPCHECK(c, hipEventRecord(c->ev0, c->stream));
     for (int r = 0; r < replays; ++r)
-      if ((rc = launchGridForm(c, nsub))) return rc;
+      if ((rc = launchCooperative(c, nsub))) return rc;
     HIPCHECK(c, hipEventRecord(c->ev1, c->stream));
     c->lastLaunches = std::max(1, replays);
     return pdx_sync(c);
@@ -1247,7 +1345,10 @@ int pdx_solve3(pdx_ctx* c, int i, const double* b, double* x) {
     c->scratchN = n3;
   }
   HIPCHECK(c, hipMemcpyAsync(c->dScratchB, b, (size_t)n3 * 8, hipMemcpyHostToDevice, c->stream));
-  if (c->launchMode == 1) {
+  if (c->mode == PDX_SOLVE_DENSE) {
+    HIPCHECK(c, launchDenseSolve(c->dInst, i, c->dScratchB, c->dScratchX, I.scene.n, I.fac.m,
+                                 c->prec == PDX_PREC_MIXED32, c->stream));
+  } else if (c->launchMode == 1) {
     const int g = gridLaunchBlocks(c);
     HIPCHECK(c, launchGridSolve(c->dInst, i, c->dScratchB, c->dScratchX, c->waveRuns, g, c->block, c->lds,
                                 c->prec == PDX_PREC_MIXED32, c->idx16, c->stream));
@@ -1366,7 +1467,8 @@ int pdx_set_pin_trajectory(pdx_ctx* c, int i, int nsub, const double* traj) {
 
 int pdx_rollout(pdx_ctx* c, int nsub, int every) {
   if (!c || nsub <= 0 || every <= 0) return PDX_EINVAL;
-  if (c->launchMode == 1) return fail(c, PDX_EUNSUPPORTED, "the differentiable rollout runs in the batch form");
+  if (c->launchMode == 1 || c->mode == PDX_SOLVE_DENSE)
+    return fail(c, PDX_EUNSUPPORTED, "the differentiable rollout runs in the batch form (exact or fast solve)");
   for (auto& I : c->inst)
     if (I.cfg.chebRho > 0.0)
       return fail(c, PDX_EUNSUPPORTED, "the differentiable rollout differentiates the plain iteration (no Chebyshev blend)");
@@ -1663,6 +1765,11 @@ int pdx_set_schedule(pdx_ctx* c, int block_threads, int chain_cap) {
   if (!c || block_threads < 64 || block_threads > 512 || block_threads % 64 != 0 || chain_cap < 1 ||
       chain_cap > 32)
     return PDX_EINVAL;
+  if (c->mode == PDX_SOLVE_DENSE) {
+    // no sweep schedule to repack: only the workgroup size applies
+    c->block = block_threads;
+    return PDX_OK;
+  }
   if (c->chainCap != chain_cap) {
     const int rc = pullState(c);
     if (rc) return rc;
@@ -1676,6 +1783,11 @@ int pdx_set_schedule(pdx_ctx* c, int block_threads, int chain_cap) {
 
 int pdx_set_launch(pdx_ctx* c, int mode, int grid_blocks) {
   if (!c || (mode != PDX_LAUNCH_BATCH && mode != PDX_LAUNCH_GRID) || grid_blocks < 0) return PDX_EINVAL;
+  if (c->mode == PDX_SOLVE_DENSE) {
+    // the dense solve mode always runs on the grid: only the workgroup count applies
+    c->gridBlocks = grid_blocks;
+    return PDX_OK;
+  }
   if (mode == PDX_LAUNCH_GRID && (c->inst.size() != 1 || c->mode != PDX_SOLVE_FAST))
     return fail(c, PDX_EUNSUPPORTED, "the grid form takes exactly one instance in fast solve mode");
   if (mode != c->launchMode) {
@@ -1722,6 +1834,32 @@ int pdx_autotune(pdx_ctx* c, int probe_substeps, int* best_block, int* best_cap,
     }
     return PDX_OK;
   };
+  if (c->mode == PDX_SOLVE_DENSE) {
+    // workgroup count of the cooperative launch: fewer workgroups make cheaper
+    // grid barriers, more stream the inverse faster
+    const int maxb = denseBlocksMax(c->block, c->lds, c->prec == PDX_PREC_MIXED32);
+    const int keep = c->gridBlocks;
+    double bestMs = 1e300;
+    int best = keep;
+    for (int g = 16; g <= 1024; g *= 2) {
+      const int gb = std::min(g, maxb);
+      c->gridBlocks = gb;
+      if ((rc = restore())) return rc;
+      if ((rc = launch(c, probe_substeps))) return rc;
+      if ((rc = pdx_sync(c))) return rc;
+      if (c->lastKernelMs < bestMs) {
+        bestMs = c->lastKernelMs;
+        best = gb;
+      }
+      if (gb == maxb) break;
+    }
+    c->gridBlocks = best;
+    if ((rc = restore())) return rc;
+    if (best_block) *best_block = best;
+    if (best_cap) *best_cap = 0;
+    if (best_ms) *best_ms = bestMs;
+    return PDX_OK;
+  }
   const int blocks[2] = {256, 512};
   const int caps[4] = {8, 16, 24, 32};
   const bool fast = c->mode == PDX_SOLVE_FAST;

@@ -112,6 +112,12 @@ struct InstDev {
   SweepDev fwd, bwd;
   BlockSweepDev bfwd, bbwd;
   void* Wg;           // global W scratch (3m Real) when W is not LDS-resident
+  // dense solve mode: the inverse of the component matrix in rank space (m rows
+  // of denseLd Real, denseLd = m rounded up to a multiple of 4 with zero padding;
+  // row r = column r: the inverse is symmetric), and whether each workgroup
+  // stages W in LDS
+  const void* Linv;
+  int denseLdsW, denseLd;
   // differentiable rollout (null / 0 outside it)
   int npin, npinEdge, npinSten, rollK;
   const int* pinList;      // pinned vertices, ascending

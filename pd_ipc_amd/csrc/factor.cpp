@@ -4,8 +4,10 @@
 // (the factor-once invariant of the reference); nothing here is on the per-substep
 // path.
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <numeric>
+#include <thread>
 
 #include "host.h"
 
@@ -496,6 +498,40 @@ int refactorNumeric(const Matrix& A, Factor& f) {
   f.rx.swap(g.rx);
   f.dinv.swap(g.dinv);
   return 0;
+}
+
+void denseInverse(const Factor& f, std::vector<double>& out, int threads) {
+  const int m = f.m;
+  out.assign((size_t)m * m, 0.0);
+  std::atomic<int> next{0};
+  auto worker = [&]() {
+    std::vector<double> y((size_t)m);
+    for (;;) {
+      const int r = next.fetch_add(1);
+      if (r >= m) return;
+      std::fill(y.begin(), y.end(), 0.0);
+      y[(size_t)r] = 1.0;
+      // L y = e_r, column by column from r (the entries above r stay zero)
+      for (int k = r; k < m; ++k) {
+        const double yk = y[(size_t)k];
+        if (yk == 0.0) continue;
+        for (int p = f.cp[(size_t)k]; p < f.cp[(size_t)k + 1]; ++p) y[(size_t)f.ci[(size_t)p]] -= f.cx[(size_t)p] * yk;
+      }
+      for (int k = r; k < m; ++k) y[(size_t)k] *= f.dinv[(size_t)k];
+      // L^T x = y, rows from the last
+      for (int k = m - 1; k >= 0; --k) {
+        double s = y[(size_t)k];
+        for (int p = f.cp[(size_t)k]; p < f.cp[(size_t)k + 1]; ++p) s -= f.cx[(size_t)p] * y[(size_t)f.ci[(size_t)p]];
+        y[(size_t)k] = s;
+      }
+      std::copy(y.begin(), y.end(), out.begin() + (size_t)r * m);
+    }
+  };
+  const int nth = std::max(1, std::min(threads, m));
+  std::vector<std::thread> pool;
+  for (int t = 1; t < nth; ++t) pool.emplace_back(worker);
+  worker();
+  for (auto& th : pool) th.join();
 }
 
 }  // namespace pdx

@@ -81,6 +81,11 @@ struct Factor {
 int analyzeAndFactor(const Scene& s, const Matrix& A, Factor& f);
 int refactorNumeric(const Matrix& A, Factor& f);
 
+// Dense inverse of the component matrix in rank space, m x m, row r = the solve
+// of e_r through the factor (forward, 1/D, backward), which is column r of the
+// symmetric inverse.  Rows are spread over `threads` host threads.
+void denseInverse(const Factor& f, std::vector<double>& out, int threads);
+
 uint64_t numericStamp(const Scene& s, double dt);
 uint64_t structureStamp(const Scene& s);
 

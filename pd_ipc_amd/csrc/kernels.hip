@@ -1290,15 +1290,39 @@ __global__ void __launch_bounds__(512) adjointKernel(const InstDev* __restrict__
 
 namespace cg = cooperative_groups;
 
-// Max of v over the grid.  Slot set `buf` alternates between consecutive calls.
+// Max of v over the grid.  Slot set `buf` (2 x gridDim.x slots at buf * 2 *
+// gridDim.x) alternates between consecutive calls, so a workgroup that runs ahead
+// into the next reduction never writes a slot another one is still reading.
 __device__ inline double gridMax(cg::grid_group& grid, double v, double* red, double* gslots, int buf) {
+  double* s = gslots + (size_t)buf * 2 * gridDim.x;
   const double b = blockMax(v, red);
-  if (threadIdx.x == 0) gslots[buf * gridDim.x + blockIdx.x] = b;
+  if (threadIdx.x == 0) s[blockIdx.x] = b;
   grid.sync();
   double m = 0.0;
-  for (int q = threadIdx.x; q < (int)gridDim.x; q += blockDim.x) m = fmax(m, gslots[buf * gridDim.x + q]);
+  for (int q = threadIdx.x; q < (int)gridDim.x; q += blockDim.x) m = fmax(m, s[q]);
   __syncthreads();
   return blockMax(m, red + 16);
+}
+
+// Maxima of a and of b over the grid with one grid barrier (same slot sets).
+__device__ inline void gridMax2(cg::grid_group& grid, double a, double b, double* red, double* gslots, int buf,
+                                double& ra, double& rb) {
+  double* s = gslots + (size_t)buf * 2 * gridDim.x;
+  const double ba = blockMax(a, red);
+  const double bb = blockMax(b, red + 8);
+  if (threadIdx.x == 0) {
+    s[blockIdx.x] = ba;
+    s[gridDim.x + blockIdx.x] = bb;
+  }
+  grid.sync();
+  double ma = 0.0, mb = 0.0;
+  for (int q = threadIdx.x; q < (int)gridDim.x; q += blockDim.x) {
+    ma = fmax(ma, s[q]);
+    mb = fmax(mb, s[gridDim.x + q]);
+  }
+  __syncthreads();
+  ra = blockMax(ma, red + 16);
+  rb = blockMax(mb, red + 24);
 }
 
 template <typename Real, bool kFwd, bool kIdx16, typename Store>
@@ -1359,7 +1383,100 @@ __device__ void gridSweep(cg::grid_group& grid, const BlockSweepDev& sw, PDX_GLO
   }
 }
 
-template <typename Real, bool kExact, bool kIdx16>
+__device__ inline double fmaT(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ inline float fmaT(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
+constexpr int kDR = 8;  // rows per wavefront pass of the dense product
+
+template <typename Real>
+struct Vec16;  // 16 bytes of Real
+template <>
+struct Vec16<double> {
+  typedef double T __attribute__((ext_vector_type(2)));
+  static constexpr int n = 2;
+};
+template <>
+struct Vec16<float> {
+  typedef float T __attribute__((ext_vector_type(4)));
+  static constexpr int n = 4;
+};
+
+// Dense solve: out_r = sum_j Linv[r][j] W_j for the three right-hand sides, for
+// every rank r, rows spread over the grid's wavefronts (global wavefront gw of GW),
+// kDR rows per pass so each W value loaded serves kDR rows.  Each lane reads 16
+// bytes of a row at a time (rows are ld Real apart, ld a multiple of 4, padding
+// zero).  A row's sum is formed by one wavefront in a fixed order (lane-strided
+// partial sums, then a butterfly), so it does not depend on the grid size.
+// store(r, a0, a1, a2) takes each row.
+template <typename Real, typename WP, typename Store>
+__device__ PDX_INLINE void denseRows(const PDX_GLOBAL Real* L, int m, int ld, WP W, int gw, int GW, Store store) {
+  typedef typename Vec16<Real>::T VT;
+  constexpr int V = Vec16<Real>::n;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int nv = ld / V;  // vectors per row
+  for (int r0 = gw * kDR; r0 < m; r0 += GW * kDR) {
+    const PDX_GLOBAL VT* row[kDR];
+#pragma unroll
+    for (int q = 0; q < kDR; ++q)  // rows past m repeat row m-1 (their sums are dropped)
+      row[q] = (const PDX_GLOBAL VT*)(L + (size_t)min(r0 + q, m - 1) * ld);
+    Real acc[kDR][3];
+#pragma unroll
+    for (int q = 0; q < kDR; ++q) acc[q][0] = acc[q][1] = acc[q][2] = Real(0);
+    for (int jv = lane; jv < nv; jv += kWave) {
+      VT lv[kDR];
+#pragma unroll
+      for (int q = 0; q < kDR; ++q) lv[q] = row[q][jv];
+#pragma unroll
+      for (int u = 0; u < V; ++u) {
+        const int j = jv * V + u;
+        const bool in = j < m;  // padding columns: zero in L, outside W
+        const Real w0 = in ? W[3 * j] : Real(0), w1 = in ? W[3 * j + 1] : Real(0), w2 = in ? W[3 * j + 2] : Real(0);
+#pragma unroll
+        for (int q = 0; q < kDR; ++q) {
+          const Real l = lv[q][u];
+          acc[q][0] = fmaT(l, w0, acc[q][0]);
+          acc[q][1] = fmaT(l, w1, acc[q][1]);
+          acc[q][2] = fmaT(l, w2, acc[q][2]);
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < kDR; ++q)
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int o = kWave / 2; o >= 1; o >>= 1) acc[q][d] += __shfl_xor(acc[q][d], o);
+#pragma unroll
+    for (int q = 0; q < kDR; ++q)
+      if (lane == q && r0 + q < m) store(r0 + q, acc[q][0], acc[q][1], acc[q][2]);
+  }
+}
+
+// One dense solve inside the grid form: W (complete, rank space) -> store, then a
+// grid barrier.  W is first copied into each workgroup's LDS when it fits.
+// Row blocks are dealt to workgroups first (wavefront w of workgroup b takes row
+// block w * gridDim + b, then every GW-th), so every CU streams rows even when
+// there are fewer row blocks than wavefronts.
+__device__ inline int denseWave() { return (int)(threadIdx.x / kWave) * (int)gridDim.x + (int)blockIdx.x; }
+
+template <typename Real, typename Store>
+__device__ PDX_INLINE void gridDense(cg::grid_group& grid, const InstDev& I, PDX_GLOBAL Real* W, Store store) {
+  const int nw = blockDim.x / kWave;
+  const int gw = denseWave();
+  const int GW = gridDim.x * nw;
+  const PDX_GLOBAL Real* L = (const PDX_GLOBAL Real*)I.Linv;
+  if (I.denseLdsW) {
+    PDX_LDS Real* Ws = ldsAt<Real>(32 * sizeof(double));
+    for (int i = threadIdx.x; i < 3 * I.m; i += blockDim.x) Ws[i] = W[i];
+    __syncthreads();
+    denseRows<Real>(L, I.m, I.denseLd, Ws, gw, GW, store);
+  } else {
+    denseRows<Real>(L, I.m, I.denseLd, (const PDX_GLOBAL Real*)W, gw, GW, store);
+  }
+  grid.sync();
+}
+
+template <typename Real, bool kExact, bool kIdx16, bool kDense>
 __global__ void __launch_bounds__(512) gridSubstepKernel(const InstDev* __restrict__ insts, int inst, int nsub,
                                                           double* __restrict__ gslots, int waveRuns) {
   extern __shared__ double smem[];
@@ -1418,9 +1535,33 @@ __global__ void __launch_bounds__(512) gridSubstepKernel(const InstDev* __restri
       }
     };
     for (int k = 0; k < I.maxIter; ++k) {
-      gridSweep<Real, true, kIdx16>(grid, I.bfwd, W, Pw, none);
-      gridSweep<Real, false, kIdx16>(grid, I.bbwd, W, Pw, put);
-      if (!kExact) {
+      if (kDense) {
+        if (kExact) {
+          gridDense<Real>(grid, I, W, put);
+        } else {
+          // the update is applied as each row finishes: the product reads only W
+          const uint8_t* pinned = I.pinned;
+          const double* pinpos = I.pinpos;
+          auto update = [=](int r, Real a0, Real a1, Real a2) {
+            const int v = rankv[r];
+            if (pinned[v]) {
+              x[3 * v] = pinpos[3 * v];
+              x[3 * v + 1] = pinpos[3 * v + 1];
+              x[3 * v + 2] = pinpos[3 * v + 2];
+            } else {
+              x[3 * v] += (double)a0;
+              x[3 * v + 1] += (double)a1;
+              x[3 * v + 2] += (double)a2;
+            }
+          };
+          gridDense<Real>(grid, I, W, update);
+        }
+      } else {
+        gridSweep<Real, true, kIdx16>(grid, I.bfwd, W, Pw, none);
+        gridSweep<Real, false, kIdx16>(grid, I.bbwd, W, Pw, put);
+      }
+      if (!kExact && !kDense) {
+        const PDX_GLOBAL Real* U = W;
         for (int v = gt; v < n; v += gs) {
           const int r = I.vrank[v];
           if (I.pinned[v]) {
@@ -1428,9 +1569,9 @@ __global__ void __launch_bounds__(512) gridSubstepKernel(const InstDev* __restri
             I.x[3 * v + 1] = I.pinpos[3 * v + 1];
             I.x[3 * v + 2] = I.pinpos[3 * v + 2];
           } else {
-            I.x[3 * v] += (double)W[3 * r];
-            I.x[3 * v + 1] += (double)W[3 * r + 1];
-            I.x[3 * v + 2] += (double)W[3 * r + 2];
+            I.x[3 * v] += (double)U[3 * r];
+            I.x[3 * v + 1] += (double)U[3 * r + 1];
+            I.x[3 * v + 2] += (double)U[3 * r + 2];
           }
         }
         grid.sync();
@@ -1464,13 +1605,12 @@ __global__ void __launch_bounds__(512) gridSubstepKernel(const InstDev* __restri
           }
         }
       }
-      const double diff = gridMax(grid, ldiff, red, gslots, buf);
+      double diff = 0.0, residual = -1.0;
+      if (I.wantResidual)
+        gridMax2(grid, ldiff, lres, red, gslots, buf, diff, residual);
+      else
+        diff = gridMax(grid, ldiff, red, gslots, buf);
       buf ^= 1;
-      double residual = -1.0;
-      if (I.wantResidual) {
-        residual = gridMax(grid, lres, red, gslots, buf);
-        buf ^= 1;
-      }
       const double rel = (scale > 0.0) ? diff / scale : diff;
       const bool absHit = (I.absTol > 0.0) && (diff <= I.absTol);
       const bool relHit = (I.relTol > 0.0) && (rel <= I.relTol);
@@ -1531,6 +1671,33 @@ __global__ void __launch_bounds__(512) gridSolveKernel(const InstDev* __restrict
   };
   gridSweep<Real, true, kIdx16>(grid, I.bfwd, W, Pw, none);
   gridSweep<Real, false, kIdx16>(grid, I.bbwd, W, Pw, put);
+}
+
+// Dense solve mode, x = Linv b: W = P b (rank space) ...
+template <typename Real>
+__global__ void __launch_bounds__(512) densePermuteKernel(const InstDev* __restrict__ insts, int inst,
+                                                          const double* __restrict__ b) {
+  const InstDev& I = insts[inst];
+  PDX_GLOBAL Real* W = (PDX_GLOBAL Real*)I.Wg;
+  for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < I.n; v += gridDim.x * blockDim.x)
+    putW<Real>(W, I.vrank[v], b[3 * v], b[3 * v + 1], b[3 * v + 2]);
+}
+
+// ... then the rows of Linv W, each written to its vertex.
+template <typename Real>
+__global__ void __launch_bounds__(512) denseSolveKernel(const InstDev* __restrict__ insts, int inst,
+                                                        double* __restrict__ xout) {
+  const InstDev& I = insts[inst];
+  const int nw = blockDim.x / kWave;
+  const int* rankv = I.rankv;
+  auto put = [=](int r, Real a0, Real a1, Real a2) {
+    const int v = rankv[r];
+    xout[3 * v] = (double)a0;
+    xout[3 * v + 1] = (double)a1;
+    xout[3 * v + 2] = (double)a2;
+  };
+  denseRows<Real>((const PDX_GLOBAL Real*)I.Linv, I.m, I.denseLd, (const PDX_GLOBAL Real*)I.Wg, denseWave(),
+                  gridDim.x * nw, put);
 }
 
 template <typename Real, bool kBlocked, bool kIdx16, typename WP>
@@ -1628,11 +1795,47 @@ hipError_t launchGrid(const InstDev* d, int inst, int nsub, double* gslots, int 
   void* args[] = {(void*)&d, (void*)&inst, (void*)&nsub, (void*)&gslots, (void*)&waveRuns};
 #define PDX_G(R, E, X16)                                                                                 \
   if (fp32 == (sizeof(R) == 4) && idx16 == X16)                                                          \
-    return hipLaunchCooperativeKernel((const void*)gridSubstepKernel<R, E, X16>, dim3(blocks), dim3(block), \
-                                      args, lds, s);
+    return hipLaunchCooperativeKernel((const void*)gridSubstepKernel<R, E, X16, false>, dim3(blocks),     \
+                                      dim3(block), args, lds, s);
   PDX_G(float, false, true) PDX_G(float, false, false) PDX_G(double, true, true) PDX_G(double, true, false)
 #undef PDX_G
   return hipErrorInvalidValue;
+}
+
+hipError_t launchDense(const InstDev* d, int inst, int nsub, double* gslots, int blocks, int block, size_t lds,
+                       bool fp32, hipStream_t s) {
+  int waveRuns = 0;
+  void* args[] = {(void*)&d, (void*)&inst, (void*)&nsub, (void*)&gslots, (void*)&waveRuns};
+  if (fp32)
+    return hipLaunchCooperativeKernel((const void*)gridSubstepKernel<float, false, false, true>, dim3(blocks),
+                                      dim3(block), args, lds, s);
+  return hipLaunchCooperativeKernel((const void*)gridSubstepKernel<double, true, false, true>, dim3(blocks),
+                                    dim3(block), args, lds, s);
+}
+
+int denseBlocksMax(int block, size_t lds, bool fp32) {
+  int perCu = 0, dev = 0, cus = 0;
+  hipGetDevice(&dev);
+  hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  const void* f = fp32 ? (const void*)gridSubstepKernel<float, false, false, true>
+                       : (const void*)gridSubstepKernel<double, true, false, true>;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, f, block, lds) != hipSuccess) return 0;
+  return perCu * cus;
+}
+
+hipError_t launchDenseSolve(const InstDev* d, int inst, const double* b, double* x, int n, int m, bool fp32,
+                            hipStream_t s) {
+  const int pb = n > 1024 * 512 ? 1024 : (n + 511) / 512;
+  const int rowBlocks = (m + 8 * kDR - 1) / (8 * kDR);  // 8 wavefronts x kDR rows per workgroup
+  const int sb = rowBlocks > 2048 ? 2048 : rowBlocks;
+  if (fp32) {
+    hipLaunchKernelGGL((densePermuteKernel<float>), dim3(pb), dim3(512), 0, s, d, inst, b);
+    hipLaunchKernelGGL((denseSolveKernel<float>), dim3(sb), dim3(512), 0, s, d, inst, x);
+  } else {
+    hipLaunchKernelGGL((densePermuteKernel<double>), dim3(pb), dim3(512), 0, s, d, inst, b);
+    hipLaunchKernelGGL((denseSolveKernel<double>), dim3(sb), dim3(512), 0, s, d, inst, x);
+  }
+  return hipGetLastError();
 }
 
 hipError_t launchGridSolve(const InstDev* d, int inst, const double* b, double* x, int waveRuns, int blocks,
@@ -1653,8 +1856,8 @@ int gridBlocksMax(int block, size_t lds, bool fp32, bool idx16) {
   hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
 #define PDX_G(R, E, X16)                                                                                        \
   if (fp32 == (sizeof(R) == 4) && idx16 == X16)                                                                 \
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, (const void*)gridSubstepKernel<R, E, X16>, block, \
-                                                     lds) != hipSuccess)                                        \
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, (const void*)gridSubstepKernel<R, E, X16, false>,  \
+                                                     block, lds) != hipSuccess)                                 \
       return 0;
   PDX_G(float, false, true) PDX_G(float, false, false) PDX_G(double, true, true) PDX_G(double, true, false)
 #undef PDX_G
@@ -1694,11 +1897,12 @@ hipError_t setLdsLimit(size_t bytes) {
   PDX_KERNEL_TABLE(PDX_L)
 #undef PDX_L
 #undef PDX_A
-#define PDX_G(R, E, X16)                                                                                  \
-  e = hipFuncSetAttribute((const void*)gridSubstepKernel<R, E, X16>, hipFuncAttributeMaxDynamicSharedMemorySize, \
-                          (int)bytes);                                                                    \
+#define PDX_G(R, E, X16, D)                                                                                  \
+  e = hipFuncSetAttribute((const void*)gridSubstepKernel<R, E, X16, D>, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                          (int)bytes);                                                                       \
   if (e != hipSuccess) return e;
-  PDX_G(float, false, true) PDX_G(float, false, false) PDX_G(double, true, true) PDX_G(double, true, false)
+  PDX_G(float, false, true, false) PDX_G(float, false, false, false) PDX_G(double, true, true, false)
+  PDX_G(double, true, false, false) PDX_G(float, false, false, true) PDX_G(double, true, false, true)
 #undef PDX_G
 #define PDX_G(R, X16)                                                                                   \
   e = hipFuncSetAttribute((const void*)gridSolveKernel<R, X16>, hipFuncAttributeMaxDynamicSharedMemorySize, \

@@ -119,7 +119,8 @@ const char* kUsage =
     "  --shear K           shear diagonals with stiffness K (grids; also PD_SHEAR)\n"
     "  --bend K            linear bend stencils with stiffness K (grids; also PD_BEND)\n"
     "  --precision P       fp64 (default, the drop-in) or mixed32\n"
-    "  --solve S           exact (default, the reference's sum order) or fast\n"
+    "  --solve S           exact (default, the reference's sum order), fast (chain-blocked\n"
+    "                      sweeps) or dense (the inverse formed once, one product per iteration)\n"
     "  --device D          HIP device (default 0)\n";
 
 struct Options {
@@ -206,11 +207,11 @@ int parse(int argc, const char* const* argv, Options& o, std::string& err) {
       o.precision = p == "fp64" ? PDX_PREC_FP64 : PDX_PREC_MIXED32;
     } else if (a == "--solve" && i + 1 < argc) {
       const std::string p = argv[++i];
-      if (p != "exact" && p != "fast") {
-        err = "--solve takes exact or fast\n";
+      if (p != "exact" && p != "fast" && p != "dense") {
+        err = "--solve takes exact, fast or dense\n";
         return -1;
       }
-      o.solve = p == "exact" ? PDX_SOLVE_EXACT : PDX_SOLVE_FAST;
+      o.solve = p == "exact" ? PDX_SOLVE_EXACT : (p == "fast" ? PDX_SOLVE_FAST : PDX_SOLVE_DENSE);
     } else if (a == "--help" || a == "-h") {
       o.help = true;
     }
@@ -371,7 +372,8 @@ extern "C" int pdx_bench_run(int argc, const char* const* argv, char* text, long
                                  cfg.gravity[2] * cfg.gravity[2]);
   out.add("=== PD cloth benchmark (pdx, MI355X) ===\n");
   out.add("vertices %d  constraints %d  mass %.6g  device %d  precision %s  solve %s\n", s.n, s.E, d.total_mass,
-          o.device, o.precision == PDX_PREC_FP64 ? "fp64" : "mixed32", o.solve == PDX_SOLVE_EXACT ? "exact" : "fast");
+          o.device, o.precision == PDX_PREC_FP64 ? "fp64" : "mixed32",
+          o.solve == PDX_SOLVE_EXACT ? "exact" : (o.solve == PDX_SOLVE_FAST ? "fast" : "dense"));
   out.add("dt %.6g  substeps/frame %d  stiffness %.6g  PD iteration cap %d\n", o.dt, o.substeps, o.stiffness, o.iters);
   if (s.B || o.shear > 0.0)
     out.add("bend stencils %d (k %.6g)  shear stiffness %.6g\n", s.B, o.bend, o.shear);

@@ -21,6 +21,7 @@ PDX_PREC_FP64 = 0
 PDX_PREC_MIXED32 = 1
 PDX_SOLVE_EXACT = 0
 PDX_SOLVE_FAST = 1
+PDX_SOLVE_DENSE = 2
 
 _STATUS = {0: "ok", -1: "invalid argument", -2: "not ready", -3: "not positive definite",
            -4: "HIP runtime error", -5: "unsupported", -6: "out of memory"}
@@ -247,7 +248,7 @@ class Engine:
     def __init__(self, precision="fp64", solve="exact", device=0):
         self.lib = load_library()
         prec = {"fp64": PDX_PREC_FP64, "mixed32": PDX_PREC_MIXED32}[precision]
-        mode = {"exact": PDX_SOLVE_EXACT, "fast": PDX_SOLVE_FAST}[solve]
+        mode = {"exact": PDX_SOLVE_EXACT, "fast": PDX_SOLVE_FAST, "dense": PDX_SOLVE_DENSE}[solve]
         h = C.c_void_p()
         rc = self.lib.pdx_create(C.byref(h), device, prec, mode)
         if rc != 0:

@@ -72,6 +72,26 @@ def test_grid_form_needs_one_instance_and_fast_mode():
     e.close()
 
 
+def test_dense_mode_preconditions():
+    """The dense solve mode takes one instance and no differentiable rollout;
+    with one instance the host factor (and the inverse) forms without a GPU."""
+    cfg = SimConfig.fixed_iterations(10, 1.0 / 600.0)
+    e = Engine("fp64", "dense")
+    e.add(_cloth(), cfg)
+    e.add(_cloth(), cfg)
+    with pytest.raises(PdxError):
+        e.factor_arrays(0)
+    e.close()
+    e = Engine("mixed32", "dense")
+    e.add(_cloth(), cfg)
+    f = e.factor_arrays(0)
+    assert len(f["dinv"]) == _cloth().n
+    with pytest.raises(PdxError):
+        e.rollout(4, 2)
+    e.set_launch("grid", 32)  # only the workgroup count applies
+    e.close()
+
+
 def test_invalid_scenes_are_rejected():
     c = _cloth()
     c.edge_a = c.edge_a.copy()
