@@ -159,6 +159,26 @@ def test_autotune_leaves_no_trace():
 
 
 @pytest.mark.gpu
+def test_workgroup_size_keeps_bits():
+    """Each row's sums follow the schedule, not the thread count: 256- and
+    512-thread launches of the batch kernel give the same bits."""
+    print("[case] mixed32 fast batch at 256 / 512 threads", flush=True)
+    cl = [scenes.config4_instance(4, i) for i in range(6)]
+    cfg = SimConfig.fixed_iterations(10, 1.0 / 600.0)
+    out = {}
+    for threads in (256, 512):
+        e = Engine("mixed32", "fast")
+        for c in cl:
+            e.add(c, cfg)
+        e.set_schedule(threads, 32)
+        e.step(6)
+        out[threads] = [e.state(i)[0] for i in range(len(cl))]
+        e.close()
+    for i in range(len(cl)):
+        assert np.array_equal(out[512][i].view(np.uint64), out[256][i].view(np.uint64))
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("precision,nx,ny,shear,bend,iters,nsub,tol", [
     ("fp64", 40, 40, 0.0, 0.0, 10, 20, 1e-10),
     ("fp64", 100, 100, 500.0, 50.0, 40, 3, 1e-10),
