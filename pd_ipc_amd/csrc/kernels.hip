@@ -633,10 +633,14 @@ template <bool kRhs>
 __device__ inline double residualAndRhsT(const InstDev& I, int v, Acc3& rhsOut) {
   const double* x = I.x;
   const double xv0 = x[3 * v], xv1 = x[3 * v + 1], xv2 = x[3 * v + 2];
+  // every load that needs only v, issued together
+  const int p0 = I.vstart[v], ne = I.vstart[v + 1] - p0;
+  const int s0 = I.bstart[v], nb = I.bstart[v + 1] - s0;
+  const double mass = I.mass[v];
+  const double xh0 = I.xhat[3 * v], xh1 = I.xhat[3 * v + 1], xh2 = I.xhat[3 * v + 2];
   Acc3 f{0.0, 0.0, 0.0};
   Acc3 r{0.0, 0.0, 0.0};
   {
-    const int p0 = I.vstart[v], ne = I.vstart[v + 1] - p0;
     for (int g = 0; g < ne; g += kMaxE) {
       int rc[kMaxE];
       double rest[kMaxE], kap[kMaxE], ux[kMaxE][3];
@@ -663,18 +667,17 @@ __device__ inline double residualAndRhsT(const InstDev& I, int v, Acc3& rhsOut) 
   }
   Acc3 fb{0.0, 0.0, 0.0};
   {
-    const int p0 = I.bstart[v], nb = I.bstart[v + 1] - p0;
     for (int g = 0; g < nb; g += kMaxS) {
       int ia[kMaxS], ib[kMaxS], ic[kMaxS];
       double K[kMaxS], w[kMaxS], xa[kMaxS][3], xb[kMaxS][3], xc[kMaxS][3];
 #pragma unroll
       for (int q = 0; q < kMaxS; ++q)
         if (g + q < nb) {
-          ia[q] = I.brA[p0 + g + q];
-          ib[q] = I.brB[p0 + g + q];
-          ic[q] = I.brC[p0 + g + q];
-          K[q] = I.brK[p0 + g + q];
-          w[q] = (double)I.bweight[p0 + g + q];
+          ia[q] = I.brA[s0 + g + q];
+          ib[q] = I.brB[s0 + g + q];
+          ic[q] = I.brC[s0 + g + q];
+          K[q] = I.brK[s0 + g + q];
+          w[q] = (double)I.bweight[s0 + g + q];
         }
 #pragma unroll
       for (int q = 0; q < kMaxS; ++q)
@@ -695,10 +698,8 @@ __device__ inline double residualAndRhsT(const InstDev& I, int v, Acc3& rhsOut) 
   f.x += fb.x;
   f.y += fb.y;
   f.z += fb.z;
-  const double mass = I.mass[v];
   const double mi = mass * I.invH2;
-  const double i0 = (xv0 - I.xhat[3 * v]) * mi, i1 = (xv1 - I.xhat[3 * v + 1]) * mi,
-               i2 = (xv2 - I.xhat[3 * v + 2]) * mi;
+  const double i0 = (xv0 - xh0) * mi, i1 = (xv1 - xh1) * mi, i2 = (xv2 - xh2) * mi;
   if (kRhs) {
     rhsOut.x = r.x - i0;
     rhsOut.y = r.y - i1;
