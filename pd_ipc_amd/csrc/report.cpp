@@ -86,12 +86,17 @@ const Preset kPresets[] = {{"preview", 20, 0.3}, {"realtime", 40, 0.3}, {"accura
 struct Text {
   std::string s;
   void add(const char* fmt, ...) __attribute__((format(printf, 2, 3))) {
-    char buf[1024];
-    va_list ap;
+    va_list ap, again;
     va_start(ap, fmt);
-    const int k = std::vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_copy(again, ap);
+    const int k = std::vsnprintf(nullptr, 0, fmt, ap);
     va_end(ap);
-    if (k > 0) s.append(buf, std::min<size_t>((size_t)k, sizeof(buf) - 1));
+    if (k > 0) {
+      std::vector<char> buf((size_t)k + 1);
+      std::vsnprintf(buf.data(), buf.size(), fmt, again);
+      s.append(buf.data(), (size_t)k);
+    }
+    va_end(again);
   }
 };
 

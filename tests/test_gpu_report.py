@@ -119,6 +119,21 @@ def test_report_matches_oracle_figures():
     assert s["converged_substeps"] == conv
 
 
+def test_report_status_lines_and_pin_all():
+    """--report N prints a status line every N substeps (pd_bench's step /
+    iterations / mean strain / energy / max speed); --pin-all holds every vertex
+    (nothing moves, zero speed, energy unchanged)."""
+    _say("report --report 5 --steps 20 and --pin-all")
+    text, s = report.run(["--grid", "12", "10", "--steps", "20", "--report", "5", "--iters", "6"])
+    lines = [ln for ln in text.splitlines() if ln.strip().startswith("step ")]
+    assert [int(ln.split()[1]) for ln in lines] == [5, 10, 15, 20]
+    assert all(len(ln.split()) == 11 for ln in lines)
+    text0, s0 = report.run(["--grid", "12", "10", "--steps", "0"])
+    _, sp = report.run(["--grid", "12", "10", "--steps", "10", "--pin-all"])
+    assert sp["n_pinned"] == 120 and sp["max_speed"] == 0.0 and sp["max_strain"] == 0.0
+    assert sp["total_energy"] == s0["total_energy"] and s0["substeps"] == 0
+
+
 def _sheet(path, nx, ny):
     import sys
     sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))

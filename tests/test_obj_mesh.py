@@ -226,6 +226,7 @@ def test_report_help_and_bad_preset_on_cpu():
     from pd_ipc_amd import report
     text, _ = report.run(["--help"])
     assert "--preset NAME" in text and "--mesh PATH" in text
+    assert text.rstrip().endswith("--device D          HIP device (default 0)")  # the whole text
     with pytest.raises(PdxError) as ei:
         report.run(["--preset", "fastest"])
     assert "preview, realtime, accurate" in str(ei.value)
