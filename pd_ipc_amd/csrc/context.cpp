@@ -57,9 +57,16 @@ struct HostSweep {
   int nlev = 0;
 };
 
+// Rows taken by a whole wavefront (FP64 schedules): every row of at least
+// kLongRowAny entries, and in a level of at most kLongLevel rows (one per
+// wavefront) every row of at least kLongRow entries.
+constexpr int kLongLevel = 8, kLongRow = 16, kLongRowAny = 64;
+
 // Lays one sweep out level by level, <=64-row chunks sorted by row length
-// (descending), entries jagged-diagonal inside a chunk.
-void packSweep(const Sweep& sw, bool fwd, const Factor& f, HostSweep& out) {
+// (descending), entries jagged-diagonal inside a chunk.  With `longRows`, the
+// long rows of narrow levels become one-row chunks marked by a row count of -1
+// (their entries in row order), which a whole wavefront takes.
+void packSweep(const Sweep& sw, bool fwd, const Factor& f, bool longRows, HostSweep& out) {
   out = HostSweep{};
   out.nlev = sw.nlev;
   out.levChunk.push_back(0);
@@ -70,7 +77,23 @@ void packSweep(const Sweep& sw, bool fwd, const Factor& f, HostSweep& out) {
       return fwd ? f.rp[(size_t)k + 1] - f.rp[(size_t)k] : f.cp[(size_t)k + 1] - f.cp[(size_t)k];
     };
     std::stable_sort(rows.begin(), rows.end(), [&](int a, int b) { return lenOf(a) > lenOf(b); });
-    for (size_t c0 = 0; c0 < rows.size(); c0 += 64) {
+    size_t first = 0;
+    const int longFrom = rows.size() <= (size_t)kLongLevel ? kLongRow : kLongRowAny;
+    if (longRows)
+      while (first < rows.size() && lenOf(rows[first]) >= longFrom) {
+        const int k = rows[first++];
+        out.chunkRows.push_back(-1);
+        out.chunkRowBase.push_back((int)out.slotRank.size());
+        out.chunkEntBase.push_back((int)out.col.size());
+        out.slotRank.push_back(k);
+        out.slotLen.push_back(lenOf(k));
+        for (int j = 0; j < lenOf(k); ++j) {
+          const size_t p = fwd ? (size_t)f.rp[(size_t)k] + j : (size_t)f.cp[(size_t)k] + j;
+          out.col.push_back(fwd ? f.rc[p] : f.ci[p]);
+          out.val.push_back(fwd ? f.rx[p] : f.cx[p]);
+        }
+      }
+    for (size_t c0 = first; c0 < rows.size(); c0 += 64) {
       const size_t c1 = std::min(rows.size(), c0 + 64);
       out.chunkRows.push_back((int)(c1 - c0));
       out.chunkRowBase.push_back((int)out.slotRank.size());
@@ -556,8 +579,9 @@ int refreshFactors(pdx_ctx* c, bool& changed) {
             packBlocked(I.fac, false, c->chainCap, runCap, runCap, denseCap, I.bb);
           }
         } else {
-          packSweep(I.fac.fwd, true, I.fac, I.hf);
-          packSweep(I.fac.bwd, false, I.fac, I.hb);
+          const bool longRows = c->prec == PDX_PREC_FP64;
+          packSweep(I.fac.fwd, true, I.fac, longRows, I.hf);
+          packSweep(I.fac.bwd, false, I.fac, longRows, I.hb);
         }
         I.packed = true;
       };

@@ -100,6 +100,10 @@ __device__ inline float subMul<float>(float acc, float l, float w) {
 
 constexpr int kU = 8;  // entries per load group of the level sweep
 
+template <bool kFwd, typename WP, typename Store>
+__device__ PDX_INLINE void longRow(const SweepDev& sw, WP W, const double* dinv, int rbase, int ebase, int lane,
+                                   Store store);
+
 // One level-scheduled sweep.  Forward: row k gathers L(k,j) y_j over ascending j.
 // Backward: column k gathers L(i,k) x_i over ascending i after scaling by 1/D_k.
 // `store(k, a0, a1, a2)` is called once per row with its final value.
@@ -112,9 +116,18 @@ __device__ PDX_INLINE void sweep(const SweepDev& sw, WP W, const Real* dinv, Sto
   for (int l = 0; l < sw.nlev; ++l) {
     const int c1 = sw.levChunk[l + 1];
     for (int ch = sw.levChunk[l] + wave; ch < c1; ch += nw) {
-      const int nrows = sw.chunkRows[ch];
+      int nrows = sw.chunkRows[ch];
       const int rbase = sw.chunkRowBase[ch];
       const int ebase = sw.chunkEntBase[ch];
+      if (nrows < 0) {
+        // one long row taken by the whole wavefront (FP64 schedules only)
+        if constexpr (sizeof(Real) == 8) {
+          longRow<kFwd>(sw, W, dinv, rbase, ebase, lane, store);
+          continue;
+        } else {
+          nrows = 1;
+        }
+      }
       int k = 0, len = 0;
       if (lane < nrows) {
         k = sw.slotRank[rbase + lane];
@@ -183,6 +196,45 @@ __device__ inline double readlaneT(double v, int l) {
 }
 __device__ inline float readlaneT(float v, int l) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+// One row of an FP64 level sweep taken by the whole wavefront, with the
+// reference's operations in the reference's order: each lane forms the products
+// l * w of its entries (each rounded, as the reference rounds its product), then
+// the wavefront subtracts them from the row's value one by one in entry order
+// (readlane broadcasts), 64 entries at a time.  Row entries are stored in row
+// order (a one-row chunk).
+template <bool kFwd, typename WP, typename Store>
+__device__ PDX_INLINE void longRow(const SweepDev& sw, WP W, const double* dinv, int rbase, int ebase, int lane,
+                                   Store store) {
+  const int k = sw.slotRank[rbase], len = sw.slotLen[rbase];
+  const double* val = static_cast<const double*>(sw.val);
+  double a0 = W[3 * k], a1 = W[3 * k + 1], a2 = W[3 * k + 2];
+  if (!kFwd) {
+    const double dv = dinv[k];
+    a0 = a0 * dv;
+    a1 = a1 * dv;
+    a2 = a2 * dv;
+  }
+  for (int j0 = 0; j0 < len; j0 += kWave) {
+    const int j = j0 + lane;
+    const int e = ebase + (j < len ? j : 0);  // lanes past the row reload its first entry
+    const int c = sw.col[e];
+    const double lv = val[e];
+    const double p0 = lv * W[3 * c], p1 = lv * W[3 * c + 1], p2 = lv * W[3 * c + 2];
+    const int cnt = min(kWave, len - j0);
+    for (int i = 0; i < cnt; ++i) {
+      a0 = a0 - readlaneT(p0, i);
+      a1 = a1 - readlaneT(p1, i);
+      a2 = a2 - readlaneT(p2, i);
+    }
+  }
+  if (lane == 0) {
+    W[3 * k + 0] = a0;
+    W[3 * k + 1] = a1;
+    W[3 * k + 2] = a2;
+    store(k, a0, a1, a2);
+  }
 }
 
 constexpr int kCap = 32;
