@@ -298,8 +298,10 @@ int pdx_matrix_nnz(const pdx_ctx* ctx, int instance, long long* nnz);
  *   --residual-tol T, --tol T, --threads N (accepted, no effect), --pin-top,
  *   --pin-all, --report N, --help
  * plus --shear K and --bend K (the reference reads PD_SHEAR / PD_BEND from the
- * environment; both are honoured too), --precision fp64|mixed32,
- * --solve exact|fast and --device D.  A preset sets the iteration cap and the
+ * environment; both are honoured too), --cheb-rho R and --cheb-start S (the
+ * Chebyshev blend; PD_CHEB=1, PD_CHEB_RHO, PD_CHEB_START as the reference's
+ * substep reads them are honoured too), --precision fp64|mixed32,
+ * --solve exact|fast|dense and --device D.  A preset sets the iteration cap and the
  * residual tolerance; explicit --iters / --residual-tol override it whatever the
  * order.  Unknown options are ignored, as in pd_bench.
  * The report text (NUL-terminated, truncated to text_cap - 1 bytes) goes to

@@ -11,6 +11,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libpdx.so")
+BENCH = os.path.join(HERE, "pdx_bench")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("PDX_ARCH", "gfx950")
 
@@ -51,6 +52,9 @@ def build(verbose=False):
     tmp = OUT + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs + ["-lpthread"])
     os.replace(tmp, OUT)
+    # pdx_bench, the report as a program (next to the library, found through its rpath)
+    _run([HIPCC, "-O2", "-std=c++17", "-x", "c++", os.path.join(CSRC, "bench_main.cpp"), "-x", "none",
+          "-o", BENCH, f"-L{HERE}", "-lpdx", "-Wl,-rpath,$ORIGIN"])
     return OUT
 
 

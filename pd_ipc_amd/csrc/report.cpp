@@ -126,6 +126,8 @@ const char* kUsage =
     "  --precision P       fp64 (default, the drop-in) or mixed32\n"
     "  --solve S           exact (default, the reference's sum order), fast (chain-blocked\n"
     "                      sweeps) or dense (the inverse formed once, one product per iteration)\n"
+    "  --cheb-rho R        Chebyshev blend with spectral radius R (also PD_CHEB=1, PD_CHEB_RHO)\n"
+    "  --cheb-start S      plain iterations before the blend (default 10; also PD_CHEB_START)\n"
     "  --device D          HIP device (default 0)\n";
 
 struct Options {
@@ -137,6 +139,8 @@ struct Options {
   int iters = 10;
   double residualTol = 1.0e-3, relTol = 1.0e-3, absTol = 1.0e-12;
   double shear = 0.0, bend = 0.0;
+  double chebRho = 0.0;  // > 0: Chebyshev blend
+  int chebStart = 10;
   bool pinAll = false, help = false;
   int preset = -1;
   int precision = PDX_PREC_FP64, solve = PDX_SOLVE_EXACT;
@@ -203,6 +207,10 @@ int parse(int argc, const char* const* argv, Options& o, std::string& err) {
       real(o.bend);
     } else if (a == "--device") {
       whole(o.device);
+    } else if (a == "--cheb-rho") {
+      real(o.chebRho);
+    } else if (a == "--cheb-start") {
+      whole(o.chebStart);
     } else if (a == "--precision" && i + 1 < argc) {
       const std::string p = argv[++i];
       if (p != "fp64" && p != "mixed32") {
@@ -236,6 +244,17 @@ int parse(int argc, const char* const* argv, Options& o, std::string& err) {
   if (const char* env = std::getenv("PD_BEND")) {
     const double k = std::atof(env);
     if (k > 0.0) o.bend = k;
+  }
+  // the Chebyshev blend switches the reference's substep reads (Integrator.cpp:493-503):
+  // PD_CHEB set, non-empty and not starting with '0' turns it on, rho 0.98 and
+  // start 10 unless PD_CHEB_RHO / PD_CHEB_START give them
+  if (const char* env = std::getenv("PD_CHEB")) {
+    if (*env != '\0' && *env != '0') {
+      const char* r = std::getenv("PD_CHEB_RHO");
+      const char* s = std::getenv("PD_CHEB_START");
+      o.chebRho = (r && *r) ? std::atof(r) : 0.98;
+      o.chebStart = (s && *s) ? std::atoi(s) : 10;
+    }
   }
   return 0;
 }
@@ -368,6 +387,8 @@ extern "C" int pdx_bench_run(int argc, const char* const* argv, char* text, long
   cfg.rel_tol = o.relTol;
   cfg.abs_tol = o.absTol;
   cfg.residual_tol = o.residualTol;
+  cfg.chebyshev_rho = o.chebRho;
+  cfg.chebyshev_start = o.chebStart;
   int inst = 0;
   if ((rc = pdx_add_instance(ctx, &sc, &cfg, &inst))) return fail("pdx_add_instance", rc);
 
@@ -382,6 +403,8 @@ extern "C" int pdx_bench_run(int argc, const char* const* argv, char* text, long
   out.add("dt %.6g  substeps/frame %d  stiffness %.6g  PD iteration cap %d\n", o.dt, o.substeps, o.stiffness, o.iters);
   if (s.B || o.shear > 0.0)
     out.add("bend stencils %d (k %.6g)  shear stiffness %.6g\n", s.B, o.bend, o.shear);
+  if (o.chebRho > 0.0)
+    out.add("Chebyshev blend: rho %.6g, from iteration %d\n", o.chebRho, o.chebStart + 1);
   if (o.preset >= 0)
     out.add("iteration preset %s\n", kPresets[o.preset].name);
   else

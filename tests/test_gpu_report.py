@@ -119,6 +119,25 @@ def test_report_matches_oracle_figures():
     assert s["converged_substeps"] == conv
 
 
+def test_report_chebyshev_from_environment(monkeypatch):
+    """pd_bench --grid 60 60 --steps 300 with a residual tolerance of 0.3, plain and
+    with PD_CHEB=1 (rho 0.98, start 10) in the environment, as the reference's
+    substep reads it (docs/chebyshev.md:284-287): final residuals 12.3328 and
+    9.37677 at 20 iterations; the explicit --cheb-rho option gives the same run."""
+    _say("report 60x60 plain / PD_CHEB=1 / --cheb-rho, 20 iterations")
+    opts = ["--grid", "60", "60", "--steps", "300", "--iters", "20", "--residual-tol", "0.3"]
+    monkeypatch.delenv("PD_CHEB", raising=False)
+    _, plain = report.run(opts)
+    assert f"{plain['last_residual']:.6g}" == "12.3328"
+    monkeypatch.setenv("PD_CHEB", "1")
+    text, cheb = report.run(opts)
+    assert "Chebyshev blend: rho 0.98, from iteration 11" in text
+    assert f"{cheb['last_residual']:.6g}" == "9.37677"
+    monkeypatch.delenv("PD_CHEB")
+    _, opt = report.run(opts + ["--cheb-rho", "0.98"])
+    assert opt["last_residual"] == cheb["last_residual"]
+
+
 def test_report_status_lines_and_pin_all():
     """--report N prints a status line every N substeps (pd_bench's step /
     iterations / mean strain / energy / max speed); --pin-all holds every vertex

@@ -230,3 +230,16 @@ def test_report_help_and_bad_preset_on_cpu():
     with pytest.raises(PdxError) as ei:
         report.run(["--preset", "fastest"])
     assert "preview, realtime, accurate" in str(ei.value)
+
+
+def test_pdx_bench_program_on_cpu():
+    """The pdx_bench program (built next to the library) parses pd_bench's command
+    line: usage on --help, exit status 2 and a message on an unknown preset."""
+    import subprocess
+    exe = os.path.join(ROOT, "pd_ipc_amd", "pdx_bench")
+    if not os.path.exists(exe):
+        pytest.skip("pdx_bench not built")
+    r = subprocess.run([exe, "--help"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0 and "--preset NAME" in r.stdout
+    r = subprocess.run([exe, "--preset", "fastest"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "preview, realtime, accurate" in r.stderr
