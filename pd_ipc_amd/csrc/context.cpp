@@ -676,8 +676,7 @@ int upload(pdx_ctx* c) {
   Arena ar;
   struct Off {
     size_t x, v, xbs, xhat, prev, mass, pinned, pinpos, brhs, ea, eb, rest, kappa, vstart, vedges, vedgesC,
-        bstart, bst, bw, sa, sb, sc, bk, vrank, rankv, dinv, W, ist, lst, dst, recE, restE, kapE, recC,
-        restC, kapC, brA, brB, brC, brK;
+        bstart, bst, sa, sb, sc, bk, vrank, rankv, dinv, W, ist, lst, dst, recE, rkE, recC, rkC, brR, brK;
     size_t f[9], b[9];
     size_t bf[15], bb[15];
   };
@@ -712,7 +711,7 @@ int upload(pdx_ctx* c) {
     {
       // flattened incidence / bend records (see InstDev)
       std::vector<int> recE(ve.size()), recC(ve.size());
-      std::vector<double> restE(ve.size()), kapE(ve.size()), restC(ve.size()), kapC(ve.size());
+      std::vector<RestKappa> rkE(ve.size()), rkC(ve.size());
       for (int v = 0; v < s.n; ++v)
         for (int p = vs[(size_t)v]; p < vs[(size_t)v + 1]; ++p)
           for (int q = 0; q < 2; ++q) {
@@ -721,28 +720,28 @@ int upload(pdx_ctx* c) {
             const int u = isB ? s.ea[(size_t)e] : s.eb[(size_t)e];
             const int r = (u << 2) | (s.pinned[(size_t)u] ? 2 : 0) | (isB ? 1 : 0);
             (q == 0 ? recE : recC)[(size_t)p] = r;
-            (q == 0 ? restE : restC)[(size_t)p] = s.rest[(size_t)e];
-            (q == 0 ? kapE : kapC)[(size_t)p] = s.kappa[(size_t)e];
+            (q == 0 ? rkE : rkC)[(size_t)p] = RestKappa{s.rest[(size_t)e], s.kappa[(size_t)e]};
           }
-      std::vector<int> bA(bst.size()), bB(bst.size()), bC(bst.size());
+      std::vector<BendRec> bR(bst.size());
       std::vector<double> bK(bst.size());
-      for (size_t p = 0; p < bst.size(); ++p) {
-        const int q = bst[p];
-        auto enc = [&](int x) { return (x << 1) | (s.pinned[(size_t)x] ? 1 : 0); };
-        bA[p] = enc(s.sa[(size_t)q]);
-        bB[p] = enc(s.sb[(size_t)q]);
-        bC[p] = enc(s.sc[(size_t)q]);
-        bK[p] = s.bk[(size_t)q];
-      }
+      auto enc = [&](int x) { return (x << 1) | (s.pinned[(size_t)x] ? 1 : 0); };
+      for (int v = 0; v < s.n; ++v)
+        for (int p = bs[(size_t)v]; p < bs[(size_t)v + 1]; ++p) {
+          const int q = bst[(size_t)p];
+          const int abc[3] = {s.sa[(size_t)q], s.sb[(size_t)q], s.sc[(size_t)q]};
+          const int role = abc[0] == v ? 0 : (abc[1] == v ? 1 : 2);
+          int others[2], k = 0;
+          for (int t = 0; t < 3; ++t)
+            if (t != role) others[k++] = enc(abc[t]);
+          bR[(size_t)p] = BendRec{others[0], others[1], role | (s.pinned[(size_t)v] ? 4 : 0),
+                                  (int)I.topo.bweight[(size_t)p]};
+          bK[(size_t)p] = s.bk[(size_t)q];
+        }
       o.recE = ar.vec(recE);
-      o.restE = ar.vec(restE);
-      o.kapE = ar.vec(kapE);
+      o.rkE = ar.vec(rkE);
       o.recC = ar.vec(recC);
-      o.restC = ar.vec(restC);
-      o.kapC = ar.vec(kapC);
-      o.brA = ar.vec(bA);
-      o.brB = ar.vec(bB);
-      o.brC = ar.vec(bC);
+      o.rkC = ar.vec(rkC);
+      o.brR = ar.vec(bR);
       o.brK = ar.vec(bK);
     }
     o.vstart = ar.vec(vs);
@@ -750,7 +749,6 @@ int upload(pdx_ctx* c) {
     o.vedgesC = ar.vec(vc);
     o.bstart = ar.vec(bs);
     o.bst = ar.vec(bst);
-    o.bw = ar.vec(I.topo.bweight);
     o.sa = ar.vec(s.sa);
     o.sb = ar.vec(s.sb);
     o.sc = ar.vec(s.sc);
@@ -919,18 +917,13 @@ int upload(pdx_ctx* c) {
     d.vedges = (const int*)(B + o.vedges);
     d.vedgesColor = (const int*)(B + o.vedgesC);
     d.recE = (const int*)(B + o.recE);
-    d.restE = (const double*)(B + o.restE);
-    d.kapE = (const double*)(B + o.kapE);
+    d.rkE = (const RestKappa*)(B + o.rkE);
     d.recC = (const int*)(B + o.recC);
-    d.restC = (const double*)(B + o.restC);
-    d.kapC = (const double*)(B + o.kapC);
-    d.brA = (const int*)(B + o.brA);
-    d.brB = (const int*)(B + o.brB);
-    d.brC = (const int*)(B + o.brC);
+    d.rkC = (const RestKappa*)(B + o.rkC);
+    d.brRec = (const BendRec*)(B + o.brR);
     d.brK = (const double*)(B + o.brK);
     d.bstart = (const int*)(B + o.bstart);
     d.bstencil = (const int*)(B + o.bst);
-    d.bweight = (const int8_t*)(B + o.bw);
     d.sa = (const int*)(B + o.sa);
     d.sb = (const int*)(B + o.sb);
     d.sc = (const int*)(B + o.sc);

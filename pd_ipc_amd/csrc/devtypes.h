@@ -5,6 +5,19 @@
 
 namespace pdx {
 
+// An edge's rest length and stiffness, loaded together.
+struct alignas(16) RestKappa {
+  double rest, kap;
+};
+
+// One vertex-stencil slot of the bend gather: the stencil's two other vertices
+// (in stencil order a, b, c with the slot's own vertex left out), each encoded
+// (index << 1) | pinned; the slot vertex's role (0, 1, 2 = a, b, c) | (its pinned
+// flag << 2); its weight (1, -2, 1).
+struct alignas(16) BendRec {
+  int other1, other2, role, weight;
+};
+
 // One level-scheduled triangular sweep in rank space.  Level l owns chunks
 // [levChunk[l], levChunk[l+1]); a chunk is <= 64 rows (one wavefront, one row per
 // lane) sorted by entry count, its entries stored jagged-diagonal: entry j of lane
@@ -87,20 +100,17 @@ struct InstDev {
   // Flattened incidence records (one per vertex-edge slot), so that the gather
   // has one dependent load (the neighbour's position) per edge:
   //   rec = (u << 2) | (u pinned << 1) | (v is the edge's b end)
+  // with the edge's rest length and kappa beside it as one 16-byte pair.
   const int* recE;         // edge order
-  const double* restE;
-  const double* kapE;
+  const RestKappa* rkE;
   const int* recC;         // colour order
-  const double* restC;
-  const double* kapC;
-  // bend records per vertex-stencil slot: vertex = (idx << 1) | pinned
-  const int* brA;
-  const int* brB;
-  const int* brC;
+  const RestKappa* rkC;
+  // bend records per vertex-stencil slot (BendRec: the stencil's other two
+  // vertices, the slot vertex's role, its weight) and the stencil's k
+  const BendRec* brRec;
   const double* brK;
   const int* bstart;
   const int* bstencil;
-  const int8_t* bweight;
   const int* sa;
   const int* sb;
   const int* sc;

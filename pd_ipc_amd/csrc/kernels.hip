@@ -571,7 +571,8 @@ __device__ inline Acc3 rhsExact(const InstDev& I, int v) {
 #pragma unroll 4
   for (int p = I.vstart[v]; p < p1; ++p) {
     const int rc = I.recC[p];
-    const double rest = I.restC[p], k = I.kapC[p];
+    const RestKappa rk = I.rkC[p];
+    const double rest = rk.rest, k = rk.kap;
     const int u = rc >> 2;
     const bool uP = (rc & 2) != 0, isB = (rc & 1) != 0;
     const double u0 = x[3 * u], u1 = x[3 * u + 1], u2 = x[3 * u + 2];
@@ -700,8 +701,9 @@ __device__ inline double residualAndRhsT(const InstDev& I, int v, Acc3& rhsOut) 
       for (int q = 0; q < kMaxE; ++q)
         if (g + q < ne) {
           rc[q] = I.recE[p0 + g + q];
-          rest[q] = I.restE[p0 + g + q];
-          kap[q] = I.kapE[p0 + g + q];
+          const RestKappa rk = I.rkE[p0 + g + q];
+          rest[q] = rk.rest;
+          kap[q] = rk.kap;
         }
 #pragma unroll
       for (int q = 0; q < kMaxE; ++q)
@@ -720,31 +722,45 @@ __device__ inline double residualAndRhsT(const InstDev& I, int v, Acc3& rhsOut) 
   Acc3 fb{0.0, 0.0, 0.0};
   {
     for (int g = 0; g < nb; g += kMaxS) {
-      int ia[kMaxS], ib[kMaxS], ic[kMaxS];
-      double K[kMaxS], w[kMaxS], xa[kMaxS][3], xb[kMaxS][3], xc[kMaxS][3];
+      BendRec br[kMaxS];
+      double K[kMaxS], p1[kMaxS][3], p2[kMaxS][3];
 #pragma unroll
       for (int q = 0; q < kMaxS; ++q)
         if (g + q < nb) {
-          ia[q] = I.brA[s0 + g + q];
-          ib[q] = I.brB[s0 + g + q];
-          ic[q] = I.brC[s0 + g + q];
+          br[q] = I.brRec[s0 + g + q];
           K[q] = I.brK[s0 + g + q];
-          w[q] = (double)I.bweight[s0 + g + q];
         }
+      // the positions of the stencil's other two vertices; the slot vertex's own
+      // position is xv
 #pragma unroll
       for (int q = 0; q < kMaxS; ++q)
         if (g + q < nb) {
-          const int a = ia[q] >> 1, b = ib[q] >> 1, c = ic[q] >> 1;
+          const int u1 = br[q].other1 >> 1, u2 = br[q].other2 >> 1;
 #pragma unroll
           for (int d = 0; d < 3; ++d) {
-            xa[q][d] = x[3 * a + d];
-            xb[q][d] = x[3 * b + d];
-            xc[q][d] = x[3 * c + d];
+            p1[q][d] = x[3 * u1 + d];
+            p2[q][d] = x[3 * u2 + d];
           }
         }
 #pragma unroll
       for (int q = 0; q < kMaxS; ++q)
-        if (g + q < nb) bendTerm<kRhs>(I, ia[q], ib[q], ic[q], K[q], w[q], xa[q], xb[q], xc[q], fb, r);
+        if (g + q < nb) {
+          // back to stencil order a, b, c
+          const int role = br[q].role & 3;
+          const int self = (v << 1) | ((br[q].role >> 2) & 1);
+          const double xs[3] = {xv0, xv1, xv2};
+          double xa[3], xb[3], xc[3];
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            xa[d] = role == 0 ? xs[d] : p1[q][d];
+            xb[d] = role == 0 ? p1[q][d] : (role == 1 ? xs[d] : p2[q][d]);
+            xc[d] = role == 2 ? xs[d] : p2[q][d];
+          }
+          const int ea = role == 0 ? self : br[q].other1;
+          const int eb = role == 0 ? br[q].other1 : (role == 1 ? self : br[q].other2);
+          const int ec = role == 2 ? self : br[q].other2;
+          bendTerm<kRhs>(I, ea, eb, ec, K[q], (double)br[q].weight, xa, xb, xc, fb, r);
+        }
     }
   }
   f.x += fb.x;
@@ -1193,7 +1209,8 @@ __device__ void adjointBody(const InstDev& I, WP W, int nsub, int subBase) {
           const double rz = isB ? u2 - xv2 : xv2 - u2;
           const double len = sqrt(rx * rx + ry * ry + rz * rz);
           if (!(len > kMinLength)) continue;
-          const double k = I.kapE[p], l = I.restE[p];
+          const RestKappa rk = I.rkE[p];
+          const double k = rk.kap, l = rk.rest;
           const double mu0 = uP ? 0.0 : muOf(u, 0), mu1 = uP ? 0.0 : muOf(u, 1), mu2 = uP ? 0.0 : muOf(u, 2);
           const double me0 = vP ? 0.0 : mv0, me1 = vP ? 0.0 : mv1, me2 = vP ? 0.0 : mv2;
           const double ma0 = isB ? mu0 : me0, ma1 = isB ? mu1 : me1, ma2 = isB ? mu2 : me2;
