@@ -98,6 +98,9 @@ __device__ inline float subMul<float>(float acc, float l, float w) {
   return __builtin_fmaf(-l, w, acc);
 }
 
+__device__ inline double fmaT(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ inline float fmaT(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
 constexpr int kU = 8;  // entries per load group of the level sweep
 
 template <bool kFwd, typename WP, typename Store>
@@ -278,18 +281,28 @@ __device__ inline void segmentFinish(DP Xs, int w, int lane, bool act, Real& a0,
   if (maxL <= 1) return;  // single rows only (uniform over the task)
   const int base = lane - o;
   Real y0 = 0, y1 = 0, y2 = 0;
+  // terms in groups (eight FP32, four FP64): a group's broadcasts and
+  // coefficients are all issued before its sums, which then run in ascending q
+  // as one term at a time would
+  constexpr int kG = sizeof(Real) == 8 ? 4 : 8;
+  for (int g = 0; g < maxL; g += kG) {
+    Real r0[kG], r1[kG], r2[kG], c[kG];
 #pragma unroll
-  for (int q = 0; q < kCap; ++q) {
-    if (q < maxL) {
+    for (int u = 0; u < kG; ++u) {
+      const int q = g + u;
       const int src = base + min(q, L - 1);
-      const Real r0 = __shfl(a0, src), r1 = __shfl(a1, src), r2 = __shfl(a2, src);
-      if (q < L) {
-        const Real c = (L == 1) ? Real(1) : (act ? Real(Xs[q * L + o]) : Real(0));
-        y0 = y0 + c * r0;
-        y1 = y1 + c * r1;
-        y2 = y2 + c * r2;
-      }
+      r0[u] = __shfl(a0, src);
+      r1[u] = __shfl(a1, src);
+      r2[u] = __shfl(a2, src);
+      c[u] = (q < L && L > 1 && act) ? Real(Xs[q * L + o]) : Real(L == 1 ? 1 : 0);
     }
+#pragma unroll
+    for (int u = 0; u < kG; ++u)
+      if (g + u < L) {
+        y0 = y0 + c[u] * r0[u];
+        y1 = y1 + c[u] * r1[u];
+        y2 = y2 + c[u] * r2[u];
+      }
   }
   a0 = y0;
   a1 = y1;
@@ -347,7 +360,9 @@ __device__ inline void loadRun(const void* colv, const Real* val, int r, RunRaw<
 
 template <typename Real, bool kIdx16, typename WP, typename PP>
 __device__ inline void runPartial(const RunRaw<Real, kIdx16>& b, WP W, PP P) {
-  Real p0 = 0, p1 = 0, p2 = 0;
+  // fused multiply-adds into two accumulators (even and odd entries), joined at
+  // the end: a fixed order, half the dependent chain of one accumulator
+  Real p0[2] = {0, 0}, p1[2] = {0, 0}, p2[2] = {0, 0};
 #pragma unroll
   for (int q = 0; q < kRun; ++q) {
     int c;
@@ -358,13 +373,13 @@ __device__ inline void runPartial(const RunRaw<Real, kIdx16>& b, WP W, PP P) {
       c = b.c[q >> 2][q & 3];
     }
     const Real l = b.v[q];
-    p0 = p0 + l * W[3 * c + 0];
-    p1 = p1 + l * W[3 * c + 1];
-    p2 = p2 + l * W[3 * c + 2];
+    p0[q & 1] = fmaT(l, (Real)W[3 * c + 0], p0[q & 1]);
+    p1[q & 1] = fmaT(l, (Real)W[3 * c + 1], p1[q & 1]);
+    p2[q & 1] = fmaT(l, (Real)W[3 * c + 2], p2[q & 1]);
   }
-  P[0] = p0;
-  P[1] = p1;
-  P[2] = p2;
+  P[0] = p0[0] + p0[1];
+  P[1] = p1[0] + p1[1];
+  P[2] = p2[0] + p2[1];
 }
 
 constexpr int kPreRuns = 2;  // runs per thread loaded during the previous step
@@ -1452,9 +1467,6 @@ __device__ void gridSweep(cg::grid_group& grid, const BlockSweepDev& sw, PDX_GLO
     grid.sync();
   }
 }
-
-__device__ inline double fmaT(double a, double b, double c) { return __builtin_fma(a, b, c); }
-__device__ inline float fmaT(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 
 constexpr int kDR = 8;  // rows per wavefront pass of the dense product (4: FP64 config 2 9.32 ms vs 8.72)
 
