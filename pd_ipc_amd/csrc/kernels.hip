@@ -494,6 +494,9 @@ __device__ PDX_INLINE void blockSweep(const BlockSweepDev& sw, WP W, PDX_LDS Rea
     barrier();
     const long long ts2 = tick();
     int slot = 0;
+    v4i nm = m1;  // lane metadata of the wavefront's next task past the two prefetched
+    Real ndv = 0;
+    int ndo = 0;
     for (int t = tb + wave; t < te; t += nw, ++slot) {
       v4i mt;
       Real dv = 0;
@@ -507,9 +510,16 @@ __device__ PDX_INLINE void blockSweep(const BlockSweepDev& sw, WP W, PDX_LDS Rea
         dv = dv1;
         doff = do1;
       } else {
-        mt = gmeta[(size_t)t * kWave + lane];
-        doff = gldense[(size_t)t * kWave + lane];
-        if (!kFwd) dv = gldinv[(size_t)t * kWave + lane];
+        mt = nm;
+        dv = ndv;
+        doff = ndo;
+      }
+      // from the second task on, the next task's metadata loads while this one finishes
+      if (slot >= 1 && t + nw < te) {
+        const size_t ti = (size_t)(t + nw) * kWave + lane;
+        nm = gmeta[ti];
+        ndo = gldense[ti];
+        if (!kFwd) ndv = gldinv[ti];
       }
       const int k = mt.x;
       const int info = mt.w;
