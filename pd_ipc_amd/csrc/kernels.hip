@@ -475,14 +475,19 @@ __device__ PDX_INLINE void blockSweep(const BlockSweepDev& sw, WP W, PDX_LDS Rea
 #pragma unroll
     for (int q = 0; q < kPreRuns; ++q)
       if (havePre[q]) runPartial<Real, kIdx16>(pre[q], W, P + 3 * (tid + q * bs));
-    for (int r = base + tid + kPreRuns * bs; r < end; r += bs) {
-      RunRaw<Real, kIdx16> tmp;
-      loadRun<Real, kIdx16>(sw.runCol, rval, r, tmp);
-      runPartial<Real, kIdx16>(tmp, W, P + 3 * (r - base));
+    // runs past the prefetched ones, two at a time so their loads overlap
+    for (int r = base + tid + kPreRuns * bs; r < end; r += 2 * bs) {
+      RunRaw<Real, kIdx16> t0, t1;
+      const bool two = r + bs < end;
+      loadRun<Real, kIdx16>(sw.runCol, rval, r, t0);
+      loadRun<Real, kIdx16>(sw.runCol, rval, two ? r + bs : r, t1);
+      runPartial<Real, kIdx16>(t0, W, P + 3 * (r - base));
+      if (two) runPartial<Real, kIdx16>(t1, W, P + 3 * (r + bs - base));
     }
     if (haveD)
 #pragma unroll
       for (int q = 0; q < 4; ++q) D[4 * tid + q] = dpre[q];
+#pragma unroll 2
     for (int d = dbase + 4 * (tid + bs); d < dend; d += 4 * bs)
 #pragma unroll
       for (int q = 0; q < 4; ++q) D[d - dbase + q] = gdense[d + q];
