@@ -99,6 +99,40 @@ def cpu_baseline(cloths, cfg, perms, budget_s, threads):
     return inst, done_sub, t_tot
 
 
+def host_cores():
+    """Host threads this process may use: the affinity mask, capped by OMP_NUM_THREADS
+    when that is set (the machine's share for this job)."""
+    n = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(1, n)
+
+
+def cpu_baseline_instances(cloths, cfg, perms, nsub, workers):
+    """Instance-parallel CPU baseline: each oracle instance runs single-threaded,
+    `workers` instances at a time (a thread pool; the oracle's C calls release
+    the GIL).  Factoring is done first and not timed; the wall clock covers
+    stepping every instance `nsub` substeps.  Returns (instances, substeps, s)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from concurrent.futures import ThreadPoolExecutor
+    from oracle_ffi import Oracle
+
+    def make(k):
+        o = Oracle(cloths[k], cfg, perms[k], threads=1)
+        o.factor()
+        return o
+
+    # largest first, so that the pool's tail holds the small instances
+    order = sorted(range(len(cloths)), key=lambda k: -cloths[k].n)
+    with ThreadPoolExecutor(max_workers=workers) as pool:
+        oracles = list(pool.map(make, order))
+        t0 = time.perf_counter()
+        list(pool.map(lambda o: o.step(nsub), oracles))
+        t = time.perf_counter() - t0
+    return len(oracles), len(oracles) * nsub, t
+
+
 def measured_traffic(precision, instances, seed, solve):
     """HBM bytes per launch of the substep kernel from the committed counter runs
     (FETCH_SIZE and WRITE_SIZE, each scaled by a plain copy kernel's reading), or
@@ -449,11 +483,19 @@ def main():
             perms.append(e.factor_arrays(0)["perm3"])
             e.close()
         inst, nsub, t = cpu_baseline(sample, cfg, perms, args.cpu_seconds, threads=4)
-        out["cpu_baseline"] = {"value": round(nsub / t, 3), "unit": "cloth-substeps/s", "cores": 4,
-                               "kind": "port",
-                               "sample": f"FP64 oracle (reference CPU path restated, OpenMP 4 threads), "
-                                         f"{inst} instances of the mix x 2 substeps"}
-        log(f"cpu baseline: {nsub / t:.2f} cloth-substeps/s")
+        four = {"value": round(nsub / t, 3), "unit": "cloth-substeps/s", "cores": 4, "kind": "port",
+                "sample": f"FP64 oracle (reference CPU path restated, OpenMP 4 threads: the "
+                          f"reference's default), {inst} instances of the mix x 2 substeps"}
+        log(f"cpu baseline, 4 threads: {nsub / t:.2f} cloth-substeps/s")
+        cores = host_cores()
+        inst, nsub, t = cpu_baseline_instances(sample, cfg, perms, 8, cores)
+        par = {"value": round(nsub / t, 3), "unit": "cloth-substeps/s", "cores": cores, "kind": "port",
+               "sample": f"FP64 oracle, instance-parallel: one single-threaded instance per host "
+                         f"core ({cores} cores), {inst} instances of the mix x 8 substeps, wall clock"}
+        log(f"cpu baseline, instance-parallel on {cores} cores: {nsub / t:.2f} cloth-substeps/s")
+        # the quoted baseline is the faster of the two CPU forms
+        best, other = (par, four) if par["value"] >= four["value"] else (four, par)
+        out["cpu_baseline"] = dict(best, other=other)
     print(json.dumps(out), flush=True)
 
 
