@@ -33,10 +33,12 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def build_batch(cloths, cfg, precision, solve):
+def build_batch(cloths, cfg, precision, solve, ordering="amd"):
     e = Engine(precision, solve)
     for c in cloths:
         e.add(c, cfg)
+    if ordering != "amd":
+        e.set_ordering(ordering)
     t0 = time.perf_counter()
     e.factor()
     return e, time.perf_counter() - t0
@@ -369,6 +371,9 @@ def main():
     ap.add_argument("--instances", type=int, default=512)
     ap.add_argument("--seed", type=int, default=4)
     ap.add_argument("--solve", default="fast", choices=["exact", "fast"])
+    ap.add_argument("--ordering", default="nd", choices=["amd", "nd"],
+                    help="factor order of the batched mixed32 form (fast solve); the FP64 batch it is "
+                         "checked against keeps the reference's AMD order")
     ap.add_argument("--dropin-seconds", type=float, default=20.0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -405,7 +410,8 @@ def main():
     nverts = sum(c.n for c in cloths)
     log(f"{len(cloths)} instances, {nverts} vertices")
 
-    mixed, tf32 = build_batch(cloths, cfg, "mixed32", args.solve)
+    order = args.ordering if args.solve == "fast" else "amd"
+    mixed, tf32 = build_batch(cloths, cfg, "mixed32", args.solve, order)
     log(f"mixed32 factor+upload {tf32:.1f}s")
     tune32 = mixed.autotune(1) if not args.no_tune else None
     log(f"mixed32 autotune: {tune32}")
@@ -450,7 +456,7 @@ def main():
         "data": "synthetic (seeded 512-instance cloth mix)",
         "config": {"workload": "512-instance cloth mix, sides 16-96, 40 PD iters, h=1/600",
                    "instances": ninst, "vertices": nverts, "pd_iterations": iters,
-                   "seed": args.seed, "solve": args.solve,
+                   "seed": args.seed, "solve": args.solve, "ordering": order,
                    "schedule": None if tune32 is None else {"threads": tune32[0], "chain_cap": tune32[1]}},
         "max_pos_error_m": max_err,
         "fp64_batched": {"value": round(ninst * args.steps / wall64, 2), "unit": "cloth-substeps/s"},
@@ -458,7 +464,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": None if achieved is None else round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": None if achieved is None else round(achieved / HBM_PEAK_GBS, 5),
-                     "traffic": measured_traffic("mixed32", ninst, args.seed, args.solve)},
+                     "traffic": measured_traffic("mixed32", ninst, args.seed,
+                                                 args.solve if order == "amd" else f"{args.solve}-{order}")},
     }
     del fp64
     if not args.no_dropin:

@@ -52,6 +52,14 @@ enum {
 };
 
 enum {
+  PDX_ORDER_AMD = 0, /* the reference's fill-reducing order (Eigen AMD on the 3n pattern,
+                        EigenDirectSolver.cpp:230-262, restated); the default */
+  PDX_ORDER_ND = 1   /* nested dissection by coordinate bisection (fast solve mode only):
+                        ~25 % more factor entries, less than half the dependent sweep
+                        steps on cloth grids; results differ from AMD's by rounding */
+};
+
+enum {
   PDX_LAUNCH_BATCH = 0, /* one workgroup per instance, persistent over the substeps */
   PDX_LAUNCH_GRID = 1   /* every CU on one instance: cooperative launch, grid barrier per sweep step
                            (fast solve mode, exactly one instance) */
@@ -183,6 +191,12 @@ int pdx_get_topology(pdx_ctx* ctx, int instance, int* color_count, uint32_t* col
  * elimination-tree chain one wavefront takes as a block (1..32, fast mode).
  * Changing the chain cap re-packs the schedules (no new factorisation is counted). */
 int pdx_set_schedule(pdx_ctx* ctx, int block_threads, int chain_cap);
+
+/* Fill-reducing order of every instance's factor (PDX_ORDER_*).  No counterpart in
+ * the reference, whose order is always AMD (EigenDirectSolver.cpp:240-241).
+ * PDX_ORDER_ND needs the fast solve mode.  A change re-analyses every instance at
+ * the next pdx_factor / pdx_step (counted in analyze_count and factorize_count). */
+int pdx_set_ordering(pdx_ctx* ctx, int ordering);
 
 /* --- differentiable rollout and adjoint ---------------------------------------
  * No counterpart in the reference (it has no adjoint); these entry points are the

@@ -15,7 +15,8 @@ BENCH = os.path.join(HERE, "pdx_bench")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("PDX_ARCH", "gfx950")
 
-SOURCES = ["kernels.hip", "context.cpp", "factor.cpp", "amd.cpp", "scene.cpp", "mesh_io.cpp", "report.cpp"]
+SOURCES = ["kernels.hip", "context.cpp", "factor.cpp", "amd.cpp", "nd.cpp", "scene.cpp", "mesh_io.cpp",
+           "report.cpp"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-Wall",
           "-Wno-unused-function", "-Wno-unused-result", "-Wno-unused-value"]

@@ -127,7 +127,7 @@ struct HostBlockSweep {
   int nstep = 0;
   int maxRuns = 0;        // most runs in one step (partial-sum slots in LDS)
   int maxTaskRuns = 0;    // most runs of one task (grid form: one wavefront's LDS slice)
-  int maxDense = 0;       // most dense-block values in one step (LDS staging)
+  int maxDense = 0;       // most dense-block values in one step
   std::vector<int> stepTask, stepRun, stepDense;  // nstep+1 each
   std::vector<int> taskInfo, taskSlot, taskDense; // per task
   std::vector<int> slotRank, slotRunBeg, slotRunEnd;
@@ -420,6 +420,7 @@ struct pdx_ctx {
   size_t lds = 0;
   int block = 512;
   int chainCap = 32;
+  int ordering = PDX_ORDER_AMD;
   bool idx16 = false;  // fast-mode run columns stored as 16-bit ranks (every instance has m <= 65535)
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double lastKernelMs = 0.0;
@@ -522,8 +523,10 @@ size_t align16(size_t b) { return (b + 15) / 16 * 16; }
 size_t partBytes(const Instance& I, size_t w) {
   return align16((size_t)std::max(I.bf.maxRuns, I.bb.maxRuns) * 3 * w);
 }
+// FP64 stages a step's chain inverse blocks in LDS; FP32 loads their rows into
+// registers and stages nothing
 size_t denseBytes(const Instance& I, size_t w) {
-  return align16((size_t)std::max(I.bf.maxDense, I.bb.maxDense) * w);
+  return w == 8 ? align16((size_t)std::max(I.bf.maxDense, I.bb.maxDense) * w) : 0;
 }
 size_t stepBytes(const Instance& I) {
   return align16((size_t)3 * (I.bf.nstep + 1 + I.bb.nstep + 1) * sizeof(int));
@@ -551,6 +554,11 @@ int refreshFactors(pdx_ctx* c, bool& changed) {
       return fail(c, PDX_EUNSUPPORTED, "the dense solve mode takes at most " + std::to_string(kDenseMaxM) + " vertices");
   }
   changed = true;
+  // W (3m Real) sits in LDS only when every instance's fits beside some staging;
+  // otherwise it stays in HBM for all of them and the steps keep the default caps
+  int maxM = 0;
+  for (const auto& I : c->inst) maxM = std::max(maxM, I.scene.n);
+  const bool wCanFit = kRedBytes + (size_t)3 * maxM * (c->prec == PDX_PREC_MIXED32 ? 4 : 8) + 16 * 1024 <= kLdsLimit;
   std::vector<int> rcs(todo.size(), 0);
   std::atomic<size_t> next{0};
   auto worker = [&]() {
@@ -573,10 +581,26 @@ int refreshFactors(pdx_ctx* c, bool& changed) {
             packBlocked(I.fac, true, c->chainCap, c->waveRuns, big, big, I.bf);
             packBlocked(I.fac, false, c->chainCap, c->waveRuns, big, big, I.bb);
           } else {
-            // per-step LDS caps: run partials <= 48 KB, dense chain blocks <= 32 KB
-            const int runCap = 48 * 1024 / (3 * w), denseCap = 32 * 1024 / w;
+            // per-step LDS caps: run partials <= 48 KB; FP64 stages the chain
+            // blocks' inverses too (<= 32 KB), FP32 loads them into registers
+            const bool stage = w == 8;
+            int runCap = 48 * 1024 / (3 * w), denseCap = stage ? 32 * 1024 / w : 1 << 29;
             packBlocked(I.fac, true, c->chainCap, runCap, runCap, denseCap, I.bf);
             packBlocked(I.fac, false, c->chainCap, runCap, runCap, denseCap, I.bb);
+            // The workgroup also holds W (3m).  When W and the staging overflow
+            // LDS, the steps are cut finer to the room W leaves (3:2 between run
+            // partials and staged blocks), shrinking by the overflow until they fit.
+            const size_t wBytes = (size_t)3 * I.fac.m * w;
+            auto need = [&]() { return kRedBytes + wBytes + partBytes(I, w) + denseBytes(I, w) + stepBytes(I); };
+            size_t room = kLdsLimit > kRedBytes + wBytes + stepBytes(I) ? kLdsLimit - kRedBytes - wBytes - stepBytes(I) : 0;
+            for (int tries = 0; wCanFit && need() > kLdsLimit && room >= 8 * 1024 && tries < 8; ++tries) {
+              runCap = std::min(runCap, (int)((stage ? room * 3 / 5 : room) / (3 * w)));
+              if (stage) denseCap = std::min(denseCap, (int)(room * 2 / 5 / w));
+              packBlocked(I.fac, true, c->chainCap, runCap, runCap, denseCap, I.bf);
+              packBlocked(I.fac, false, c->chainCap, runCap, runCap, denseCap, I.bb);
+              const size_t over = need() > kLdsLimit ? need() - kLdsLimit : 0;
+              room = room > over + 1024 ? room - over - 1024 : 0;
+            }
           }
         } else {
           const bool longRows = c->prec == PDX_PREC_FP64;
@@ -600,7 +624,7 @@ int refreshFactors(pdx_ctx* c, bool& changed) {
       const uint64_t ss = structureStamp(I.scene);
       int rc = 0;
       if (!I.analyzed || ss != I.strStamp) {
-        rc = analyzeAndFactor(I.scene, I.A, I.fac);
+        rc = analyzeAndFactor(I.scene, I.A, I.fac, c->ordering == PDX_ORDER_ND ? kOrderNd : kOrderAmd);
         if (rc == 0) {
           I.analyzed = true;
           I.strStamp = ss;
@@ -645,13 +669,16 @@ int upload(pdx_ctx* c) {
   const size_t w = fp32 ? 4 : 8;
   c->idx16 = true;
   for (auto& I : c->inst) c->idx16 = c->idx16 && I.fac.m <= 65535;
-  size_t maxW = 0, maxExtra = 0;
+  // every workgroup lays out its own instance: W (3m), then the run partials,
+  // the dense blocks and the step tables, so the launch needs the largest sum
+  size_t maxExtra = 0, maxBoth = 0;
   for (auto& I : c->inst) {
-    maxW = std::max(maxW, (size_t)3 * I.fac.m * w);
-    if (c->mode == PDX_SOLVE_FAST) maxExtra = std::max(maxExtra, partBytes(I, w) + denseBytes(I, w) + stepBytes(I));
+    const size_t e = c->mode == PDX_SOLVE_FAST ? partBytes(I, w) + denseBytes(I, w) + stepBytes(I) : 0;
+    maxExtra = std::max(maxExtra, e);
+    maxBoth = std::max(maxBoth, (size_t)3 * I.fac.m * w + e);
   }
-  c->ldsW = kRedBytes + maxW + maxExtra <= kLdsLimit;
-  c->lds = kRedBytes + (c->ldsW ? maxW : 0) + maxExtra;
+  c->ldsW = kRedBytes + maxBoth <= kLdsLimit;
+  c->lds = kRedBytes + (c->ldsW ? maxBoth : maxExtra);
   if (c->launchMode == 1) {
     for (auto& I : c->inst)
       if (I.cfg.chebRho > 0.0) return fail(c, PDX_EUNSUPPORTED, "the grid form has no Chebyshev blend");
@@ -1795,6 +1822,24 @@ int pdx_set_schedule(pdx_ctx* c, int block_threads, int chain_cap) {
     c->dirty = true;
   }
   c->block = block_threads;
+  return PDX_OK;
+}
+
+int pdx_set_ordering(pdx_ctx* c, int ordering) {
+  if (!c || (ordering != PDX_ORDER_AMD && ordering != PDX_ORDER_ND)) return PDX_EINVAL;
+  if (ordering == PDX_ORDER_ND && c->mode != PDX_SOLVE_FAST)
+    return fail(c, PDX_EUNSUPPORTED, "the nested-dissection order needs the fast solve mode");
+  if (ordering != c->ordering) {
+    const int rc = pullState(c);
+    if (rc) return rc;
+    c->ordering = ordering;
+    for (auto& I : c->inst) {
+      I.analyzed = false;  // new symbolic analysis and factor
+      I.factored = false;
+      I.packed = false;
+    }
+    c->dirty = true;
+  }
   return PDX_OK;
 }
 

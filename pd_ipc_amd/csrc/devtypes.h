@@ -71,7 +71,8 @@ struct BlockSweepDev {
 struct InstDev {
   int n, E, B, m;
   int maxIter, wantResidual, hasBendRhs, ldsW;
-  int partOff, denseOff;  // byte offsets of the run partials and dense staging in LDS
+  int partOff, denseOff;  // byte offsets of the run partials and (FP64) the staged
+                          // inverse blocks of a step in LDS
   int stepOff, pad2;      // byte offset of the step tables of both sweeps in LDS
   double h, invH, invH2, dampInvH;
   double gh2[3];

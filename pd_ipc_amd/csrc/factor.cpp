@@ -446,18 +446,28 @@ void buildSweeps(Factor& f) {
 // (L = A (x) I3, so the pattern is A's, replicated on dofs 3v+d), pivot order split
 // into the three x/y/z components (:106-155), numeric LDL^T of one component and the
 // shared compact factor (:157-225), then the sweep level schedules.
-int analyzeAndFactor(const Scene& s, const Matrix& A, Factor& f) {
+int analyzeAndFactor(const Scene& s, const Matrix& A, Factor& f, int ordering) {
   const int n = s.n;
   const int n3 = 3 * n;
-  std::vector<int> cp((size_t)n3 + 1, 0), ri;
-  ri.reserve((size_t)3 * A.rows.size());
-  for (int vc = 0; vc < n; ++vc)
-    for (int d = 0; d < 3; ++d) {
-      const int J = 3 * vc + d;
-      for (int p = A.colptr[(size_t)vc]; p < A.colptr[(size_t)vc + 1]; ++p) ri.push_back(3 * A.rows[(size_t)p] + d);
-      cp[(size_t)J + 1] = (int)ri.size();
-    }
-  amdOrder(n3, cp, ri, f.perm3);
+  if (ordering == kOrderNd) {
+    // the three components of a vertex share its place in the vertex order
+    std::vector<int> vo;
+    ndOrder(A, s.x, kNdLeaf, vo);
+    f.perm3.resize((size_t)n3);
+    for (int k = 0; k < n; ++k)
+      for (int d = 0; d < 3; ++d) f.perm3[(size_t)3 * k + d] = 3 * vo[(size_t)k] + d;
+  } else {
+    std::vector<int> cp((size_t)n3 + 1, 0), ri;
+    ri.reserve((size_t)3 * A.rows.size());
+    for (int vc = 0; vc < n; ++vc)
+      for (int d = 0; d < 3; ++d) {
+        const int J = 3 * vc + d;
+        for (int p = A.colptr[(size_t)vc]; p < A.colptr[(size_t)vc + 1]; ++p)
+          ri.push_back(3 * A.rows[(size_t)p] + d);
+        cp[(size_t)J + 1] = (int)ri.size();
+      }
+    amdOrder(n3, cp, ri, f.perm3);
+  }
   f.m = n;
   for (int c = 0; c < 3; ++c) {
     f.rankVertex[c].clear();

@@ -50,6 +50,9 @@ void bendingRhs(const Scene& s, std::vector<double>& out);
 
 void amdOrder(int n, const std::vector<int>& colptr, const std::vector<int>& rowidx,
               std::vector<int>& perm);
+// Nested-dissection vertex order of A's graph by coordinate bisection of the
+// positions x (3n); parts of at most `leaf` vertices stay in sorted order.
+void ndOrder(const Matrix& A, const std::vector<double>& x, int leaf, std::vector<int>& order);
 
 // Level-scheduled sweep of a unit triangular factor (rank space).
 struct Sweep {
@@ -77,8 +80,12 @@ struct Factor {
   Sweep fwd, bwd;
 };
 
+// Fill-reducing orders of analyzeAndFactor.
+enum { kOrderAmd = 0, kOrderNd = 1 };
+constexpr int kNdLeaf = 16;
+
 // Returns 0 or a negative status (-3 = not positive definite).
-int analyzeAndFactor(const Scene& s, const Matrix& A, Factor& f);
+int analyzeAndFactor(const Scene& s, const Matrix& A, Factor& f, int ordering = kOrderAmd);
 int refactorNumeric(const Matrix& A, Factor& f);
 
 // Dense inverse of the component matrix in rank space, m x m, row r = the solve

@@ -382,25 +382,83 @@ __device__ inline void runPartial(const RunRaw<Real, kIdx16>& b, WP W, PP P) {
   P[2] = p2[0] + p2[1];
 }
 
-constexpr int kPreRuns = 2;  // runs per thread loaded during the previous step
+constexpr int kPreRuns = 2;    // runs per thread loaded during the previous step
+constexpr int kMaxChain = 32;  // longest chain block (the schedule's chain cap is at most 32)
+
+// A lane's row of its segment's inverse block, X(o, q) for q < the task's longest
+// segment, loaded from the step's dense range into registers.  Lanes without a
+// segment of their own (single rows, lanes past the task) load the range's first
+// value instead, which exists whenever the task holds a chain; their values are
+// not used.  Tasks without chains load nothing.
+template <typename Real>
+__device__ inline void loadChainRow(const PDX_GLOBAL Real* Xs, v4i m, int doff, Real (&x)[kMaxChain]) {
+  const int w = m.w;
+  const int L = w & 0x7f, o = (w >> 7) & 0x7f;
+  const int maxL = __builtin_amdgcn_readfirstlane((w >> 21) & 0x7f);
+  if (maxL <= 1) return;
+  const bool own = m.x >= 0 && L > 1;
+  const PDX_GLOBAL Real* p = Xs + (own ? doff + o : 0);
+  const int stride = own ? L : 0;
+  const int last = own ? L - 1 : 0;
+#pragma unroll
+  for (int q = 0; q < kMaxChain; ++q)
+    if (q < maxL) x[q] = p[min(q, last) * stride];
+}
+
+// segmentFinish with the lane's inverse-block row in registers (loadChainRow).
+template <typename Real>
+__device__ inline void segmentFinishR(const Real (&x)[kMaxChain], int w, int lane, Real& a0, Real& a1, Real& a2) {
+  const int L = w & 0x7f, o = (w >> 7) & 0x7f;
+  const int maxL = __builtin_amdgcn_readfirstlane((w >> 21) & 0x7f);
+  if (maxL <= 1) return;  // single rows only
+  const int base = lane - o;
+  Real y0 = 0, y1 = 0, y2 = 0;
+  constexpr int kG = sizeof(Real) == 8 ? 4 : 8;
+#pragma unroll
+  for (int g = 0; g < kMaxChain; g += kG) {
+    if (g < maxL) {
+      Real r0[kG], r1[kG], r2[kG];
+#pragma unroll
+      for (int u = 0; u < kG; ++u) {
+        const int src = base + min(g + u, L - 1);
+        r0[u] = __shfl(a0, src);
+        r1[u] = __shfl(a1, src);
+        r2[u] = __shfl(a2, src);
+      }
+#pragma unroll
+      for (int u = 0; u < kG; ++u)
+        if (g + u < L) {
+          const Real c = L > 1 ? x[g + u] : Real(1);
+          y0 = y0 + c * r0[u];
+          y1 = y1 + c * r1[u];
+          y2 = y2 + c * r2[u];
+        }
+    }
+  }
+  a0 = y0;
+  a1 = y1;
+  a2 = y2;
+}
 
 // Block-scheduled sweep.  Step st: (1) the workgroup turns the step's runs into
-// partial sums in LDS (each thread's first kPreRuns runs, the first two tasks'
-// lane metadata of its wavefront and the step's dense blocks were loaded during
-// the previous step) and copies the dense chain blocks into LDS; (2) each
+// partial sums in LDS (each thread's first kPreRuns runs and the first two tasks'
+// lane metadata of its wavefront were loaded during the previous step); (2) each
 // wavefront takes whole tasks: a row subtracts its partials in run order, a chain
-// block then applies its dense triangle column by column, broadcasting each
-// finished value with readlane.  Two barriers per step; fixed order, no float
-// atomics.  `steps` is the step table in LDS: run, dense and task offsets.
+// block then applies its segment's inverse block column by column, broadcasting
+// each finished value with a shuffle.  Lane metadata loads two tasks ahead.  Two
+// barriers per step; fixed order, no float atomics.  `steps` is the step table in
+// LDS: run, dense and task offsets.
 struct SweepTimes {
   long long stream = 0, wait1 = 0, fin = 0, wait2 = 0;
 };
 
 template <typename Real, bool kFwd, bool kIdx16, typename WP, typename Store>
 __device__ PDX_INLINE void blockSweep(const BlockSweepDev& sw, WP W, PDX_LDS Real* P, PDX_LDS Real* D,
-                           const PDX_LDS int* steps, const Real* dinvUnused, Store store, SweepTimes& tstream,
-                           bool wInLds) {
-  (void)dinvUnused;
+                                      const PDX_LDS int* steps, Store store, SweepTimes& tstream, bool wInLds) {
+  // FP32: each lane's inverse-block row goes from HBM to registers, one task
+  // ahead.  FP64 (twice the registers per row): the step's blocks are staged in
+  // LDS (D) during the run phase, as loaded during the previous step.
+  constexpr bool kRegRows = sizeof(Real) == 4;
   auto barrier = [wInLds]() {
     if (wInLds)
       ldsBarrier();
@@ -417,9 +475,6 @@ __device__ PDX_INLINE void blockSweep(const BlockSweepDev& sw, WP W, PDX_LDS Rea
   const PDX_LDS int* sDense = steps + (ns + 1);
   const PDX_LDS int* sTask = steps + 2 * (ns + 1);
   const Real* rval = static_cast<const Real*>(sw.runVal);
-  const Real* dense = static_cast<const Real*>(sw.dense);
-  const v4i* meta = reinterpret_cast<const v4i*>(sw.laneMeta);
-  const Real* ldinv = static_cast<const Real*>(sw.laneDinv);
   RunRaw<Real, kIdx16> pre[kPreRuns];
   bool havePre[kPreRuns];
   Real dpre[4] = {0, 0, 0, 0};
@@ -427,19 +482,24 @@ __device__ PDX_INLINE void blockSweep(const BlockSweepDev& sw, WP W, PDX_LDS Rea
   v4i mpre[2];
   Real dvpre[2] = {0, 0};
   int dopre[2] = {0, 0};
-  bool haveM[2] = {false, false};
   // Every thread issues its loads unconditionally (indices clamped to valid
   // entries) so that no branch or register copy waits for them before use.
   const int lastRun = max(sRun[ns] - 1, 0);
   const int lastDense = max(sDense[ns] - 4, 0);
   const int lastTask = max(sTask[ns] - 1, 0);
-  const PDX_GLOBAL Real* gdense = gview<Real>(dense);
-  const PDX_GLOBAL v4i* gmeta = gview<v4i>(meta);
-  const PDX_GLOBAL Real* gldinv = gview<Real>(ldinv);
+  const PDX_GLOBAL Real* gdense = gview<Real>(sw.dense);
+  const PDX_GLOBAL v4i* gmeta = gview<v4i>(sw.laneMeta);
+  const PDX_GLOBAL Real* gldinv = gview<Real>(sw.laneDinv);
   const PDX_GLOBAL int* gldense = gview<int>(sw.laneDense);
-  // Lanes past the step's end load the step's first item instead: all such
-  // lanes of a wavefront then hit one address and coalesce into one request,
-  // rather than fetching unrelated runs.
+  auto loadMeta = [&](int t, v4i& m, int& doff, Real& dv) {
+    const size_t ti = (size_t)min(t, lastTask) * kWave + lane;
+    m = gmeta[ti];
+    doff = gldense[ti];
+    if (!kFwd) dv = gldinv[ti];
+  };
+  // Lanes past the step's end load the step's first item instead: all such lanes
+  // of a wavefront then hit one address and coalesce into one request, rather
+  // than fetching unrelated runs.
   auto prefetch = [&](int st) {
     const int rb = sRun[st], re = sRun[st + 1];
     const int rfirst = min(rb, lastRun);
@@ -449,24 +509,28 @@ __device__ PDX_INLINE void blockSweep(const BlockSweepDev& sw, WP W, PDX_LDS Rea
       havePre[q] = r < re;
       loadRun<Real, kIdx16>(sw.runCol, rval, havePre[q] ? r : rfirst, pre[q]);
     }
-    const int db = sDense[st];
-    const int d = db + 4 * tid;
-    haveD = d < sDense[st + 1];
-    const int dc = haveD ? d : min(db, lastDense);
+    if constexpr (!kRegRows) {
+      const int db = sDense[st];
+      const int d = db + 4 * tid;
+      haveD = d < sDense[st + 1];
+      const int dc = haveD ? d : min(db, lastDense);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) dpre[q] = gdense[dc + q];
-    const int tb = sTask[st];
+      for (int q = 0; q < 4; ++q) dpre[q] = gdense[dc + q];
+    }
+    const int tb = sTask[st], te = sTask[st + 1];
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
       const int t = tb + wave + q * nw;
-      haveM[q] = t < sTask[st + 1];
-      const size_t ti = (size_t)(haveM[q] ? t : min(tb, lastTask)) * kWave + lane;
-      mpre[q] = gmeta[ti];
-      dopre[q] = gldense[ti];
-      if (!kFwd) dvpre[q] = gldinv[ti];
+      loadMeta(t < te ? t : tb, mpre[q], dopre[q], dvpre[q]);
     }
   };
+  Real xc[kRegRows ? kMaxChain : 1], xn[kRegRows ? kMaxChain : 1];
+  (void)xc;
+  (void)xn;
   prefetch(0);
+  if constexpr (kRegRows) {
+    if (sTask[0] + wave < sTask[1]) loadChainRow<Real>(gdense + sDense[0], mpre[0], dopre[0], xc);
+  }
   for (int st = 0; st < ns; ++st) {
     const long long ts0 = tick();
     const int base = sRun[st], end = sRun[st + 1];
@@ -484,48 +548,31 @@ __device__ PDX_INLINE void blockSweep(const BlockSweepDev& sw, WP W, PDX_LDS Rea
       runPartial<Real, kIdx16>(t0, W, P + 3 * (r - base));
       if (two) runPartial<Real, kIdx16>(t1, W, P + 3 * (r + bs - base));
     }
-    if (haveD)
+    if constexpr (!kRegRows) {
+      if (haveD)
 #pragma unroll
-      for (int q = 0; q < 4; ++q) D[4 * tid + q] = dpre[q];
+        for (int q = 0; q < 4; ++q) D[4 * tid + q] = dpre[q];
 #pragma unroll 2
-    for (int d = dbase + 4 * (tid + bs); d < dend; d += 4 * bs)
+      for (int d = dbase + 4 * (tid + bs); d < dend; d += 4 * bs)
 #pragma unroll
-      for (int q = 0; q < 4; ++q) D[d - dbase + q] = gdense[d + q];
-    const v4i m0 = mpre[0], m1 = mpre[1];
-    const Real dv0 = dvpre[0], dv1 = dvpre[1];
-    const int do0 = dopre[0], do1 = dopre[1];
+        for (int q = 0; q < 4; ++q) D[d - dbase + q] = gdense[d + q];
+    }
+    v4i mt = mpre[0], mn = mpre[1], mnn = mpre[1];
+    Real dv = dvpre[0], dvn = dvpre[1], dvnn = 0;
+    int doff = dopre[0], don = dopre[1], donn = 0;
     if (st + 1 < ns) prefetch(st + 1);
     const long long ts1 = tick();
     barrier();
     const long long ts2 = tick();
-    int slot = 0;
-    v4i nm = m1;  // lane metadata of the wavefront's next task past the two prefetched
-    Real ndv = 0;
-    int ndo = 0;
-    for (int t = tb + wave; t < te; t += nw, ++slot) {
-      v4i mt;
-      Real dv = 0;
-      int doff = 0;
-      if (slot == 0) {
-        mt = m0;
-        dv = dv0;
-        doff = do0;
-      } else if (slot == 1) {
-        mt = m1;
-        dv = dv1;
-        doff = do1;
-      } else {
-        mt = nm;
-        dv = ndv;
-        doff = ndo;
+    const PDX_GLOBAL Real* Xs = gdense + dbase;
+    (void)Xs;
+    for (int t = tb + wave; t < te; t += nw) {
+      // the next task's inverse-block rows (its metadata is here) and the
+      // metadata of the task after it load while this task finishes
+      if constexpr (kRegRows) {
+        if (t + nw < te) loadChainRow<Real>(Xs, mn, don, xn);
       }
-      // from the second task on, the next task's metadata loads while this one finishes
-      if (slot >= 1 && t + nw < te) {
-        const size_t ti = (size_t)(t + nw) * kWave + lane;
-        nm = gmeta[ti];
-        ndo = gldense[ti];
-        if (!kFwd) ndv = gldinv[ti];
-      }
+      if (t + 2 * nw < te) loadMeta(t + 2 * nw, mnn, donn, dvnn);
       const int k = mt.x;
       const int info = mt.w;
       const bool act = k >= 0;
@@ -541,13 +588,31 @@ __device__ PDX_INLINE void blockSweep(const BlockSweepDev& sw, WP W, PDX_LDS Rea
         }
         subtractPartials<Real>(P, mt.y, mt.z, a0, a1, a2);
       }
-      segmentFinish<Real>(D + doff, info, lane, act, a0, a1, a2);
+      if constexpr (kRegRows)
+        segmentFinishR<Real>(xc, info, lane, a0, a1, a2);
+      else
+        segmentFinish<Real>(D + doff, info, lane, act, a0, a1, a2);
       if (act) {
         W[3 * k + 0] = a0;
         W[3 * k + 1] = a1;
         W[3 * k + 2] = a2;
         store(k, a0, a1, a2);
       }
+      mt = mn;
+      dv = dvn;
+      doff = don;
+      mn = mnn;
+      dvn = dvnn;
+      don = donn;
+      if constexpr (kRegRows) {
+#pragma unroll
+        for (int q = 0; q < kMaxChain; ++q) xc[q] = xn[q];
+      }
+    }
+    if constexpr (kRegRows) {
+      // the next step's first task: its inverse-block rows load across the barrier
+      if (st + 1 < ns && sTask[st + 1] + wave < sTask[st + 2])
+        loadChainRow<Real>(gdense + sDense[st + 1], mpre[0], dopre[0], xc);
     }
     const long long ts3 = tick();
     barrier();
@@ -844,7 +909,7 @@ __device__ PDX_INLINE void runForward(const InstDev& I, WP W, SweepTimes& tst) {
   const Real* dinv = static_cast<const Real*>(I.dinv);
   auto none = [](int, Real, Real, Real) {};
   if (kBlocked)
-    blockSweep<Real, true, kIdx16>(I.bfwd, W, ldsAt<Real>(I.partOff), ldsAt<Real>(I.denseOff), ldsAt<int>(I.stepOff), dinv, none, tst, I.ldsW != 0);
+    blockSweep<Real, true, kIdx16>(I.bfwd, W, ldsAt<Real>(I.partOff), ldsAt<Real>(I.denseOff), ldsAt<int>(I.stepOff), none, tst, I.ldsW != 0);
   else
     sweep<Real, true>(I.fwd, W, dinv, none);
 }
@@ -870,7 +935,7 @@ __device__ PDX_INLINE void runBackward(const InstDev& I, WP W, SweepTimes& tst) 
   (void)pinned;
   (void)pinpos;
   if (kBlocked)
-    blockSweep<Real, false, kIdx16>(I.bbwd, W, ldsAt<Real>(I.partOff), ldsAt<Real>(I.denseOff), ldsAt<int>(I.stepOff) + 3 * (I.bfwd.nstep + 1), dinv, put, tst, I.ldsW != 0);
+    blockSweep<Real, false, kIdx16>(I.bbwd, W, ldsAt<Real>(I.partOff), ldsAt<Real>(I.denseOff), ldsAt<int>(I.stepOff) + 3 * (I.bfwd.nstep + 1), put, tst, I.ldsW != 0);
   else
     sweep<Real, false>(I.bwd, W, dinv, put);
 }
@@ -1207,10 +1272,9 @@ __device__ void adjointBody(const InstDev& I, WP W, int nsub, int subBase) {
       __syncthreads();
       if (kBlocked) {
         blockSweep<Real, true, kIdx16>(I.bfwd, W, ldsAt<Real>(I.partOff), ldsAt<Real>(I.denseOff),
-                                       ldsAt<int>(I.stepOff), dinv, none, swt, I.ldsW != 0);
+                                       ldsAt<int>(I.stepOff), none, swt, I.ldsW != 0);
         blockSweep<Real, false, kIdx16>(I.bbwd, W, ldsAt<Real>(I.partOff), ldsAt<Real>(I.denseOff),
-                                        ldsAt<int>(I.stepOff) + 3 * (I.bfwd.nstep + 1), dinv, none, swt,
-                                        I.ldsW != 0);
+                                        ldsAt<int>(I.stepOff) + 3 * (I.bfwd.nstep + 1), none, swt, I.ldsW != 0);
       } else {
         sweep<Real, true>(I.fwd, W, dinv, none);
         sweep<Real, false>(I.bwd, W, dinv, none);
@@ -1825,7 +1889,7 @@ __device__ void solveBody(const InstDev& I, WP W, const double* __restrict__ b, 
   SweepTimes tst;
   auto none = [](int, Real, Real, Real) {};
   if (kBlocked)
-    blockSweep<Real, true, kIdx16>(I.bfwd, W, ldsAt<Real>(I.partOff), ldsAt<Real>(I.denseOff), ldsAt<int>(I.stepOff), dinv, none, tst, I.ldsW != 0);
+    blockSweep<Real, true, kIdx16>(I.bfwd, W, ldsAt<Real>(I.partOff), ldsAt<Real>(I.denseOff), ldsAt<int>(I.stepOff), none, tst, I.ldsW != 0);
   else
     sweep<Real, true>(I.fwd, W, dinv, none);
   const int* rankv = I.rankv;
@@ -1836,7 +1900,7 @@ __device__ void solveBody(const InstDev& I, WP W, const double* __restrict__ b, 
     xout[3 * v + 2] = (double)a2;
   };
   if (kBlocked)
-    blockSweep<Real, false, kIdx16>(I.bbwd, W, ldsAt<Real>(I.partOff), ldsAt<Real>(I.denseOff), ldsAt<int>(I.stepOff) + 3 * (I.bfwd.nstep + 1), dinv, put, tst, I.ldsW != 0);
+    blockSweep<Real, false, kIdx16>(I.bbwd, W, ldsAt<Real>(I.partOff), ldsAt<Real>(I.denseOff), ldsAt<int>(I.stepOff) + 3 * (I.bfwd.nstep + 1), put, tst, I.ldsW != 0);
   else
     sweep<Real, false>(I.bwd, W, dinv, put);
 }

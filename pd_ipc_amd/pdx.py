@@ -22,6 +22,8 @@ PDX_PREC_MIXED32 = 1
 PDX_SOLVE_EXACT = 0
 PDX_SOLVE_FAST = 1
 PDX_SOLVE_DENSE = 2
+PDX_ORDER_AMD = 0
+PDX_ORDER_ND = 1
 
 _STATUS = {0: "ok", -1: "invalid argument", -2: "not ready", -3: "not positive definite",
            -4: "HIP runtime error", -5: "unsupported", -6: "out of memory"}
@@ -122,6 +124,7 @@ def load_library(path=LIB_PATH):
         "pdx_set_launch": (I, [P, I, I]),
         "pdx_get_schedule_stats": (I, [P, I, P]),
         "pdx_set_schedule": (I, [P, I, I]),
+        "pdx_set_ordering": (I, [P, I]),
         "pdx_bytes_per_iteration": (D, [P]),
         "pdx_bytes_per_substep": (D, [P]),
         "pdx_grid_counts": (I, [I, I, I, I, C.POINTER(I), C.POINTER(I), C.POINTER(I)]),
@@ -436,6 +439,12 @@ class Engine:
 
     def set_schedule(self, block_threads, chain_cap=32):
         self._check(self.lib.pdx_set_schedule(self.h, block_threads, chain_cap), "set_schedule")
+
+    def set_ordering(self, ordering):
+        """Fill-reducing order of the factors: 'amd' (the reference's, default) or
+        'nd' (nested dissection: fewer dependent sweep steps; fast mode only)."""
+        code = {"amd": PDX_ORDER_AMD, "nd": PDX_ORDER_ND}[ordering]
+        self._check(self.lib.pdx_set_ordering(self.h, code), "set_ordering")
 
     def set_launch(self, mode, grid_blocks=0):
         """mode: 'batch' (one workgroup per instance) or 'grid' (the whole GPU on

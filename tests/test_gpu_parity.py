@@ -17,9 +17,11 @@ def _say(*a):
     print("[case]", *a, flush=True)
 
 
-def _pair(cloth, cfg, precision="fp64", solve="exact"):
+def _pair(cloth, cfg, precision="fp64", solve="exact", ordering="amd"):
     e = Engine(precision, solve)
     e.add(cloth, cfg)
+    if ordering != "amd":
+        e.set_ordering(ordering)
     e.factor()
     o = Oracle(cloth, cfg, e.factor_arrays(0)["perm3"])
     return e, o
@@ -88,27 +90,29 @@ def test_mixed32_error_bound():
     assert err <= 1e-5
 
 
+@pytest.mark.parametrize("ordering", ["amd", "nd"])
 @pytest.mark.parametrize("nx,ny,shear,bend", [(16, 16, 0.0, 0.0), (33, 21, 500.0, 50.0), (60, 60, 500.0, 50.0)])
-def test_solve3_fast_tolerance(nx, ny, shear, bend):
+def test_solve3_fast_tolerance(nx, ny, shear, bend, ordering):
     """Chain-blocked sweeps reorder the backward sums: equal to the oracle's solve
-    to rounding."""
-    _say("solve3 fast", nx, ny, shear, bend)
+    (same factor order) to rounding."""
+    _say("solve3 fast", nx, ny, shear, bend, ordering)
     c = scenes.grid_cloth(nx, ny, 2300.0, shear=shear, bend=bend)
     cfg = SimConfig.fixed_iterations(10, scenes.H600)
-    e, o = _pair(c, cfg, solve="fast")
+    e, o = _pair(c, cfg, solve="fast", ordering=ordering)
     b = np.random.default_rng(11).standard_normal(3 * c.n)
     xg = e.solve(0, b)
     xo = o.solve(b)
     assert np.abs(xg - xo).max() <= 1e-12 * np.abs(xo).max()
 
 
+@pytest.mark.parametrize("ordering", ["amd", "nd"])
 @pytest.mark.parametrize("nx,ny,shear,bend,iters,nsub", [(40, 40, 0.0, 0.0, 10, 60), (30, 22, 500.0, 50.0, 40, 30)])
-def test_step_fast_within_1e10(nx, ny, shear, bend, iters, nsub):
+def test_step_fast_within_1e10(nx, ny, shear, bend, iters, nsub, ordering):
     """The fast FP64 drop-in stays within 1e-10 relative of the reference path."""
-    _say("step fast", nx, ny, shear, bend, iters, nsub)
+    _say("step fast", nx, ny, shear, bend, iters, nsub, ordering)
     c = scenes.grid_cloth(nx, ny, 2300.0, shear=shear, bend=bend)
     cfg = SimConfig.fixed_iterations(iters, scenes.H600)
-    e, o = _pair(c, cfg, solve="fast")
+    e, o = _pair(c, cfg, solve="fast", ordering=ordering)
     e.step(nsub)
     o.step(nsub)
     xg, _ = e.state(0)
@@ -118,11 +122,12 @@ def test_step_fast_within_1e10(nx, ny, shear, bend, iters, nsub):
     assert rel <= 1e-10
 
 
-def test_mixed32_fast_error_bound():
-    _say("mixed32 fast vs oracle 48x40 shear+bend")
+@pytest.mark.parametrize("ordering", ["amd", "nd"])
+def test_mixed32_fast_error_bound(ordering):
+    _say("mixed32 fast vs oracle 48x40 shear+bend", ordering)
     c = scenes.grid_cloth(48, 40, 1000.0, shear=200.0, bend=20.0, pins=[0, 47])
     cfg = SimConfig.fixed_iterations(40, scenes.H600)
-    e, o = _pair(c, cfg, precision="mixed32", solve="fast")
+    e, o = _pair(c, cfg, precision="mixed32", solve="fast", ordering=ordering)
     e.step(50)
     o.step(50)
     xg, _ = e.state(0)
@@ -258,3 +263,51 @@ def test_bench60_published_chebyshev_on_gpu():
     e.step(300)
     assert f"{e.stats(0)['last_residual']:.6g}" == "9.37677"
     e.close()
+
+
+def test_nd_order_against_reference_order():
+    """The nested-dissection order changes only rounding: the fast FP64 path with
+    it stays within 1e-10 relative of the oracle run in the reference's AMD order."""
+    _say("step fast nd vs oracle amd 36x29 shear+bend")
+    c = scenes.grid_cloth(36, 29, 2300.0, shear=500.0, bend=50.0)
+    cfg = SimConfig.fixed_iterations(40, scenes.H600)
+    e, _ = _pair(c, cfg, solve="fast", ordering="nd")
+    ref = Engine("fp64", "exact")
+    ref.add(c, cfg)
+    ref.factor()
+    o = Oracle(c, cfg, ref.factor_arrays(0)["perm3"])
+    e.step(30)
+    o.step(30)
+    xg, _ = e.state(0)
+    xo = o.positions()
+    rel = np.abs(xg - xo).max() / np.abs(xo).max()
+    print("relative position difference", rel)
+    assert rel <= 1e-10
+
+
+def test_nd_batch_mixed32_against_fp64():
+    """A batch of the 512-instance mix (sizes 16-96 a side) in the nested-dissection
+    order, mixed32, against the FP64 batch in the reference's order: every cloth
+    within 1e-5 m after 20 substeps, and the nested-dissection schedule has fewer
+    sweep steps than AMD's on the largest cloth."""
+    cloths, cfg, _ = scenes.config4(seed=4, count=512)
+    pick = sorted(range(len(cloths)), key=lambda i: cloths[i].n)
+    sample = [cloths[i] for i in pick[:: len(pick) // 7]] + [cloths[pick[-1]]]
+    _say("nd batch", [c.n for c in sample])
+    e32 = Engine("mixed32", "fast")
+    e64 = Engine("fp64", "fast")
+    for c in sample:
+        e32.add(c, cfg)
+        e64.add(c, cfg)
+    e32.set_ordering("nd")
+    e32.factor()
+    e64.factor()
+    last = len(sample) - 1
+    assert e32.schedule_stats(last)["fwd"]["steps"] < e64.schedule_stats(last)["fwd"]["steps"]
+    e32.step(20)
+    e64.step(20)
+    worst = 0.0
+    for i in range(len(sample)):
+        worst = max(worst, float(np.abs(e32.state(i)[0] - e64.state(i)[0]).max()))
+    print("max position error", worst)
+    assert worst <= 1e-5
