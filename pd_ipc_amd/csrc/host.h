@@ -82,7 +82,8 @@ struct Factor {
 
 // Fill-reducing orders of analyzeAndFactor.
 enum { kOrderAmd = 0, kOrderNd = 1 };
-constexpr int kNdLeaf = 16;
+constexpr int kNdLeaf = 8;  // 512-instance mix, mixed32: leaves of 2/4/8/16/32/64 gave
+                            // 29.7/29.8/29.6/30.0/30.8/31.8 ms per launch
 
 // Returns 0 or a negative status (-3 = not positive definite).
 int analyzeAndFactor(const Scene& s, const Matrix& A, Factor& f, int ordering = kOrderAmd);

@@ -19,9 +19,11 @@ def _cpu_reference(c, cfg, perm, nsub, traj, seed=9):
     return xT, target, r.adjoint(traj)
 
 
-def _gpu(c, cfg, nsub, every, traj, target, precision, solve):
+def _gpu(c, cfg, nsub, every, traj, target, precision, solve, ordering="amd"):
     e = Engine(precision, solve)
     e.add(c, cfg)
+    if ordering != "amd":
+        e.set_ordering(ordering)
     e.set_pin_trajectory(0, traj)
     e.rollout(nsub, every)
     xT, vT = e.state(0)
@@ -56,15 +58,17 @@ def test_adjoint_exact_fp64_matches_cpu_adjoint():
         assert err <= 1e-12, k
 
 
-@pytest.mark.parametrize("precision,solve,tol", [("fp64", "fast", 1e-9), ("mixed32", "fast", 2e-4),
-                                                 ("mixed32", "exact", 2e-4)])
-def test_adjoint_forms_match_cpu_adjoint(precision, solve, tol):
+@pytest.mark.parametrize("precision,solve,tol,ordering", [
+    ("fp64", "fast", 1e-9, "amd"), ("mixed32", "fast", 2e-4, "amd"), ("mixed32", "exact", 2e-4, "amd"),
+    # the nested-dissection order against the CPU adjoint in the reference's AMD order
+    ("fp64", "fast", 1e-9, "nd"), ("mixed32", "fast", 2e-4, "nd")])
+def test_adjoint_forms_match_cpu_adjoint(precision, solve, tol, ordering):
     c, cfg, perm = _scene(nx=14, ny=11, iters=8)
     nsub = 7
     traj = _traj(c, nsub, cfg.dt)
     xT_cpu, target, gc = _cpu_reference(c, cfg, perm, nsub, traj)
-    xT, g = _gpu(c, cfg, nsub, 3, traj, target, precision, solve)
-    print(precision, solve, "max |xT - xT_fp64| =", float(np.max(np.abs(xT - xT_cpu))))
+    xT, g = _gpu(c, cfg, nsub, 3, traj, target, precision, solve, ordering)
+    print(precision, solve, ordering, "max |xT - xT_fp64| =", float(np.max(np.abs(xT - xT_cpu))))
     for k in KEYS:
         err = _rel(g[k], gc[k])
         print(k, "max rel err vs FP64 CPU adjoint", err)
