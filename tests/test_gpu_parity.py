@@ -311,3 +311,24 @@ def test_nd_batch_mixed32_against_fp64():
         worst = max(worst, float(np.abs(e32.state(i)[0] - e64.state(i)[0]).max()))
     print("max position error", worst)
     assert worst <= 1e-5
+
+
+def test_nd_order_closed_surface_against_reference_order(tmp_path):
+    """A closed 3-D triangle surface (OBJ) in the nested-dissection order, fast
+    FP64, against the oracle in the reference's AMD order: 1e-10 relative."""
+    from test_nd_order import _sphere_obj
+    c = scenes.obj_cloth(_sphere_obj(tmp_path), 2300.0, 1.0, pins=(0,))
+    _say("sphere nd vs oracle amd", c.n)
+    cfg = SimConfig.fixed_iterations(20, scenes.H600)
+    e, _ = _pair(c, cfg, solve="fast", ordering="nd")
+    ref = Engine("fp64", "exact")
+    ref.add(c, cfg)
+    ref.factor()
+    o = Oracle(c, cfg, ref.factor_arrays(0)["perm3"])
+    e.step(25)
+    o.step(25)
+    xg, _ = e.state(0)
+    xo = o.positions()
+    rel = np.abs(xg - xo).max() / np.abs(xo).max()
+    print("relative position difference", rel)
+    assert rel <= 1e-10
