@@ -312,18 +312,21 @@ def dropin_workload(args):
 
 def large_workload(args):
     """512x512 cloth (262,144 vertices, distance + shear + bend, four corners
-    pinned), 40 PD iterations: FP64 in the one-workgroup form (W in HBM) and
-    FP32-solve / FP64-residual in the cooperative grid form (every CU on the one
-    cloth), max position error of the FP32 form against FP64 after the run."""
+    pinned), 40 PD iterations: FP64 in the one-workgroup form (W in HBM, the
+    reference's AMD order) and FP32-solve / FP64-residual in the cooperative grid
+    form (every CU on the one cloth, --ordering: nested dissection by default),
+    max position error of the FP32 form against FP64 after the run."""
     c, cfg, nsub = scenes.config3()
     nsub = min(nsub, args.large_substeps)
     res = {"vertices": c.n, "substeps": nsub, "pd_iterations": cfg.max_iterations}
     xs = {}
-    for prec, mode, g in (("fp64", "batch", 0), ("mixed32", "grid", 32)):
+    for prec, mode, g, order in (("fp64", "batch", 0, "amd"), ("mixed32", "grid", 32, args.ordering)):
         e = Engine(prec, "fast")
         e.add(c, cfg)
         if mode == "grid":
             e.set_launch("grid", g)
+        if order != "amd":
+            e.set_ordering(order)
         t0 = time.perf_counter()
         e.factor()
         tf = time.perf_counter() - t0
@@ -333,7 +336,7 @@ def large_workload(args):
         xs[prec] = e.state(0)[0]
         st = e.stats(0)
         bps = e.bytes_per_substep()
-        res[prec] = {"form": mode if mode == "batch" else f"grid ({g} workgroups)",
+        res[prec] = {"form": mode if mode == "batch" else f"grid ({g} workgroups)", "ordering": order,
                      "ms_per_substep": round(1e3 * dt / nsub, 2),
                      "cloth_substeps_per_s": round(nsub / dt, 4),
                      "algorithmic_GB_per_substep": round(bps / 1e9, 3),
