@@ -157,11 +157,13 @@ def _set_kappa(e, i, c, scale):
         e.set_bend_k(i, c.bend_k * scale)
 
 
-def _rollout_batch(cloths, cfg, nsub, every, precision, trajs, kappas=None, v0s=None):
+def _rollout_batch(cloths, cfg, nsub, every, precision, trajs, kappas=None, v0s=None, ordering="amd"):
     e = Engine(precision, "fast")
     for i, c in enumerate(cloths):
         e.add(c, cfg)
         e.set_pin_trajectory(i, trajs[i])
+    if ordering != "amd":
+        e.set_ordering(ordering)
     for i, c in enumerate(cloths):
         if kappas is not None:
             _set_kappa(e, i, c, kappas[i] / c.meta["kappa"])
@@ -178,21 +180,23 @@ def adjoint_workload(args):
     seeded hidden stiffness and pin trajectories; gradients of sum |x_T - target|^2
     w.r.t. per-instance kappa, v0 and the pin trajectory are taken at the starting
     guess (kappa 2300, v0 = 0, static pins) in mixed precision (FP32 solves) and in
-    FP64, and checked against central differences of the FP64 forward."""
+    FP64, and checked against central differences of the FP64 forward.  Every run
+    factors in the --ordering order (nested dissection by default)."""
     cloths, cfg, nsub, hidden = scenes.config5(count=args.adj_instances)
     every = 10
     ni = len(cloths)
     log(f"adjoint workload: {ni} instances x {cloths[0].n} vertices, {nsub} substeps")
     t0 = time.perf_counter()
+    order = args.ordering
     eh = _rollout_batch(cloths, cfg, nsub, every, "fp64", [h["traj"] for h in hidden],
-                        kappas=[h["kappa"] for h in hidden])
+                        kappas=[h["kappa"] for h in hidden], ordering=order)
     targets = [eh.state(i)[0] for i in range(ni)]
     eh.close()
     log(f"targets {time.perf_counter() - t0:.1f}s")
     static = [scenes.pin_trajectory(c, nsub, cfg.dt, amplitude=0.0) for c in cloths]
 
     def grads(precision):
-        e = _rollout_batch(cloths, cfg, nsub, every, precision, static)
+        e = _rollout_batch(cloths, cfg, nsub, every, precision, static, ordering=order)
         fwd_ms, _ = e.rollout_timing()
         loss = 0.0
         xs = []
@@ -219,7 +223,7 @@ def adjoint_workload(args):
 
     def loss64(kappas=None, v0s=None, trajs=None):
         e = _rollout_batch(cloths, cfg, nsub, every, "fp64", trajs if trajs is not None else static,
-                           kappas=kappas, v0s=v0s)
+                           kappas=kappas, v0s=v0s, ordering=order)
         val = sum(float(np.sum((e.state(i)[0] - targets[i]) ** 2)) for i in range(ni))
         e.close()
         return val
@@ -247,6 +251,7 @@ def adjoint_workload(args):
     out = {
         "workload": f"differentiable batch: {ni} x {cloths[0].n}-vertex cloths (128x128, distance+shear+bend), "
                     f"{nsub} substeps, {cfg.max_iterations} PD iters, checkpoints every {every}",
+        "ordering": order,
         "forward_ms_mixed32": round(g32["fwd_ms"], 2), "adjoint_ms_mixed32": round(g32["adj_ms"], 2),
         "forward_ms_fp64": round(g64["fwd_ms"], 2), "adjoint_ms_fp64": round(g64["adj_ms"], 2),
         "gradient_cloth_substeps_per_s_mixed32": round(sub_total / ((g32["fwd_ms"] + g32["adj_ms"]) * 1e-3), 2),
