@@ -332,3 +332,23 @@ def test_nd_order_closed_surface_against_reference_order(tmp_path):
     rel = np.abs(xg - xo).max() / np.abs(xo).max()
     print("relative position difference", rel)
     assert rel <= 1e-10
+
+
+@pytest.mark.parametrize("ordering", ["amd", "nd"])
+def test_mixed32_w_in_hbm_against_oracle(ordering):
+    """A cloth whose FP32 W (3n) does not fit in LDS (120 x 120: 169 KB), so the
+    batch form keeps W in HBM: mixed32 fast within 1e-5 m of the oracle (in the
+    reference's AMD order) after 5 substeps."""
+    _say("mixed32 W in HBM 120x120", ordering)
+    c = scenes.grid_cloth(120, 120, 2300.0, shear=500.0, bend=50.0)
+    cfg = SimConfig.fixed_iterations(40, scenes.H600)
+    e, _ = _pair(c, cfg, precision="mixed32", solve="fast", ordering=ordering)
+    ref = Engine("fp64", "exact")
+    ref.add(c, cfg)
+    ref.factor()
+    o = Oracle(c, cfg, ref.factor_arrays(0)["perm3"])
+    e.step(5)
+    o.step(5)
+    err = np.abs(e.state(0)[0] - o.positions()).max()
+    print("max position error", err)
+    assert err <= 1e-5
