@@ -25,6 +25,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 from pd_ipc_amd import Engine, scenes  # noqa: E402
+from pd_ipc_amd.pdx import device_info  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 
@@ -468,6 +469,7 @@ def main():
                    "schedule": None if tune32 is None else {"threads": tune32[0], "chain_cap": tune32[1]}},
         "max_pos_error_m": max_err,
         "fp64_batched": {"value": round(ninst * args.steps / wall64, 2), "unit": "cloth-substeps/s"},
+        "device": device_info(0),
         "kernel_ms_per_launch": round(kms32, 4),
         "roofline": {"bound": "hbm", "achieved": None if achieved is None else round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -248,6 +248,10 @@ int pdx_autotune(pdx_ctx* ctx, int probe_substeps, int* best_block, int* best_ca
 /* Average duration (ms) of the substep kernel over the last `pdx_step*` call,
  * measured with HIP events on the stream the kernel runs on. */
 double pdx_last_kernel_ms(const pdx_ctx* ctx);
+/* The device's own figures (no counterpart in the reference; for reports):
+ * compute units, peak shader clock (kHz), peak memory clock (kHz) and memory bus
+ * width (bits), as the HIP runtime states them.  Any pointer may be NULL. */
+int pdx_device_info(int device, int* compute_units, int* clock_khz, int* mem_clock_khz, int* bus_width_bits);
 /* Algorithmic bytes of one PD iteration (factor both sweeps, maps, gather of
  * edges/incidence for the right-hand side and the residual, vector passes),
  * summed over instances; each array counted once per phase at storage width. */

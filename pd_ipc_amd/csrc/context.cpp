@@ -1955,6 +1955,21 @@ int pdx_autotune(pdx_ctx* c, int probe_substeps, int* best_block, int* best_cap,
 }
 
 double pdx_last_kernel_ms(const pdx_ctx* c) { return c ? c->lastKernelMs : 0.0; }
+
+int pdx_device_info(int device, int* cus, int* clock_khz, int* mem_clock_khz, int* bus_bits) {
+  const struct {
+    hipDeviceAttribute_t a;
+    int* out;
+  } q[4] = {{hipDeviceAttributeMultiprocessorCount, cus},
+            {hipDeviceAttributeClockRate, clock_khz},
+            {hipDeviceAttributeMemoryClockRate, mem_clock_khz},
+            {hipDeviceAttributeMemoryBusWidth, bus_bits}};
+  for (const auto& e : q) {
+    if (!e.out) continue;
+    if (hipDeviceGetAttribute(e.out, e.a, device) != hipSuccess) return PDX_EHIP;
+  }
+  return PDX_OK;
+}
 double pdx_bytes_per_iteration(const pdx_ctx* c) { return c ? c->bytesIter : 0.0; }
 
 double pdx_bytes_per_substep(const pdx_ctx* c) {

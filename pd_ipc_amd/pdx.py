@@ -120,6 +120,7 @@ def load_library(path=LIB_PATH):
         "pdx_get_gradients": (I, [P, I, P, P, P, P, P]),
         "pdx_rollout_timing": (I, [P, C.POINTER(D), C.POINTER(D)]),
         "pdx_last_kernel_ms": (D, [P]),
+        "pdx_device_info": (I, [I, P, P, P, P]),
         "pdx_autotune": (I, [P, I, C.POINTER(I), C.POINTER(I), C.POINTER(D)]),
         "pdx_set_launch": (I, [P, I, I]),
         "pdx_get_schedule_stats": (I, [P, I, P]),
@@ -147,6 +148,18 @@ def load_library(path=LIB_PATH):
 
 def _ptr(a):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def device_info(device=0):
+    """The device's compute units, shader clock (kHz), memory clock (kHz) and
+    memory bus width (bits) as the HIP runtime states them."""
+    lib = load_library()
+    v = [C.c_int() for _ in range(4)]
+    rc = lib.pdx_device_info(device, *[C.byref(x) for x in v])
+    if rc != 0:
+        raise PdxError(rc, "device_info")
+    return dict(compute_units=v[0].value, clock_khz=v[1].value, mem_clock_khz=v[2].value,
+                bus_width_bits=v[3].value)
 
 
 @dataclass
