@@ -352,3 +352,12 @@ def test_mixed32_w_in_hbm_against_oracle(ordering):
     err = np.abs(e.state(0)[0] - o.positions()).max()
     print("max position error", err)
     assert err <= 1e-5
+
+
+def test_device_info_reports_the_device():
+    from pd_ipc_amd.pdx import device_info
+    _say("device info")
+    d = device_info(0)
+    print(d)
+    assert d["compute_units"] > 0 and d["clock_khz"] > 0
+    assert d["mem_clock_khz"] > 0 and d["bus_width_bits"] >= 64
