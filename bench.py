@@ -270,7 +270,9 @@ def dropin_workload(args):
     """Plain FP64 drop-in on the reference-method configurations, one cloth per
     context, against the FP64 oracle (the reference CPU path restated) on the
     same inputs: 40x40 distance only (10 iters x 600 substeps) and 100x100 with
-    shear and bend, sigma 1e-3 m position noise (40 iters x 300 substeps)."""
+    shear and bend, sigma 1e-3 m position noise (40 iters x 300 substeps).  Forms:
+    dense, fast (AMD order), fast_nd (nested-dissection order), exact; the oracle
+    runs in the reference's AMD order."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_ffi import Oracle
     out = {}
@@ -278,14 +280,20 @@ def dropin_workload(args):
         if args.dropin_substeps:
             nsub = min(nsub, args.dropin_substeps)
         res = {"substeps": nsub, "pd_iterations": cfg.max_iterations}
-        for solve in ("dense", "fast", "exact"):
-            e = Engine("fp64", solve)
+        amd_perm = None
+        for solve in ("dense", "fast", "fast_nd", "exact"):
+            e = Engine("fp64", "fast" if solve == "fast_nd" else solve)
             e.add(c, cfg)
+            if solve == "fast_nd":
+                e.set_ordering("nd")
             t0 = time.perf_counter()
             e.factor()
             tf = time.perf_counter() - t0
             tune = e.autotune(2) if (solve != "exact" and not args.no_tune) else None
-            perm = e.factor_arrays(0)["perm3"]
+            # the oracle runs in the reference's AMD order for every form
+            perm = e.factor_arrays(0)["perm3"] if solve != "fast_nd" else amd_perm
+            if amd_perm is None:
+                amd_perm = perm
             ns = nsub if solve != "exact" else min(nsub, 600 if c.n < 5000 else 2)
             t0 = time.perf_counter()
             e.step(ns)
