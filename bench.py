@@ -136,20 +136,79 @@ def cpu_baseline_instances(cloths, cfg, perms, nsub, workers):
     return len(oracles), len(oracles) * nsub, t
 
 
+def kernel_source_hash():
+    """sha256 (16 hex digits) of the sources that decide the substep kernel's memory
+    traffic: the kernels, the device descriptors and the schedule packer."""
+    import hashlib
+    h = hashlib.sha256()
+    for name in ("kernels.hip", "devtypes.h", "context.cpp"):
+        with open(os.path.join(ROOT, "pd_ipc_amd", "csrc", name), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
 def measured_traffic(precision, instances, seed, solve):
     """HBM bytes per launch of the substep kernel from the committed counter runs
-    (FETCH_SIZE and WRITE_SIZE, each scaled by a plain copy kernel's reading), or
-    None when no run of this exact workload is on record."""
+    (FETCH_SIZE and WRITE_SIZE, each scaled by a plain copy kernel's reading), with
+    their provenance.  A record counts only when it was measured on the same kernel
+    sources (same hash): otherwise bytes is None and the record's hash is returned."""
     path = os.path.join(ROOT, "profiles", "substep_kernel_512mix_hbm_traffic.md")
+    src = kernel_source_hash()
     try:
         lines = open(path).read().splitlines()
     except OSError:
-        return None
+        return None, {"source": None, "kernel_source_sha": src, "status": "no counter record"}
     key = f"traffic_bytes_per_launch {precision} {instances} {seed} {solve} "
     for ln in lines:
         if ln.startswith("machine-readable: " + key):
-            return float(ln.split()[-1])
-    return None
+            parts = ln.split()
+            rec_hash, val = parts[-2], float(parts[-1])
+            prov = {"source": "profiles/substep_kernel_512mix_hbm_traffic.md (rocprofv3 --pmc FETCH_SIZE, "
+                              "WRITE_SIZE, copy-calibrated)", "kernel_source_sha": src,
+                    "record_kernel_source_sha": rec_hash}
+            if rec_hash != src:
+                prov["status"] = "stale: the kernel sources changed since the counter run"
+                return None, prov
+            prov["status"] = "measured on these kernel sources"
+            return val, prov
+    return None, {"source": None, "kernel_source_sha": src, "status": "no counter record for this workload"}
+
+
+def accuracy_run(mixed, fp64, cloths, cfg, nsub, sample):
+    """Untimed: both batches restart from the initial state and run the workload's
+    `nsub` substeps; every instance's max |x32 - x64| (mixed32 in its order against
+    the FP64 batch in the reference's AMD order), and the sampled instances against
+    the FP64 CPU oracle run on host threads meanwhile."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from concurrent.futures import ThreadPoolExecutor
+    from oracle_ffi import Oracle
+    for i, c in enumerate(cloths):
+        mixed.set_state(i, c.positions, c.velocities)
+        fp64.set_state(i, c.positions, c.velocities)
+    perms = {}
+    for i in sample:
+        e = Engine("fp64")
+        e.add(cloths[i], cfg)
+        perms[i] = e.factor_arrays(0)["perm3"]
+        e.close()
+
+    def oracle(i):
+        o = Oracle(cloths[i], cfg, perms[i], threads=max(1, host_cores() // len(sample)))
+        o.step(nsub)
+        return o.positions()
+
+    with ThreadPoolExecutor(max_workers=len(sample)) as pool:
+        fut = {i: pool.submit(oracle, i) for i in sample}
+        mixed.step_graph(1, nsub)
+        fp64.step_graph(1, nsub)
+        errs = np.array([float(np.abs(mixed.state(i)[0] - fp64.state(i)[0]).max()) for i in range(len(cloths))])
+        orc = {}
+        for i in sample:
+            xo = fut[i].result()
+            orc[str(i)] = {"vertices": cloths[i].n,
+                           "fp64_batch_max_rel_diff": float(np.abs(fp64.state(i)[0] - xo).max() / np.abs(xo).max()),
+                           "mixed32_max_pos_error_m": float(np.abs(mixed.state(i)[0] - xo).max())}
+    return errs, orc
 
 
 def _set_kappa(e, i, c, scale):
@@ -199,6 +258,15 @@ def adjoint_workload(args):
     def grads(precision):
         e = _rollout_batch(cloths, cfg, nsub, every, precision, static, ordering=order)
         fwd_ms, _ = e.rollout_timing()
+        # algorithmic bytes of the reverse pass (every checkpoint segment re-run with the
+        # iterate tape, then the adjoint kernel): per substep, the forward substep plus
+        # its tape writes ((K+1) x 24n), and K adjoint iterations, each the transposed
+        # solve and Jacobian gather (one forward iteration's arrays) plus the tape reads
+        # of x^k and x^{k+1} (2 x 24n), plus the prediction / velocity adjoints (27n x 8)
+        n_tot = sum(c.n for c in cloths)
+        K = cfg.max_iterations
+        rev_bytes = nsub * (e.bytes_per_substep() + (K + 1) * 24.0 * n_tot
+                            + K * (e.bytes_per_iteration() + 48.0 * n_tot) + 216.0 * n_tot)
         loss = 0.0
         xs = []
         for i in range(ni):
@@ -212,7 +280,7 @@ def adjoint_workload(args):
         e.close()
         gk = np.array([gi["kappa"] @ (c.edge_kappa / c.meta["kappa"]) + gi["bend"] @ (c.bend_k / c.meta["kappa"])
                        for gi, c in zip(g, cloths)])
-        return dict(loss=loss, xs=xs, fwd_ms=fwd_ms, adj_ms=adj_ms, gk=gk,
+        return dict(loss=loss, xs=xs, fwd_ms=fwd_ms, adj_ms=adj_ms, gk=gk, rev_bytes=rev_bytes,
                     gv=np.stack([gi["v0"] for gi in g]), gp=np.stack([gi["pin"] for gi in g]))
 
     g32 = grads("mixed32")
@@ -262,6 +330,11 @@ def adjoint_workload(args):
         "central_difference_rel_err_fp64": {k: abs(v[0] - v[1]) / abs(v[0]) for k, v in fd.items()},
         "central_difference_rel_err_mixed32": {k: abs(v[0] - v[2]) / abs(v[0]) for k, v in fd.items()},
         "fd_steps": {"v0": eps, "pins": eps, "kappa_relative": ks},
+        "roofline": {"kernel": "reverse pass, mixed32 (segment re-runs with the iterate tape + adjointKernel)",
+                     "bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBS,
+                     "achieved": round(g32["rev_bytes"] / (g32["adj_ms"] * 1e-3) / 1e9, 2),
+                     "frac": round(g32["rev_bytes"] / (g32["adj_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
+                     "algorithmic_GB": round(g32["rev_bytes"] / 1e9, 3), "traffic": None},
     }
     return out
 
@@ -400,6 +473,8 @@ def main():
     ap.add_argument("--dropin-substeps", type=int, default=0, help="cap on the drop-in configs' substeps")
     ap.add_argument("--large-substeps", type=int, default=10)
     ap.add_argument("--adj-instances", type=int, default=64)
+    ap.add_argument("--error-substeps", type=int, default=200,
+                    help="substeps of the untimed accuracy run (the workload's length)")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
@@ -443,13 +518,12 @@ def main():
         f"{mixed.bytes_per_substep() / 1e6:.1f} MB algorithmic per launch)")
     wall64, kms64 = time_batch(fp64, args.steps, args.warmup)
     log(f"fp64 batched: {args.steps} steps in {wall64:.3f}s (kernel {kms64:.2f} ms per launch)")
-    errs = []
-    for i in range(len(cloths)):
-        x32, _ = mixed.state(i)
-        x64, _ = fp64.state(i)
-        errs.append(float(np.abs(x32 - x64).max()))
-    max_err = max(errs)
-    log(f"max |x32 - x64| over instances: {max_err:.3e} m")
+    # accuracy over the workload's full length, outside the timed region
+    sizes = [c.n for c in cloths]
+    sample = sorted({int(np.argmin(sizes)), int(np.argmax(sizes))})
+    errs, orc = accuracy_run(mixed, fp64, cloths, cfg, args.error_substeps, sample)
+    max_err = float(errs.max())
+    log(f"max |x32 - x64| over instances after {args.error_substeps} substeps: {max_err:.3e} m; oracle: {orc}")
 
     ninst = len(cloths)
     value = ninst * args.steps / wall32
@@ -457,6 +531,8 @@ def main():
     iters = cfg.max_iterations
     bytes_launch = mixed.bytes_per_substep()  # one launch = one substep of the batch
     achieved = bytes_launch / (kms32 * 1e-3) / 1e9 if kms32 > 0 else None
+    traffic, traffic_prov = measured_traffic("mixed32", ninst, args.seed,
+                                             args.solve if order == "amd" else f"{args.solve}-{order}")
 
     out = {
         "metric": "cloth substeps/sec (PD iters fixed), batched FP32 vs plain FP64 at <=1e-5 m error",
@@ -476,14 +552,21 @@ def main():
                    "seed": args.seed, "solve": args.solve, "ordering": order,
                    "schedule": None if tune32 is None else {"threads": tune32[0], "chain_cap": tune32[1]}},
         "max_pos_error_m": max_err,
+        "max_pos_error_basis": f"max over all {len(cloths)} instances of max |x_mixed32 - x_fp64| after "
+                               f"{args.error_substeps} substeps from the initial state (untimed run): mixed32 "
+                               f"in the {order.upper()} order against the FP64 batch in the reference's AMD order",
+        "error_bound_m": 1e-5,
+        "error_bound_met": bool(max_err <= 1e-5),
+        "oracle_check": {"substeps": args.error_substeps, "instances": orc,
+                         "oracle": "FP64 CPU oracle (reference path restated), AMD order"},
         "fp64_batched": {"value": round(ninst * args.steps / wall64, 2), "unit": "cloth-substeps/s"},
         "device": device_info(0),
         "kernel_ms_per_launch": round(kms32, 4),
         "roofline": {"bound": "hbm", "achieved": None if achieved is None else round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": None if achieved is None else round(achieved / HBM_PEAK_GBS, 5),
-                     "traffic": measured_traffic("mixed32", ninst, args.seed,
-                                                 args.solve if order == "amd" else f"{args.solve}-{order}")},
+                     "traffic": traffic},
+        "traffic_provenance": traffic_prov,
     }
     del fp64
     if not args.no_dropin:
@@ -522,6 +605,9 @@ def main():
         best, other = (par, four) if par["value"] >= four["value"] else (four, par)
         out["cpu_baseline"] = dict(best, other=other)
     print(json.dumps(out), flush=True)
+    if max_err > 1e-5:
+        log(f"the FP32 form exceeds the 1e-5 m error bound ({max_err:.3e} m)")
+        sys.exit(3)
 
 
 if __name__ == "__main__":

@@ -138,7 +138,9 @@ int pdx_destroy(pdx_ctx* ctx);
 const char* pdx_last_error(const pdx_ctx* ctx);
 
 /* Adds an instance (topology upload: Scene.cpp makeScene's products).  Returns
- * its index through *instance.  Must precede the first step. */
+ * its index through *instance.  An instance added after the context has stepped
+ * is analysed, factored and uploaded with the next step (the other instances keep
+ * their state, factors and counters). */
 int pdx_add_instance(pdx_ctx* ctx, const pdx_scene* scene, const pdx_config* cfg, int* instance);
 int pdx_instance_count(const pdx_ctx* ctx);
 
@@ -165,6 +167,20 @@ int pdx_set_pin_positions(pdx_ctx* ctx, int instance, const double* pin_position
 int pdx_set_edge_kappa(pdx_ctx* ctx, int instance, const double* kappa);
 /* Changes bend stencil stiffness (BendStencil::stiffness); refactors once likewise. */
 int pdx_set_bend_k(pdx_ctx* ctx, int instance, const double* k);
+/* Changes the lumped masses (Mesh::masses, n values >= 0).  Masses enter
+ * L = M/h^2 + ..., so the next step factors once more, without a new analysis
+ * (the numeric stamp changes, the structure stamp does not: Assembler.cpp:29-92). */
+int pdx_set_masses(pdx_ctx* ctx, int instance, const double* masses);
+/* Changes the pin mask (Mesh::pinned, n bytes, nonzero = pinned).  The mask
+ * decides which rows of L are overwritten and which coupling blocks exist, so
+ * both stamps change and the next step analyses and factors again
+ * (Assembler.cpp:29-92; tests/chain/spring_vertical.cpp:366-371, :528-563).
+ * A newly pinned vertex is held at its pin position (pdx_set_pin_positions to
+ * move it); a pin trajectory set for the instance is cleared when the pin count
+ * changes.  An unchanged mask is a no-op. */
+int pdx_set_pinned(pdx_ctx* ctx, int instance, const uint8_t* pinned);
+/* Changes the substep configuration.  Only a dt change refactors (stamp); an
+ * unchanged configuration is a no-op. */
 int pdx_set_config(pdx_ctx* ctx, int instance, const pdx_config* cfg);
 int pdx_get_stats(pdx_ctx* ctx, int instance, pdx_stats* out);
 

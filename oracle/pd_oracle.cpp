@@ -53,7 +53,7 @@ uint64_t bitsOf(double v) {
   std::memcpy(&u, &v, 8);
   return u;
 }
-// Assembler.cpp:104-107
+// Assembler.cpp:22-25
 uint64_t mix(uint64_t h, uint64_t v) { return h ^ (v + 0x9e3779b97f4a7c15ULL + (h << 6) + (h >> 2)); }
 
 }  // namespace
@@ -100,6 +100,9 @@ struct orc_ctx {
   std::vector<V3> predicted, positionsBeforeStep, positionsBeforePrevStep;
   std::vector<double> b, bBase, xSolution, bendRhs;
   int iterationsUsed = 0, earlyExitCount = 0;
+  // when set, stepOnce records every iterate x^{k+1} (xSolution after iteration k)
+  // at iterTape + k * 3n: the adjoint's tape (no counterpart in the reference)
+  double* iterTape = nullptr;
   bool converged = false;
   double lastResidual = -1.0;
   std::vector<double> forceBuf;
@@ -236,7 +239,7 @@ void buildColoring(orc_ctx& c) {
   }
 }
 
-// Assembler.cpp:111-147 (numeric stamp; damping excluded from the comparison)
+// Assembler.cpp:29-65 (numeric stamp; damping excluded from the comparison)
 void computeStamp(const orc_ctx& c, uint64_t& stiff, uint64_t& mass, uint64_t& dt,
                   uint64_t& pin) {
   uint64_t sh = 0xcbf29ce484222325ULL, mh = 0xcbf29ce484222325ULL, ph = 0xcbf29ce484222325ULL;
@@ -250,7 +253,7 @@ void computeStamp(const orc_ctx& c, uint64_t& stiff, uint64_t& mass, uint64_t& d
   pin = ph;
 }
 
-// Assembler.cpp:149-174
+// Assembler.cpp:67-92
 uint64_t computeStructureStamp(const orc_ctx& c) {
   uint64_t h = 0xcbf29ce484222325ULL;
   h = mix(h, 1);
@@ -270,7 +273,7 @@ uint64_t computeStructureStamp(const orc_ctx& c) {
   return h;
 }
 
-// Assembler.cpp:176-219
+// Assembler.cpp:94-137
 void assembleBendingRhs(const orc_ctx& c, std::vector<double>& out) {
   out.assign((size_t)3 * c.n, 0.0);
   if (c.B == 0) return;
@@ -303,7 +306,7 @@ struct Trip {
   double v;
 };
 
-// Assembler.cpp:221-340 + DistanceTerm.cpp:436-466, one scalar component; the
+// Assembler.cpp:139-278 + DistanceTerm.cpp:319-349, one scalar component; the
 // per-entry sums follow triplet insertion order (Eigen setFromTriplets).
 void assembleScalar(orc_ctx& c) {
   const int n = c.n;
@@ -341,7 +344,7 @@ void assembleScalar(orc_ctx& c) {
       if (c.isPinned(c.ea[e]) || c.isPinned(c.eb[e])) continue;
       if (c.ea[e] == v || c.eb[e] == v) existing += c.kappa[e];
     }
-    // bendDiagonalAt(v) is 0 for a pinned vertex (Assembler.cpp:309-320).
+    // bendDiagonalAt(v) is 0 for a pinned vertex (Assembler.cpp:227-245).
     t.push_back({v, v, 1.0 - existing});
   }
   // collapse duplicates in insertion order, then CSC by column with rows ascending
@@ -685,7 +688,7 @@ double nonlinearResidualSerial(orc_ctx& c) {
   return worst;
 }
 
-// DistanceTerm.cpp:211-219 projectEdgeValue + :238-269 scatterEdgeValue
+// DistanceTerm.cpp:94-102 projectEdgeValue + :121-152 scatterEdgeValue
 inline void projectScatterEdge(const orc_ctx& c, int e, double* out) {
   const int a = c.ea[(size_t)e], b = c.eb[(size_t)e];
   const V3 raw = c.positions[(size_t)a] - c.positions[(size_t)b];
@@ -775,7 +778,7 @@ int stepOnce(orc_ctx& c) {
     }
     double chebOmega = 1.0;
     for (int k = 0; k < c.cfg.max_iterations; ++k) {
-      // DistanceTerm.cpp:396-434 fused projection + colour-ordered scatter
+      // DistanceTerm.cpp:279-317 fused projection + colour-ordered scatter
 #pragma omp for
       for (long long i = 0; i < (long long)dim; ++i)
         c.b[(size_t)i] = c.bBase[(size_t)i] + (extra ? extra[i] : 0.0);
@@ -785,7 +788,7 @@ int stepOnce(orc_ctx& c) {
 #pragma omp for
         for (long long q = beg; q < end; ++q) projectScatterEdge(c, (int)c.corder[(size_t)q], c.b.data());
       }
-      // Assembler.cpp:401-414 pin overwrite per component + component solve
+      // Assembler.cpp:319-332 pin overwrite per component + component solve
 #pragma omp for schedule(static, 1)
       for (int comp = 0; comp < nComp; ++comp) {
         for (int v = 0; v < n; ++v) {
@@ -874,6 +877,7 @@ int stepOnce(orc_ctx& c) {
           resHit = (residual <= c.cfg.residual_tol * gScale);
         }
         c.lastResidual = residual;
+        if (c.iterTape) std::memcpy(c.iterTape + (size_t)k * dim, c.xSolution.data(), dim * sizeof(double));
         const bool conv = (c.cfg.residual_tol > 0.0) ? resHit : (absHit || relHit);
         c.converged = conv;
         if (conv) c.earlyExitCount += 1;
@@ -999,6 +1003,12 @@ void orc_set_pin_positions(orc_ctx* c, const double* x) {
 }
 void orc_set_edge_kappa(orc_ctx* c, const double* kappa) { c->kappa.assign(kappa, kappa + c->E); }
 void orc_set_bend_k(orc_ctx* c, const double* k) { c->bk.assign(k, k + c->B); }
+// Mesh::masses / Mesh::pinned edits; the stamps (computeStamp, computeStructureStamp)
+// decide at the next step whether to refactor or re-analyse, as in the reference.
+void orc_set_masses(orc_ctx* c, const double* m) { c->masses.assign(m, m + c->n); }
+void orc_set_pinned(orc_ctx* c, const uint8_t* p) {
+  for (int v = 0; v < c->n; ++v) c->pinned[(size_t)v] = p[v] ? 1 : 0;
+}
 
 // Reverse-mode derivative of `nsub` substeps with a fixed iteration count
 // (the unrolled map of stepOnce above: predict, K x [projection + colour-ordered
@@ -1045,29 +1055,13 @@ int orc_rollout_adjoint(orc_ctx* c, int nsub, const double* pin_traj, const doub
     }
     orc_get_velocities(c, &vtape[(size_t)s * n3]);
     getX(&tape[((size_t)s * (K + 1)) * n3]);
-    // run the substep one iteration at a time through the reference step with
-    // max_iterations = k+1 would change the factor stamp only through dt (not
-    // touched), so instead the iterates are recomputed here by the same calls:
-    const orc_config saved = c->cfg;
-    std::vector<V3> x0 = c->positions, v0 = c->velocities, pbs = c->positionsBeforeStep,
-                    ppbs = c->positionsBeforePrevStep;
-    for (int k = 1; k <= K; ++k) {
-      c->positions = x0;
-      c->velocities = v0;
-      c->positionsBeforeStep = pbs;
-      c->positionsBeforePrevStep = ppbs;
-      c->cfg.max_iterations = k;
-      const int used = stepOnce(*c);
-      if (used < 0) return used;
-      if (used != k) {
-        c->cfg = saved;
-        return -5;  // the stop test fired: the unrolled map is not fixed-K
-      }
-      // stepOnce re-pins after the velocity update; x^k of the iteration is
-      // xSolution (pinned rows hold q, equal to the re-pinned positions)
-      std::memcpy(&tape[((size_t)s * (K + 1) + k) * n3], c->xSolution.data(), n3 * sizeof(double));
-    }
-    c->cfg = saved;
+    // one reference substep, recording every iterate x^1 .. x^K as it is solved
+    // (xSolution; pinned rows hold q, equal to the re-pinned positions)
+    c->iterTape = &tape[((size_t)s * (K + 1) + 1) * n3];
+    const int used = stepOnce(*c);
+    c->iterTape = nullptr;
+    if (used < 0) return used;
+    if (used != K) return -5;  // the stop test fired: the unrolled map is not fixed-K
   }
 
   const double h = c->cfg.dt, invH2 = 1.0 / (h * h);
@@ -1075,7 +1069,7 @@ int orc_rollout_adjoint(orc_ctx* c, int nsub, const double* pin_traj, const doub
   std::vector<double> xbar(seed_x, seed_x + n3), vbar(n3, 0.0);
   if (seed_v) vbar.assign(seed_v, seed_v + n3);
   std::vector<double> gk((size_t)c->E, 0.0), gb((size_t)c->B, 0.0), gp((size_t)nsub * np * 3, 0.0);
-  std::vector<double> cur(n3), xnbar(n3), bbar(n3), mu(n3), work;
+  std::vector<double> cur(n3), xnbar(n3), bbar(n3), mu(n3);
   const double w3[3] = {1.0, -2.0, 1.0};
   for (int s = nsub - 1; s >= 0; --s) {
     const double* q = &qtape[(size_t)s * np * 3];
@@ -1095,8 +1089,12 @@ int orc_rollout_adjoint(orc_ctx* c, int nsub, const double* pin_traj, const doub
       }
     std::fill(bbar.begin(), bbar.end(), 0.0);
     for (int k = K - 1; k >= 0; --k) {
-      for (int comp = 0; comp < c->components; ++comp)
-        solveComponent(*c, c->components == 1 ? -1 : comp, cur.data(), mu.data(), work);
+      // the components write disjoint dofs: solved in parallel, same bits
+#pragma omp parallel for num_threads(std::min(c->components, std::max(1, c->threads))) if (c->threads > 1)
+      for (int comp = 0; comp < c->components; ++comp) {
+        std::vector<double> w;
+        solveComponent(*c, c->components == 1 ? -1 : comp, cur.data(), mu.data(), w);
+      }
       for (size_t i = 0; i < n3; ++i) bbar[i] += mu[i];
       const double* xk = &tape[((size_t)s * (K + 1) + k) * n3];
       const double* xk1 = &tape[((size_t)s * (K + 1) + k + 1) * n3];

@@ -64,6 +64,8 @@ void orc_set_pin_positions(orc_ctx* c, const double* q3n);
 void orc_set_edge_kappa(orc_ctx* c, const double* kappa);
 void orc_set_config(orc_ctx* c, const orc_config* cfg);
 void orc_set_bend_k(orc_ctx* c, const double* k);
+void orc_set_masses(orc_ctx* c, const double* masses);
+void orc_set_pinned(orc_ctx* c, const uint8_t* pinned);
 
 /* FP64 reverse-mode derivative of nsub substeps run from the current state with
  * a fixed iteration count (max_iterations every substep; returns -5 if the stop

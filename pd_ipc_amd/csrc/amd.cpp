@@ -11,6 +11,12 @@
 // length including the diagonal; a node whose only entry is its diagonal is empty)
 // and a node without a structural diagonal is treated as dense.
 //
+// Provenance: the code follows the structure of CSparse's cs_amd (T. Davis,
+// LGPL-2.1+) - its flip / wclear / tdfs helpers, the max(16, 10 sqrt(n)) dense
+// threshold and the workspace split - because reproducing the order of Eigen's
+// AMDOrdering (Eigen/src/OrderingMethods/Amd.h, MPL-2.0, itself derived from the
+// same code) needs its tie-breaks; it is written anew, not copied from either.
+//
 // Input: column pointers/indices of the FULL symmetric pattern with the diagonal,
 // rows sorted within each column.  Output: perm[k] = node eliminated k-th.
 #include <algorithm>

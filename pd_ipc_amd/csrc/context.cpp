@@ -508,6 +508,7 @@ int pullState(pdx_ctx* c) {
   if (!c->uploaded) return PDX_OK;
   HIPCHECK(c, hipStreamSynchronize(c->stream));
   for (auto& I : c->inst) {
+    if (!I.dx) continue;  // added since the last upload: its host state is current
     HIPCHECK(c, hipMemcpy(I.scene.x.data(), I.dx, I.scene.x.size() * 8, hipMemcpyDeviceToHost));
     HIPCHECK(c, hipMemcpy(I.scene.v.data(), I.dv, I.scene.v.size() * 8, hipMemcpyDeviceToHost));
   }
@@ -1053,10 +1054,12 @@ int ensureReady(pdx_ctx* c) {
   std::vector<std::vector<long long>> lst;
   if (c->uploaded) {
     for (auto& I : c->inst) {
-      std::vector<int> a(4);
-      std::vector<long long> b(16);
-      HIPCHECK(c, hipMemcpy(a.data(), I.distats, 16, hipMemcpyDeviceToHost));
-      HIPCHECK(c, hipMemcpy(b.data(), I.dlstats, 128, hipMemcpyDeviceToHost));
+      std::vector<int> a(4, 0);
+      std::vector<long long> b(16, 0);
+      if (I.distats) {  // instances added since the last upload start from zero
+        HIPCHECK(c, hipMemcpy(a.data(), I.distats, 16, hipMemcpyDeviceToHost));
+        HIPCHECK(c, hipMemcpy(b.data(), I.dlstats, 128, hipMemcpyDeviceToHost));
+      }
       ist.push_back(a);
       lst.push_back(b);
     }
@@ -1076,7 +1079,16 @@ int gridLaunchBlocks(pdx_ctx* c) {
   return std::min(std::min(want, maxb), 1024);
 }
 
+// LDS the grid form needs at the current workgroup size: the reduction slots and
+// one run-partial slice per wavefront.
+size_t gridLdsNeed(const pdx_ctx* c) {
+  const size_t w = c->prec == PDX_PREC_MIXED32 ? 4 : 8;
+  return kRedBytes + (size_t)(c->block / 64) * c->waveRuns * 3 * w;
+}
+
 int launchGridForm(pdx_ctx* c, int nsub) {
+  if (c->lds < gridLdsNeed(c))
+    return fail(c, PDX_EINVAL, "grid form: the launch's LDS does not hold a slice per wavefront");
   const int g = gridLaunchBlocks(c);
   if (g <= 0) return fail(c, PDX_EHIP, "grid form: no co-resident workgroup fits");
   HIPCHECK(c, launchGrid(c->dInst, 0, nsub, c->dGridSlots, c->waveRuns, g, c->block, c->lds,
@@ -1245,26 +1257,28 @@ int pdx_step_graph(pdx_ctx* c, int nsub, int replays) {
     c->lastLaunches = std::max(1, replays);
     return pdx_sync(c);
   }
-  hipGraph_t g = nullptr;
-  hipGraphExec_t ge = nullptr;
+  // the graph and its executable are destroyed on every exit path
+  struct GraphGuard {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+    ~GraphGuard() {
+      if (ge) hipGraphExecDestroy(ge);
+      if (g) hipGraphDestroy(g);
+    }
+  } gg;
   HIPCHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   const hipError_t le = launchSubstep(c->dInst, c->dOrder, (int)c->inst.size(), nsub, c->block, c->lds,
                                       c->prec == PDX_PREC_MIXED32, c->mode == PDX_SOLVE_FAST, c->idx16, c->ldsW, c->stream,
                                       0, 0);
-  HIPCHECK(c, hipStreamEndCapture(c->stream, &g));
-  if (le != hipSuccess) {
-    hipGraphDestroy(g);
-    return fail(c, PDX_EHIP, std::string("capture: ") + hipGetErrorString(le));
-  }
-  HIPCHECK(c, hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+  const hipError_t ce = hipStreamEndCapture(c->stream, &gg.g);
+  if (le != hipSuccess) return fail(c, PDX_EHIP, std::string("capture: ") + hipGetErrorString(le));
+  if (ce != hipSuccess) return fail(c, PDX_EHIP, std::string("end capture: ") + hipGetErrorString(ce));
+  HIPCHECK(c, hipGraphInstantiate(&gg.ge, gg.g, nullptr, nullptr, 0));
   HIPCHECK(c, hipEventRecord(c->ev0, c->stream));
-  for (int r = 0; r < replays; ++r) HIPCHECK(c, hipGraphLaunch(ge, c->stream));
+  for (int r = 0; r < replays; ++r) HIPCHECK(c, hipGraphLaunch(gg.ge, c->stream));
   HIPCHECK(c, hipEventRecord(c->ev1, c->stream));
   c->lastLaunches = std::max(1, replays);
-  rc = pdx_sync(c);
-  hipGraphExecDestroy(ge);
-  hipGraphDestroy(g);
-  return rc;
+  return pdx_sync(c);
 }
 
 int pdx_get_state(pdx_ctx* c, int i, double* x, double* v) {
@@ -1328,12 +1342,56 @@ int pdx_set_bend_k(pdx_ctx* c, int i, const double* k) {
   return PDX_OK;
 }
 
+int pdx_set_masses(pdx_ctx* c, int i, const double* masses) {
+  if (!c || !masses || i < 0 || i >= (int)c->inst.size()) return PDX_EINVAL;
+  Instance& I = c->inst[(size_t)i];
+  for (int v = 0; v < I.scene.n; ++v)
+    if (!(masses[v] >= 0.0) || !std::isfinite(masses[v])) return fail(c, PDX_EINVAL, "invalid mass");
+  if (std::memcmp(I.scene.mass.data(), masses, I.scene.mass.size() * 8) == 0) return PDX_OK;
+  const int rc = pullState(c);
+  if (rc) return rc;
+  std::memcpy(I.scene.mass.data(), masses, I.scene.mass.size() * 8);
+  // masses enter L = M/h^2 + ...: the numeric stamp changes, the next step refactors
+  if (numericStamp(I.scene, I.cfg.dt) != I.numStamp) c->dirty = true;
+  return PDX_OK;
+}
+
+int pdx_set_pinned(pdx_ctx* c, int i, const uint8_t* pinned) {
+  if (!c || !pinned || i < 0 || i >= (int)c->inst.size()) return PDX_EINVAL;
+  Instance& I = c->inst[(size_t)i];
+  bool same = true;
+  for (int v = 0; v < I.scene.n && same; ++v) same = (pinned[v] != 0) == (I.scene.pinned[(size_t)v] != 0);
+  if (same) return PDX_OK;
+  const int rc = pullState(c);
+  if (rc) return rc;
+  const int before = pdx_pin_count(c, i);
+  for (int v = 0; v < I.scene.n; ++v) I.scene.pinned[(size_t)v] = pinned[v] ? 1 : 0;
+  // the mask decides which rows are overwritten and which coupling blocks exist:
+  // both stamps change, so the next step analyses and factors again; the
+  // incidence and bend records carry the pinned bits and are rebuilt with the upload
+  if (I.scene.B > 0) bendingRhs(I.scene, I.brhs);
+  if (pdx_pin_count(c, i) != before && i < (int)c->roll.traj.size()) {
+    c->roll.traj[(size_t)i].clear();  // a pin trajectory no longer matches the pins
+    c->roll.trajSub[(size_t)i] = 0;
+  }
+  c->dirty = true;
+  return PDX_OK;
+}
+
 int pdx_set_config(pdx_ctx* c, int i, const pdx_config* cfg) {
   if (!c || !cfg || i < 0 || i >= (int)c->inst.size()) return PDX_EINVAL;
   if (cfg->dt <= 0.0 || cfg->max_iterations < 0) return fail(c, PDX_EINVAL, "invalid config");
+  Config next = c->inst[(size_t)i].cfg;
+  fillConfig(next, cfg);
+  const Config& cur = c->inst[(size_t)i].cfg;
+  const bool same = next.dt == cur.dt && next.g[0] == cur.g[0] && next.g[1] == cur.g[1] && next.g[2] == cur.g[2] &&
+                    next.damping == cur.damping && next.maxIter == cur.maxIter && next.relTol == cur.relTol &&
+                    next.absTol == cur.absTol && next.resTol == cur.resTol && next.chebRho == cur.chebRho &&
+                    next.chebStart == cur.chebStart;
+  if (same) return PDX_OK;  // unchanged: nothing to upload
   const int rc = pullState(c);
   if (rc) return rc;
-  fillConfig(c->inst[(size_t)i].cfg, cfg);
+  c->inst[(size_t)i].cfg = next;
   c->dirty = true;  // damping/tolerances re-upload; only a dt change refactors (stamp)
   return PDX_OK;
 }
@@ -1393,6 +1451,8 @@ int pdx_solve3(pdx_ctx* c, int i, const double* b, double* x) {
     HIPCHECK(c, launchDenseSolve(c->dInst, i, c->dScratchB, c->dScratchX, I.scene.n, I.fac.m,
                                  c->prec == PDX_PREC_MIXED32, c->stream));
   } else if (c->launchMode == 1) {
+    if (c->lds < gridLdsNeed(c))
+      return fail(c, PDX_EINVAL, "grid form: the launch's LDS does not hold a slice per wavefront");
     const int g = gridLaunchBlocks(c);
     HIPCHECK(c, launchGridSolve(c->dInst, i, c->dScratchB, c->dScratchX, c->waveRuns, g, c->block, c->lds,
                                 c->prec == PDX_PREC_MIXED32, c->idx16, c->stream));
@@ -1409,7 +1469,8 @@ int pdx_get_factor(pdx_ctx* c, int i, int* m, long long* nnz, int32_t* ptr, int3
                    double* dinv, int32_t* rank_vertex, int32_t* perm3) {
   if (!c || i < 0 || i >= (int)c->inst.size()) return PDX_EINVAL;
   Instance& I = c->inst[(size_t)i];
-  if (!I.factored) {
+  if (!I.factored || !I.packed || numericStamp(I.scene, I.cfg.dt) != I.numStamp) {
+    // host-side refresh under the stamp rules (the device copy follows at the next step)
     bool changed = false;
     const int rc = refreshFactors(c, changed);
     if (rc) return rc;
@@ -1776,7 +1837,7 @@ int pdx_get_topology(pdx_ctx* c, int i, int* color_count, uint32_t* color_of, ui
 int pdx_get_schedule_stats(pdx_ctx* c, int i, int64_t* out) {
   if (!c || !out || i < 0 || i >= (int)c->inst.size()) return PDX_EINVAL;
   Instance& I = c->inst[(size_t)i];
-  if (!I.factored || !I.packed) {
+  if (!I.factored || !I.packed || numericStamp(I.scene, I.cfg.dt) != I.numStamp) {
     bool changed = false;
     const int rc = refreshFactors(c, changed);
     if (rc) return rc;
@@ -1819,6 +1880,13 @@ int pdx_set_schedule(pdx_ctx* c, int block_threads, int chain_cap) {
     if (rc) return rc;
     c->chainCap = chain_cap;
     for (auto& I : c->inst) I.packed = false;  // repack the same factor
+    c->dirty = true;
+  }
+  if (c->launchMode == 1 && c->block != block_threads) {
+    // the grid form gives every wavefront its own LDS slice: the launch's LDS
+    // follows the workgroup size (upload sizes it)
+    const int rc = pullState(c);
+    if (rc) return rc;
     c->dirty = true;
   }
   c->block = block_threads;

@@ -32,7 +32,7 @@ uint64_t bitsOf(double v) {
   std::memcpy(&u, &v, 8);
   return u;
 }
-// core/assemble/Assembler.cpp:104-107
+// core/assemble/Assembler.cpp:22-25
 inline uint64_t mix(uint64_t h, uint64_t v) { return h ^ (v + 0x9e3779b97f4a7c15ULL + (h << 6) + (h >> 2)); }
 }  // namespace
 
@@ -154,8 +154,8 @@ void buildTopology(const Scene& s, Topology& t) {
   }
 }
 
-// core/assemble/Assembler.cpp:221-340 with DistanceTerm::assembleMatrix
-// (core/energy/DistanceTerm.cpp:436-466).  Every 3x3 block is scalar x I3, so one
+// core/assemble/Assembler.cpp:139-278 with DistanceTerm::assembleMatrix
+// (core/energy/DistanceTerm.cpp:319-349).  Every 3x3 block is scalar x I3, so one
 // scalar component is assembled; duplicate triplets are summed in insertion order
 // as Eigen's setFromTriplets does.
 void assembleMatrix(const Scene& s, double dt, Matrix& A) {
@@ -192,7 +192,7 @@ void assembleMatrix(const Scene& s, double dt, Matrix& A) {
       }
     }
   }
-  // pinned rows: diagonal completed to 1 (Assembler.cpp:322-335).  Every edge at a
+  // pinned rows: diagonal completed to 1 (Assembler.cpp:227-253).  Every edge at a
   // pinned vertex is skipped by the "either end pinned" filter and the bend term of
   // a pinned row is 0, so only the inertia term is "existing".
   for (int v = 0; v < n; ++v) {
@@ -220,7 +220,7 @@ void assembleMatrix(const Scene& s, double dt, Matrix& A) {
   for (int v = 0; v < n; ++v) A.colptr[(size_t)v + 1] += A.colptr[(size_t)v];
 }
 
-// core/assemble/Assembler.cpp:176-219
+// core/assemble/Assembler.cpp:94-137
 void bendingRhs(const Scene& s, std::vector<double>& out) {
   out.assign((size_t)3 * s.n, 0.0);
   const double w[3] = {1.0, -2.0, 1.0};
@@ -249,7 +249,7 @@ void bendingRhs(const Scene& s, std::vector<double>& out) {
   }
 }
 
-// core/assemble/Assembler.cpp:111-147 (damping excluded)
+// core/assemble/Assembler.cpp:29-65 (damping excluded)
 uint64_t numericStamp(const Scene& s, double dt) {
   uint64_t sh = 0xcbf29ce484222325ULL, mh = sh, ph = sh;
   for (int e = 0; e < s.E; ++e) sh = mix(sh, bitsOf(s.kappa[(size_t)e]));
@@ -262,7 +262,7 @@ uint64_t numericStamp(const Scene& s, double dt) {
   return mix(h, ph);
 }
 
-// core/assemble/Assembler.cpp:149-174
+// core/assemble/Assembler.cpp:67-92
 uint64_t structureStamp(const Scene& s) {
   uint64_t h = 0xcbf29ce484222325ULL;
   h = mix(h, 1);
@@ -289,6 +289,9 @@ namespace {
 // column counts, then row k of L by a sparse triangular solve over the etree
 // reach, visited in the same stack order.  The x/y/z components of the reference's
 // 3n factor are this factor, bit for bit, in each component's induced order.
+// Provenance: the steps are those of Davis' LDL package (LGPL-2.1+) as Eigen's
+// SimplicialLDLT (MPL-2.0) restates them; the code is written anew around this
+// library's rank-space matrix.
 int ldlRank(const Matrix& A, const std::vector<int>& vertexRank, const std::vector<int>* symbolic,
             Factor& f) {
   const int m = A.n;

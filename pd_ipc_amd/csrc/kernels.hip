@@ -9,7 +9,7 @@
 //
 // Two arithmetic forms:
 //  * exact (FP64):  b is gathered per vertex in the colour order of the reference
-//    scatter (DistanceTerm.cpp:396-434), the sweeps run each row's dot product in
+//    scatter (DistanceTerm.cpp:279-317), the sweeps run each row's dot product in
 //    the reference's column order (EigenDirectSolver.cpp:418-446), the residual
 //    is the reference gather (Integrator.cpp:230-263).  Same bits as the CPU path.
 //  * mixed (FP32 factor):  each PD iteration solves for the update
@@ -644,7 +644,7 @@ struct Acc3 {
 
 // Reference right-hand side of one vertex (exact form): seed bBase + bendRhs, then
 // each incident edge in colour order: +-kappa d_e and the pin compensation
-// kappa q (DistanceTerm.cpp:211-269, :396-434); pinned rows take q (Assembler.cpp:401-414).
+// kappa q (DistanceTerm.cpp:94-102, :121-152, :279-317); pinned rows take q (Assembler.cpp:319-332).
 __device__ inline Acc3 rhsExact(const InstDev& I, int v) {
   if (I.pinned[v]) return {I.pinpos[3 * v], I.pinpos[3 * v + 1], I.pinpos[3 * v + 2]};
   const double mh2 = I.mass[v] * I.invH2;

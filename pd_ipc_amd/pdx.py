@@ -107,6 +107,8 @@ def load_library(path=LIB_PATH):
         "pdx_set_pin_positions": (I, [P, I, P]),
         "pdx_set_edge_kappa": (I, [P, I, P]),
         "pdx_set_bend_k": (I, [P, I, P]),
+        "pdx_set_masses": (I, [P, I, P]),
+        "pdx_set_pinned": (I, [P, I, P]),
         "pdx_set_config": (I, [P, I, C.POINTER(_Config)]),
         "pdx_get_stats": (I, [P, I, C.POINTER(_Stats)]),
         "pdx_solve3": (I, [P, I, P, P]),
@@ -148,6 +150,15 @@ def load_library(path=LIB_PATH):
 
 def _ptr(a):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _arr(a, dtype, size, what):
+    """`a` as a contiguous 1-D array of `size` values, or ValueError: the C side
+    copies exactly that many values, so a short array would be read past its end."""
+    out = np.ascontiguousarray(a, dtype).reshape(-1)
+    if out.size != size:
+        raise ValueError(f"{what}: expected {size} values, got {out.size}")
+    return out
 
 
 def device_info(device=0):
@@ -321,22 +332,38 @@ class Engine:
         self._check(self.lib.pdx_get_state(self.h, i, _ptr(x), _ptr(v)), "get_state")
         return x, v
 
+    def _cloth(self, i):
+        if not 0 <= i < len(self.cloths):
+            raise IndexError(f"instance {i} of {len(self.cloths)}")
+        return self.cloths[i]
+
     def set_state(self, i, x=None, v=None):
-        x = None if x is None else np.ascontiguousarray(x, np.float64)
-        v = None if v is None else np.ascontiguousarray(v, np.float64)
+        n3 = 3 * self._cloth(i).n
+        x = None if x is None else _arr(x, np.float64, n3, "set_state positions")
+        v = None if v is None else _arr(v, np.float64, n3, "set_state velocities")
         self._check(self.lib.pdx_set_state(self.h, i, _ptr(x), _ptr(v)), "set_state")
 
     def set_pin_positions(self, i, q):
-        q = np.ascontiguousarray(q, np.float64)
+        q = _arr(q, np.float64, 3 * self._cloth(i).n, "set_pin_positions")
         self._check(self.lib.pdx_set_pin_positions(self.h, i, _ptr(q)), "set_pin_positions")
 
     def set_edge_kappa(self, i, kappa):
-        k = np.ascontiguousarray(kappa, np.float64)
+        k = _arr(kappa, np.float64, self._cloth(i).n_edges, "set_edge_kappa")
         self._check(self.lib.pdx_set_edge_kappa(self.h, i, _ptr(k)), "set_edge_kappa")
 
     def set_bend_k(self, i, k):
-        k = np.ascontiguousarray(k, np.float64)
+        k = _arr(k, np.float64, self._cloth(i).n_bends, "set_bend_k")
         self._check(self.lib.pdx_set_bend_k(self.h, i, _ptr(k)), "set_bend_k")
+
+    def set_masses(self, i, masses):
+        """Mesh::masses edit: the next step refactors once (no new analysis)."""
+        m = _arr(masses, np.float64, self._cloth(i).n, "set_masses")
+        self._check(self.lib.pdx_set_masses(self.h, i, _ptr(m)), "set_masses")
+
+    def set_pinned(self, i, pinned):
+        """Mesh::pinned edit: the next step analyses and factors again."""
+        p = _arr(np.asarray(pinned) != 0, np.uint8, self._cloth(i).n, "set_pinned")
+        self._check(self.lib.pdx_set_pinned(self.h, i, _ptr(p)), "set_pinned")
 
     def set_config(self, i, cfg):
         c = cfg._c()
@@ -365,7 +392,7 @@ class Engine:
         return v.value
 
     def solve(self, i, b):
-        b = np.ascontiguousarray(b, np.float64)
+        b = _arr(b, np.float64, 3 * self._cloth(i).n, "solve")
         x = np.empty_like(b)
         self._check(self.lib.pdx_solve3(self.h, i, _ptr(b), _ptr(x)), "solve3")
         return x
@@ -414,7 +441,8 @@ class Engine:
             self._check(self.lib.pdx_set_pin_trajectory(self.h, i, 0, None), "set_pin_trajectory")
             return
         t = np.ascontiguousarray(traj, np.float64)
-        assert t.ndim == 3 and t.shape[1] == self.pin_count(i) and t.shape[2] == 3
+        if t.ndim != 3 or t.shape[1] != self.pin_count(i) or t.shape[2] != 3:
+            raise ValueError(f"set_pin_trajectory: expected (nsub, {self.pin_count(i)}, 3), got {t.shape}")
         self._check(self.lib.pdx_set_pin_trajectory(self.h, i, t.shape[0], _ptr(t)), "set_pin_trajectory")
 
     def rollout(self, nsub, checkpoint_every=10):
@@ -423,8 +451,9 @@ class Engine:
 
     def set_adjoint_seed(self, i, dx, dv=None):
         self._seeds = getattr(self, "_seeds", {})
-        dx = np.ascontiguousarray(dx, np.float64).reshape(-1)
-        dv = None if dv is None else np.ascontiguousarray(dv, np.float64).reshape(-1)
+        n3 = 3 * self._cloth(i).n
+        dx = _arr(dx, np.float64, n3, "set_adjoint_seed dx")
+        dv = None if dv is None else _arr(dv, np.float64, n3, "set_adjoint_seed dv")
         self._seeds[i] = (dx, dv)
         self._check(self.lib.pdx_set_adjoint_seed(self.h, i, _ptr(dx), None if dv is None else _ptr(dv)),
                     "set_adjoint_seed")

@@ -46,7 +46,8 @@ def lib():
             ("orc_coloring", I, [P, P, P, P]), ("orc_factor", I, [P]),
             ("orc_compact_factor", I, [P, P, P, P, P, P, P]), ("orc_matrix", C.c_int64, [P, P, P, P]),
             ("orc_solve3", I, [P, P, P]), ("orc_nonlinear_residual", D, [P]), ("orc_bending_rhs", None, [P, P]),
-            ("orc_set_bend_k", None, [P, P]),
+            ("orc_set_bend_k", None, [P, P]), ("orc_set_masses", None, [P, P]),
+            ("orc_set_pinned", None, [P, P]),
             ("orc_rollout_adjoint", I, [P, I, P, P, P, P, P, P, P, P]),
         ]:
             fn = getattr(L, name)
@@ -129,6 +130,16 @@ class Oracle:
 
     def set_bend_k(self, k):
         self.L.orc_set_bend_k(self.h, _p(np.ascontiguousarray(k, np.float64)))
+
+    def set_masses(self, m):
+        self.L.orc_set_masses(self.h, _p(np.ascontiguousarray(m, np.float64)))
+
+    def set_pinned(self, p):
+        self.L.orc_set_pinned(self.h, _p(np.ascontiguousarray(np.asarray(p) != 0, np.uint8)))
+
+    def set_ordering(self, perm3):
+        perm3 = np.ascontiguousarray(perm3, np.int32)
+        assert self.L.orc_set_ordering(self.h, _p(perm3)) == 0
 
     def rollout_adjoint(self, nsub, pin_traj, seed_x, seed_v=None):
         """FP64 CPU adjoint of nsub fixed-iteration substeps (advances the state).

@@ -1,0 +1,237 @@
+"""The workloads at their full sizes on the GPU, checked against the FP64 paths:
+
+* the 512-instance mix, 200 substeps, through pdx_step_graph (the benchmark's
+  path): the FP32-solve form in the nested-dissection order against the FP64
+  batch in the reference's AMD order, every instance within 1e-5 m, and four
+  sampled instances against the FP64 CPU oracle;
+* the 512 x 512 cloth: the FP32-solve grid form against the FP64 batch form;
+* the differentiable batch, two 128 x 128 instances, 120 substeps, 20 iterations,
+  checkpoints every 10: GPU adjoint gradients (FP64 and FP32 solves) against
+  central differences of the GPU FP64 forward, and one instance against the FP64
+  CPU adjoint of the oracle.
+
+The CPU oracle runs on host threads while the GPU works.  Every GPU step prints
+its case first."""
+import threading
+import time
+
+import numpy as np
+import pytest
+
+from pd_ipc_amd import Cloth, Engine, SimConfig, scenes
+from oracle_ffi import Oracle
+
+pytestmark = pytest.mark.gpu
+
+T0 = time.time()
+
+
+def _say(*a):
+    print(f"[case {time.time() - T0:7.1f}s]", *a, flush=True)
+
+
+class _Bg(threading.Thread):
+    """Runs fn() on a host thread (the oracle's ctypes calls release the GIL)."""
+
+    def __init__(self, fn):
+        super().__init__(daemon=True)
+        self.fn, self.out, self.err = fn, None, None
+        self.start()
+
+    def run(self):
+        try:
+            self.out = self.fn()
+        except BaseException as e:  # re-raised by result()
+            self.err = e
+
+    def result(self):
+        self.join()
+        if self.err is not None:
+            raise self.err
+        return self.out
+
+
+@pytest.mark.timeout(900)
+def test_mix_512_full_size_through_graph():
+    cloths, cfg, nsub = scenes.config4()
+    assert len(cloths) == 512 and nsub == 200
+    sizes = np.array([c.n for c in cloths])
+    bends = [i for i, c in enumerate(cloths) if c.n_bends > 0]
+    sample = sorted({int(np.argmin(sizes)), int(np.argmax(sizes)), bends[0], bends[len(bends) // 2]})
+    _say("mix: factor FP64 AMD and mixed32 ND contexts,", int(sizes.sum()), "vertices")
+    e64 = Engine("fp64", "fast")
+    e32 = Engine("mixed32", "fast")
+    for c in cloths:
+        e64.add(c, cfg)
+        e32.add(c, cfg)
+    e32.set_ordering("nd")
+    e64.factor()
+    e32.factor()
+    # the oracle on the sampled instances (reference AMD order), on host threads
+    perms = {i: e64.factor_arrays(i)["perm3"] for i in sample}
+
+    def oracle_run(i):
+        o = Oracle(cloths[i], cfg, perms[i], threads=4)
+        o.step(nsub)
+        return o.positions()
+
+    bg = {i: _Bg(lambda i=i: oracle_run(i)) for i in sample}
+    _say("mix: mixed32 ND, 200 substeps as 200 graph replays")
+    e32.step_graph(1, nsub)
+    _say("mix: FP64 AMD, 200 substeps")
+    e64.step_graph(1, nsub)
+    err = np.zeros(len(cloths))
+    for i in range(len(cloths)):
+        x32, _ = e32.state(i)
+        x64, _ = e64.state(i)
+        assert np.all(np.isfinite(x32))
+        err[i] = np.abs(x32 - x64).max()
+        s = e32.stats(i)
+        assert s["substeps"] == nsub and s["factorize_count"] == 1 and s["iterations_used"] == 40
+    print("max |x32 - x64| over instances", err.max(), "at", int(err.argmax()), "median", np.median(err))
+    assert err.max() <= 1e-5
+    for i in sample:
+        _say("mix: oracle instance", i, "n =", cloths[i].n)
+        xo = bg[i].result()
+        x64, _ = e64.state(i)
+        x32, _ = e32.state(i)
+        rel = np.abs(x64 - xo).max() / np.abs(xo).max()
+        e32o = np.abs(x32 - xo).max()
+        print("instance", i, "FP64 batch vs oracle rel", rel, "mixed32 vs oracle", e32o)
+        assert rel <= 1e-10
+        assert e32o <= 1e-5
+
+
+@pytest.mark.timeout(900)
+def test_large_cloth_512_grid_form():
+    c, cfg, _ = scenes.config3()
+    nsub = 3
+    _say("512x512: factor mixed32 ND grid form")
+    e32 = Engine("mixed32", "fast")
+    e32.add(c, cfg)
+    e32.set_ordering("nd")
+    e32.set_launch("grid")
+    e32.factor()
+    _say("512x512: factor FP64 batch (one workgroup, AMD)")
+    e64 = Engine("fp64", "fast")
+    e64.add(c, cfg)
+    e64.factor()
+    _say("512x512: mixed32 grid form,", nsub, "substeps")
+    e32.step(nsub)
+    _say("512x512: FP64 batch,", nsub, "substeps")
+    e64.step(nsub)
+    x32, _ = e32.state(0)
+    x64, _ = e64.state(0)
+    err = np.abs(x32 - x64).max()
+    print("512x512 max |x32 - x64|", err, "after", nsub, "substeps")
+    assert np.all(np.isfinite(x32)) and err <= 1e-5
+    for e in (e32, e64):
+        s = e.stats(0)
+        assert s["factorize_count"] == 1 and s["substeps"] == nsub and s["iterations_used"] == 40
+
+
+def _scaled(c, s):
+    d = Cloth(**{k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in c.__dict__.items()})
+    d.edge_kappa = c.edge_kappa * s
+    d.bend_k = c.bend_k * s
+    return d
+
+
+def _with_v0(c, v0):
+    d = Cloth(**{k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in c.__dict__.items()})
+    d.velocities = v0
+    return d
+
+
+def _rollout(cloths, trajs, cfg, nsub, precision, every=10, seeds=None):
+    e = Engine(precision, "fast")
+    for c in cloths:
+        e.add(c, cfg)
+    e.set_ordering("nd")
+    for i, t in enumerate(trajs):
+        e.set_pin_trajectory(i, t)
+    e.rollout(nsub, every)
+    xs = [e.state(i)[0] for i in range(len(cloths))]
+    g = None
+    if seeds is not None:
+        for i, s in enumerate(seeds):
+            e.set_adjoint_seed(i, s(xs[i]))
+        e.adjoint()
+        g = [e.gradients(i) for i in range(len(cloths))]
+        fwd, adj = e.rollout_timing()
+        print(precision, "rollout", round(fwd, 1), "ms, adjoint", round(adj, 1), "ms")
+    e.close()
+    return xs, g
+
+
+@pytest.mark.timeout(900)
+def test_adjoint_config5_two_instances_full_length():
+    cloths, cfg, nsub, hidden = scenes.config5(count=2)
+    assert cloths[0].n == 128 * 128 and nsub == 120 and cfg.max_iterations == 20
+    base_traj = [scenes.pin_trajectory(c, nsub, cfg.dt, amplitude=0.0) for c in cloths]  # static pins
+    _say("config 5: hidden forward (FP64) for the targets")
+    targets, _ = _rollout([_scaled(c, h["kappa"] / 2300.0) for c, h in zip(cloths, hidden)],
+                          [h["traj"] for h in hidden], cfg, nsub, "fp64")
+    seeds = [lambda x, t=t: 2.0 * (x - t) for t in targets]
+    # FP64 CPU adjoint of instance 0 (reference AMD order) on host threads
+    e = Engine("fp64")
+    e.add(cloths[0], cfg)
+    perm = e.factor_arrays(0)["perm3"]
+    e.close()
+
+    def cpu_adjoint():
+        o = Oracle(cloths[0], cfg, perm, threads=8)
+        o_fwd = Oracle(cloths[0], cfg, perm, threads=8)
+        q = cloths[0].pin_positions.copy().reshape(-1, 3)
+        pins = np.flatnonzero(cloths[0].pinned)
+        for s in range(nsub):  # the forward of the loss, with the same pin targets
+            q[pins] = base_traj[0][s]
+            o_fwd.set_pin_positions(q.reshape(-1))
+            o_fwd.step(1)
+        xT = o_fwd.positions()
+        return xT, o.rollout_adjoint(nsub, base_traj[0], 2.0 * (xT - targets[0]))
+
+    bg = _Bg(cpu_adjoint)
+    _say("config 5: GPU adjoint, FP64 solves")
+    x64, g64 = _rollout(cloths, base_traj, cfg, nsub, "fp64", seeds=seeds)
+    _say("config 5: GPU adjoint, FP32 solves")
+    x32, g32 = _rollout(cloths, base_traj, cfg, nsub, "mixed32", seeds=seeds)
+    print("max |x32 - x64| after the rollout", max(np.abs(a - b).max() for a, b in zip(x32, x64)))
+    # central differences of the GPU FP64 forward, all perturbed copies in one launch:
+    # per instance, stiffness scale at a relative step of 1e-3 (the FP64 forward's
+    # ~1e-12 m rounding noise swamps a 1e-6 step on this parameter), velocities and
+    # pin trajectory at a step of 1e-6 along seeded directions
+    eps, ks = 1e-6, 1e-3
+    rng = np.random.default_rng(55)
+    dirs_v, dirs_p, fd_cloths, fd_trajs = [], [], [], []
+    for c, t in zip(cloths, base_traj):
+        dv = rng.standard_normal(3 * c.n)
+        dv[np.repeat(c.pinned.astype(bool), 3)] = 0.0
+        dp = rng.standard_normal(t.shape)
+        dirs_v.append(dv)
+        dirs_p.append(dp)
+        fd_cloths += [_scaled(c, 1 + ks), _scaled(c, 1 - ks), _with_v0(c, c.velocities + eps * dv),
+                      _with_v0(c, c.velocities - eps * dv), c, c]
+        fd_trajs += [t, t, t, t, t + eps * dp, t - eps * dp]
+    _say("config 5: central differences, 12 perturbed rollouts in one launch (FP64)")
+    xs, _ = _rollout(fd_cloths, fd_trajs, cfg, nsub, "fp64")
+    for i in range(2):
+        L = [float(np.sum((x - targets[i]) ** 2)) for x in xs[6 * i:6 * i + 6]]
+        fd = {"kappa": (L[0] - L[1]) / (2 * ks), "v0": (L[2] - L[3]) / (2 * eps), "pin": (L[4] - L[5]) / (2 * eps)}
+        for name, g in (("fp64", g64[i]), ("mixed32", g32[i])):
+            ad = {"kappa": g["kappa"] @ cloths[i].edge_kappa + g["bend"] @ cloths[i].bend_k,
+                  "v0": g["v0"] @ dirs_v[i], "pin": float(np.sum(g["pin"] * dirs_p[i]))}
+            for k in fd:
+                rel = abs(ad[k] - fd[k]) / abs(fd[k])
+                print(f"instance {i} {name} {k}: fd {fd[k]:.10g} adjoint {ad[k]:.10g} rel {rel:.2e}")
+                assert rel <= 1e-4, (i, name, k)
+    _say("config 5: FP64 CPU adjoint of instance 0")
+    xT_cpu, gc = bg.result()
+    print("GPU FP64 (ND order) vs CPU oracle (AMD order) xT", np.abs(x64[0] - xT_cpu).max())
+    for k in ("x0", "v0", "kappa", "bend", "pin"):
+        a, b = np.asarray(g64[0][k]).reshape(-1), np.asarray(gc[k]).reshape(-1)
+        rel = np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+        a32 = np.asarray(g32[0][k]).reshape(-1)
+        rel32 = np.abs(a32 - b).max() / max(np.abs(b).max(), 1e-300)
+        print(f"{k}: GPU FP64 vs CPU adjoint {rel:.2e}, GPU mixed32 vs CPU adjoint {rel32:.2e}")
+        assert rel <= 1e-7, k

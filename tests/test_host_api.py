@@ -107,7 +107,7 @@ def test_pin_trajectory_shape_checked():
     e = Engine("mixed32", "fast")
     e.add(c, SimConfig.fixed_iterations(10, 1.0 / 600.0))
     assert e.pin_count(0) == 2
-    with pytest.raises(AssertionError):
+    with pytest.raises(ValueError):
         e.set_pin_trajectory(0, np.zeros((5, 3, 3)))
     e.set_pin_trajectory(0, np.zeros((5, 2, 3)))
     e.set_pin_trajectory(0, None)

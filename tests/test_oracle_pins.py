@@ -236,14 +236,15 @@ def test_assembled_blocks():
 
 
 def test_factor_once_stamp_rules():
-    """spring_vertical.cpp:312-378: 1 factorisation over many steps; kappa / h / pin
-    mask change -> +1; moving a pin or changing damping -> +0."""
+    """spring_vertical.cpp:312-378 and :528-563: 1 factorisation over many steps;
+    kappa / h / mass change -> +1 factorisation; pin mask change -> +1 analysis and
+    +1 factorisation; moving a pin or changing damping -> +0."""
     c = scenes.grid_cloth(10, 8, 2300.0)
     cfg = SimConfig.fixed_iterations(3, 1 / 600)
-    _, f = _perm(c, cfg)
+    e, f = _perm(c, cfg)
     o = Oracle(c, cfg, f["perm3"])
     o.step(20)
-    assert o.stats()["factorize_count"] == 1
+    assert o.stats()["factorize_count"] == 1 and o.stats()["analyze_count"] == 1
     q = c.pin_positions.copy()
     q[3 * (c.n - 1)] += 0.01
     o.set_pin_positions(q)
@@ -258,6 +259,84 @@ def test_factor_once_stamp_rules():
     o.set_config(cfg3)
     o.step(2)
     assert o.stats()["factorize_count"] == 3
+    o.set_masses(c.masses * 1.5)
+    o.step(2)
+    assert o.stats()["factorize_count"] == 4 and o.stats()["analyze_count"] == 1
+    # pin an interior vertex where it is now: the pattern loses its coupling
+    # blocks, so the order is analysed again (the product's AMD of the new pattern)
+    v = 3 * 10 + 4
+    x = o.positions()
+    pinned = c.pinned.copy()
+    pinned[v] = 1
+    q2 = q.copy()
+    q2[3 * v:3 * v + 3] = x[3 * v:3 * v + 3]
+    e.set_pinned(0, pinned)
+    f2 = e.factor_arrays(0)
+    assert not np.array_equal(f2["perm3"], f["perm3"])
+    o.set_pinned(pinned)
+    o.set_pin_positions(q2)
+    o.set_ordering(f2["perm3"])
+    o.step(3)
+    st = o.stats()
+    assert st["factorize_count"] == 5 and st["analyze_count"] == 2
+    assert np.array_equal(o.positions()[3 * v:3 * v + 3], q2[3 * v:3 * v + 3])  # held exactly
+
+
+def test_product_stamp_counters_host_side():
+    """The library's own counters under the same rules, host side (no device):
+    pdx_get_factor refreshes the factor when a stamp changed."""
+    c = scenes.grid_cloth(9, 7, 2300.0, shear=500.0, bend=50.0)
+    cfg = SimConfig.fixed_iterations(3, 1 / 600)
+    e = Engine("fp64")
+    e.add(c, cfg)
+
+    def counts():
+        e.factor_arrays(0)
+        s = e.stats(0)
+        return s["factorize_count"], s["analyze_count"]
+
+    assert counts() == (1, 1)
+    e.set_masses(0, c.masses * 2.0)
+    assert counts() == (2, 1)
+    e.set_masses(0, c.masses * 2.0)  # unchanged: no refactor
+    assert counts() == (2, 1)
+    pinned = c.pinned.copy()
+    pinned[20] = 1
+    e.set_pinned(0, pinned)
+    assert counts() == (3, 2)
+    e.set_pinned(0, pinned)  # unchanged mask: a no-op
+    assert counts() == (3, 2)
+    q = c.pin_positions.copy()
+    q[3 * 20 + 1] -= 0.01
+    e.set_pin_positions(0, q)
+    e.set_config(0, SimConfig.fixed_iterations(3, 1 / 600, damping=0.3))
+    assert counts() == (3, 2)
+    e.set_config(0, SimConfig.fixed_iterations(3, 1 / 500, damping=0.3))
+    assert counts() == (4, 2)
+    e.set_edge_kappa(0, c.edge_kappa * 3.0)
+    e.set_bend_k(0, c.bend_k * 3.0)
+    assert counts() == (5, 2)
+    assert e.pin_count(0) == int(pinned.sum())
+
+
+def test_engine_rejects_mis_sized_arrays():
+    """The C side copies exactly n, 3n, E or B values: the mirror refuses other sizes."""
+    c = scenes.grid_cloth(6, 5, 2300.0, bend=50.0)
+    e = Engine("fp64")
+    e.add(c, SimConfig.fixed_iterations(3, 1 / 600))
+    n = c.n
+    bad = [(e.set_state, (0, np.zeros(3 * n - 1))), (e.set_state, (0, None, np.zeros(3 * n + 3))),
+           (e.set_pin_positions, (0, np.zeros(n))), (e.set_edge_kappa, (0, np.zeros(c.n_edges + 1))),
+           (e.set_bend_k, (0, np.zeros(c.n_bends - 1))), (e.set_masses, (0, np.zeros(n - 1))),
+           (e.set_pinned, (0, np.zeros(n + 1))), (e.solve, (0, np.zeros(3))),
+           (e.set_adjoint_seed, (0, np.zeros(3 * n - 3)))]
+    for fn, args in bad:
+        with pytest.raises(ValueError):
+            fn(*args)
+    with pytest.raises(ValueError):
+        e.set_pin_trajectory(0, np.zeros((4, e.pin_count(0) + 1, 3)))
+    with pytest.raises(IndexError):
+        e.set_masses(1, c.masses)
 
 
 def test_translation_invariance():
