@@ -214,6 +214,30 @@ int pdx_set_schedule(pdx_ctx* ctx, int block_threads, int chain_cap);
  * the next pdx_factor / pdx_step (counted in analyze_count and factorize_count). */
 int pdx_set_ordering(pdx_ctx* ctx, int ordering);
 
+/* Splits every instance of at least min_vertices vertices over two workgroups
+ * (0, the default, keeps one workgroup per instance).  No counterpart in the
+ * reference, whose solve is one CPU call per cloth (EigenDirectSolver.cpp:411-447).
+ * Applies to the batch launch of PDX_PREC_MIXED32 / PDX_SOLVE_FAST contexts in
+ * PDX_ORDER_ND whose factor has a top-level split and W in LDS: the two
+ * workgroups sweep one half of the split each and meet at its separator, and
+ * each gathers the right-hand sides and residuals of its own vertices, so the
+ * instance's critical path is about halved.  Both workgroups run in one launch;
+ * they wait for each other with a bound, and a launch where a wait gave up
+ * returns PDX_EHIP.  Results are bit for bit those of one workgroup (the
+ * schedules cut the same blocks either way).  The differentiable rollout runs
+ * every instance on one workgroup.  Re-packs the schedules (no new factorisation). */
+int pdx_set_split(pdx_ctx* ctx, int min_vertices);
+/* Number of instances currently split over two workgroups (after pdx_factor or a step). */
+int pdx_split_count(const pdx_ctx* ctx);
+/* Host check of an instance's fast-mode sweep schedules (no device use): replays
+ * them in FP64 in the kernel's phase order - for a split instance, each half seeing
+ * only what its workgroup holds - and fails with PDX_EUNSUPPORTED (message in
+ * pdx_last_error) when a step reads a value before it is final, reads a rank its
+ * workgroup does not hold, leaves a rank unsolved or indexes outside an array.
+ * *rel_err receives max |x - x_direct| / max |x_direct| against a plain LDL^T solve
+ * of the same factor.  No counterpart in the reference. */
+int pdx_verify_schedule(pdx_ctx* ctx, int instance, double* rel_err);
+
 /* --- differentiable rollout and adjoint ---------------------------------------
  * No counterpart in the reference (it has no adjoint); these entry points are the
  * "seed dLoss/dx_T, return dLoss/dv_0, dLoss/dkappa, dLoss/dq(t)" boundary of the

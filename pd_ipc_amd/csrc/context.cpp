@@ -23,7 +23,7 @@ hipError_t launchSubstep(const InstDev* d, const int* order, int ninst, int nsub
                          int flags);
 hipError_t launchAdjoint(const InstDev* d, const int* order, int ninst, int nsub, int subBase, int block,
                          size_t lds, bool fp32, bool blocked, bool idx16, bool wl, hipStream_t s);
-hipError_t launchSolve(const InstDev* d, int inst, const double* b, double* x, int block,
+hipError_t launchSolve(const InstDev* d, int inst, int inst2, const double* b, double* x, int block,
                        size_t lds, bool fp32, bool blocked, bool idx16, bool wl, hipStream_t s);
 hipError_t setLdsLimit(size_t bytes);
 hipError_t launchGrid(const InstDev* d, int inst, int nsub, double* gslots, int waveRuns, int blocks,
@@ -148,10 +148,16 @@ struct HostBlockSweep {
 // per run; each run yields one partial sum.  The tasks of a step (chain blocks,
 // one wavefront each, and batches of up to 64 single rows) then combine their
 // rows' partials in run order and finish the chains.
+//
+// [lo, hi) restricts the schedule to those ranks (the rest is solved in other
+// phases, before this one where it is an input); blocks never cross a rank in
+// `cuts`, so that every row keeps the same external entries, runs and partial
+// sums whichever phase split is packed.
 void packBlocked(const Factor& f, bool fwd, int cap, int runCap, int stepRunCap, int denseCap,
-                 HostBlockSweep& out) {
+                 HostBlockSweep& out, int lo = 0, int hi = -1, const std::vector<int>& cuts = {}) {
   out = HostBlockSweep{};
   const int m = f.m;
+  if (hi < 0) hi = m;
   std::vector<int> parent((size_t)m, -1), nchild((size_t)m, 0);
   for (int j = 0; j < m; ++j)
     if (f.cp[(size_t)j + 1] > f.cp[(size_t)j]) {
@@ -163,10 +169,19 @@ void packBlocked(const Factor& f, bool fwd, int cap, int runCap, int stepRunCap,
     const int len = fwd ? f.rp[(size_t)k + 1] - f.rp[(size_t)k] : f.cp[(size_t)k + 1] - f.cp[(size_t)k];
     return (len + kRun - 1) / kRun;
   };
-  std::vector<int> bs{0};
-  int blockRuns = runsBound(0);
-  for (int k = 1; k < m; ++k) {
-    const bool chain = parent[(size_t)k - 1] == k && nchild[(size_t)k] == 1;
+  if (hi <= lo) {
+    out.stepTask.push_back(0);
+    out.stepRun.push_back(0);
+    out.stepDense.push_back(0);
+    return;
+  }
+  std::vector<char> cut((size_t)m + 1, 0);
+  for (int k : cuts)
+    if (k > 0 && k < m) cut[(size_t)k] = 1;
+  std::vector<int> bs{lo};
+  int blockRuns = runsBound(lo);
+  for (int k = lo + 1; k < hi; ++k) {
+    const bool chain = parent[(size_t)k - 1] == k && nchild[(size_t)k] == 1 && !cut[(size_t)k];
     const int rk = runsBound(k);
     if (!chain || k - bs.back() >= cap || blockRuns + rk > runCap) {
       bs.push_back(k);
@@ -174,9 +189,9 @@ void packBlocked(const Factor& f, bool fwd, int cap, int runCap, int stepRunCap,
     }
     blockRuns += rk;
   }
-  bs.push_back(m);
+  bs.push_back(hi);
   const int nb = (int)bs.size() - 1;
-  std::vector<int> blk((size_t)m);
+  std::vector<int> blk((size_t)m, -1);  // -1: rank of another phase (already final)
   for (int b = 0; b < nb; ++b)
     for (int k = bs[(size_t)b]; k < bs[(size_t)b + 1]; ++k) blk[(size_t)k] = b;
   auto extRange = [&](int k, int b, int& p0, int& p1) {
@@ -200,13 +215,13 @@ void packBlocked(const Factor& f, bool fwd, int cap, int runCap, int stepRunCap,
       extRange(k, b, p0, p1);
       for (int p = p0; p < p1; ++p) {
         const int o = fwd ? f.rc[(size_t)p] : f.ci[(size_t)p];
-        lv = std::max(lv, blev[(size_t)blk[(size_t)o]] + 1);
+        if (blk[(size_t)o] >= 0) lv = std::max(lv, blev[(size_t)blk[(size_t)o]] + 1);
       }
     }
     blev[(size_t)b] = lv;
     nstep = std::max(nstep, lv + 1);
   }
-  out.nstep = m ? nstep : 0;
+  out.nstep = nb ? nstep : 0;
   std::vector<std::vector<int>> byLevel((size_t)out.nstep);
   for (int b = 0; b < nb; ++b) byLevel[(size_t)blev[(size_t)b]].push_back(b);
   auto extLen = [&](int k, int b) {
@@ -374,7 +389,12 @@ struct Instance {
   int factorizeCount = 0, analyzeCount = 0;
   double analyzeSec = 0, factorSec = 0;
   HostSweep hf, hb;
-  HostBlockSweep bf, bb;
+  // fast mode: the forward / backward block schedules.  An instance split over
+  // two workgroups (paired) holds them per region of the nested-dissection top
+  // split: bf / bb over the first half, sf / sb over the separator (the lead's),
+  // pf / pb over the second half (the partner's).
+  HostBlockSweep bf, bb, sf, sb, pf, pb;
+  bool paired = false;
   std::vector<double> inv;  // dense solve mode: inverse of the component matrix, rank space
   bool packed = false;
   // device addresses of the writable state (inside the arena)
@@ -444,6 +464,12 @@ struct pdx_ctx {
   size_t linvBytes = 0;
   bool linvStale = true;
   bool denseLds = false;  // W staged in each workgroup's LDS for the product
+  // instances of at least splitMin vertices run on two workgroups (0 = never);
+  // the differentiable rollout (rollMode) packs every instance for one workgroup
+  int splitMin = 0;
+  bool rollMode = false;
+  int npairs = 0;
+  int* dGiveUp = nullptr;      // the pairs' give-up flag (device)
   // differentiable rollout: pin trajectories (host), device buffers, seeds
   struct Roll {
     int nsub = 0, every = 0, nseg = 0, epoch = -1;
@@ -521,21 +547,41 @@ double sweepBytes(const HostSweep& s, size_t w) {
 }
 
 size_t align16(size_t b) { return (b + 15) / 16 * 16; }
+int maxOverSchedules(const Instance& I, int HostBlockSweep::*field) {
+  const HostBlockSweep* all[6] = {&I.bf, &I.bb, &I.sf, &I.sb, &I.pf, &I.pb};
+  int r = 0;
+  for (const HostBlockSweep* h : all) r = std::max(r, h->*field);
+  return r;
+}
 size_t partBytes(const Instance& I, size_t w) {
-  return align16((size_t)std::max(I.bf.maxRuns, I.bb.maxRuns) * 3 * w);
+  return align16((size_t)maxOverSchedules(I, &HostBlockSweep::maxRuns) * 3 * w);
 }
 // FP64 stages a step's chain inverse blocks in LDS; FP32 loads their rows into
 // registers and stages nothing
 size_t denseBytes(const Instance& I, size_t w) {
-  return w == 8 ? align16((size_t)std::max(I.bf.maxDense, I.bb.maxDense) * w) : 0;
+  return w == 8 ? align16((size_t)maxOverSchedules(I, &HostBlockSweep::maxDense) * w) : 0;
 }
+// step tables of a workgroup's four sweeps (the kernel's loadStepTables): the
+// lead (or a single workgroup) bf, bb, sf, sb; a pair's partner pf, pb, sf, sb
 size_t stepBytes(const Instance& I) {
-  return align16((size_t)3 * (I.bf.nstep + 1 + I.bb.nstep + 1) * sizeof(int));
+  const size_t sep = (size_t)(I.sf.nstep + 1 + I.sb.nstep + 1);
+  const size_t lead = (size_t)(I.bf.nstep + 1 + I.bb.nstep + 1) + sep;
+  const size_t part = (size_t)(I.pf.nstep + 1 + I.pb.nstep + 1) + sep;
+  return align16(3 * std::max(lead, part) * sizeof(int));
 }
 
 double blockBytes(const HostBlockSweep& s, size_t w) {
   return (double)s.runCol.size() * (4 + w) + (double)s.dense.size() * w + (double)s.slotRank.size() * 12 +
          (double)s.taskInfo.size() * 12 + (double)s.stepTask.size() * 12;
+}
+
+// Whether an instance runs on two workgroups: the fast FP32-solve batch form with
+// W in LDS, a nested-dissection factor with a top-level split, no Chebyshev
+// blend, outside the differentiable rollout, and at least splitMin vertices.
+bool wantPair(const pdx_ctx* c, const Instance& I, bool wCanFit) {
+  return c->splitMin > 0 && !c->rollMode && c->mode == PDX_SOLVE_FAST && c->prec == PDX_PREC_MIXED32 &&
+         c->launchMode == 0 && wCanFit && I.cfg.chebRho <= 0.0 && I.scene.n >= c->splitMin &&
+         I.fac.top[0] > 0 && I.fac.top[1] > 0 && I.fac.top[0] + I.fac.top[1] < I.fac.m;
 }
 
 // Factor every instance whose stamps changed (parallel over instances), repack.
@@ -575,19 +621,36 @@ int refreshFactors(pdx_ctx* c, bool& changed) {
           c->linvStale = true;
         } else if (c->mode == PDX_SOLVE_FAST) {
           const int w = c->prec == PDX_PREC_MIXED32 ? 4 : 8;
+          // blocks never cross the top split of a nested-dissection order, so
+          // that a row's runs (and bits) are the same in one workgroup or two
+          const int m = I.fac.m, rA = I.fac.top[0], rB = I.fac.top[0] + I.fac.top[1];
+          const std::vector<int> cuts = rA > 0 ? std::vector<int>{rA, rB} : std::vector<int>{};
+          I.paired = wantPair(c, I, wCanFit);
+          for (HostBlockSweep* h : {&I.sf, &I.sb, &I.pf, &I.pb}) *h = HostBlockSweep{};
+          auto packAll = [&](int runCap, int stepRunCap, int denseCap) {
+            if (I.paired) {
+              packBlocked(I.fac, true, c->chainCap, runCap, stepRunCap, denseCap, I.bf, 0, rA, cuts);
+              packBlocked(I.fac, false, c->chainCap, runCap, stepRunCap, denseCap, I.bb, 0, rA, cuts);
+              packBlocked(I.fac, true, c->chainCap, runCap, stepRunCap, denseCap, I.pf, rA, rB, cuts);
+              packBlocked(I.fac, false, c->chainCap, runCap, stepRunCap, denseCap, I.pb, rA, rB, cuts);
+              packBlocked(I.fac, true, c->chainCap, runCap, stepRunCap, denseCap, I.sf, rB, m, cuts);
+              packBlocked(I.fac, false, c->chainCap, runCap, stepRunCap, denseCap, I.sb, rB, m, cuts);
+            } else {
+              packBlocked(I.fac, true, c->chainCap, runCap, stepRunCap, denseCap, I.bf, 0, m, cuts);
+              packBlocked(I.fac, false, c->chainCap, runCap, stepRunCap, denseCap, I.bb, 0, m, cuts);
+            }
+          };
           if (c->launchMode == 1) {
             // grid form: a task's runs fit one wavefront's LDS slice; steps are
             // not staged in LDS
             const int big = 1 << 29;
-            packBlocked(I.fac, true, c->chainCap, c->waveRuns, big, big, I.bf);
-            packBlocked(I.fac, false, c->chainCap, c->waveRuns, big, big, I.bb);
+            packAll(c->waveRuns, big, big);
           } else {
             // per-step LDS caps: run partials <= 48 KB; FP64 stages the chain
             // blocks' inverses too (<= 32 KB), FP32 loads them into registers
             const bool stage = w == 8;
             int runCap = 48 * 1024 / (3 * w), denseCap = stage ? 32 * 1024 / w : 1 << 29;
-            packBlocked(I.fac, true, c->chainCap, runCap, runCap, denseCap, I.bf);
-            packBlocked(I.fac, false, c->chainCap, runCap, runCap, denseCap, I.bb);
+            packAll(runCap, runCap, denseCap);
             // The workgroup also holds W (3m).  When W and the staging overflow
             // LDS, the steps are cut finer to the room W leaves (3:2 between run
             // partials and staged blocks), shrinking by the overflow until they fit.
@@ -597,8 +660,7 @@ int refreshFactors(pdx_ctx* c, bool& changed) {
             for (int tries = 0; wCanFit && need() > kLdsLimit && room >= 8 * 1024 && tries < 8; ++tries) {
               runCap = std::min(runCap, (int)((stage ? room * 3 / 5 : room) / (3 * w)));
               if (stage) denseCap = std::min(denseCap, (int)(room * 2 / 5 / w));
-              packBlocked(I.fac, true, c->chainCap, runCap, runCap, denseCap, I.bf);
-              packBlocked(I.fac, false, c->chainCap, runCap, runCap, denseCap, I.bb);
+              packAll(runCap, runCap, denseCap);
               const size_t over = need() > kLdsLimit ? need() - kLdsLimit : 0;
               room = room > over + 1024 ? room - over - 1024 : 0;
             }
@@ -701,12 +763,17 @@ int upload(pdx_ctx* c) {
   }
   if (c->lds > kLdsLimit)
     return fail(c, PDX_EUNSUPPORTED, "sweep staging does not fit in LDS (" + std::to_string(c->lds) + " B)");
+  for (auto& I : c->inst)
+    if (I.paired && !c->ldsW)
+      return fail(c, PDX_EUNSUPPORTED, "an instance split over two workgroups needs W in LDS");
   Arena ar;
   struct Off {
     size_t x, v, xbs, xhat, prev, mass, pinned, pinpos, brhs, ea, eb, rest, kappa, vstart, vedges, vedgesC,
         bstart, bst, sa, sb, sc, bk, vrank, rankv, dinv, W, ist, lst, dst, recE, rkE, recC, rkC, brR, brK;
     size_t f[9], b[9];
-    size_t bf[15], bb[15];
+    size_t sch[6][15];  // block schedules bf, bb, sf, sb, pf, pb
+    size_t ownLead = 0, ownPart = 0, pairFlags = 0, pairRed = 0, xbuf = 0;
+    int nOwnLead = 0, nOwnPart = 0;
   };
   std::vector<Off> offs(c->inst.size());
   c->bytesIter = 0;
@@ -797,10 +864,10 @@ int upload(pdx_ctx* c) {
       fo[q][6] = ar.vec(h.col);
       fo[q][7] = fp32 ? ar.vec(castVec<float>(h.val)) : ar.vec(h.val);
     }
-    HostBlockSweep* bsw[2] = {&I.bf, &I.bb};
-    size_t* bo[2] = {o.bf, o.bb};
-    for (int q = 0; q < 2; ++q) {
+    HostBlockSweep* bsw[6] = {&I.bf, &I.bb, &I.sf, &I.sb, &I.pf, &I.pb};
+    for (int q = 0; q < 6; ++q) {
       HostBlockSweep& h = *bsw[q];
+      size_t* bo[6] = {o.sch[0], o.sch[1], o.sch[2], o.sch[3], o.sch[4], o.sch[5]};
       bo[q][0] = ar.vec(h.stepTask);
       bo[q][1] = ar.vec(h.stepRun);
       bo[q][2] = ar.vec(h.stepDense);
@@ -820,6 +887,23 @@ int upload(pdx_ctx* c) {
         std::vector<uint16_t> c16(h.runCol.begin(), h.runCol.end());
         bo[q][9] = ar.vec(c16);
       }
+    }
+    if (I.paired) {
+      // owned vertices (ascending): the lead takes the first half and the
+      // separator, the partner the second half; the exchange area of the pair
+      const int rA = I.fac.top[0], rB = I.fac.top[0] + I.fac.top[1];
+      std::vector<int> lead, part;
+      for (int v = 0; v < s.n; ++v) {
+        const int r = I.fac.vertexRank[0][(size_t)v];
+        (r >= rA && r < rB ? part : lead).push_back(v);
+      }
+      o.nOwnLead = (int)lead.size();
+      o.nOwnPart = (int)part.size();
+      o.ownLead = ar.vec(lead);
+      o.ownPart = ar.vec(part);
+      o.pairFlags = ar.add(nullptr, 64 * sizeof(int));
+      o.pairRed = ar.add(nullptr, 16 * sizeof(double));
+      o.xbuf = ar.add(nullptr, (size_t)3 * I.fac.m * w);
     }
     o.W = c->ldsW ? 0 : ar.add(nullptr, (size_t)3 * I.fac.m * w);
     std::vector<int> ist(4, 0);
@@ -850,13 +934,39 @@ int upload(pdx_ctx* c) {
   }
   // grid reductions: 2 alternating sets of 2 slots per workgroup, at most 1024 workgroups
   const size_t slotOff = ar.add(nullptr, 4 * 1024 * sizeof(double));
-  const size_t instOff = ar.add(nullptr, sizeof(InstDev) * c->inst.size());
-  // LPT order: heaviest instance (factor entries) first
-  std::vector<int> order(c->inst.size());
-  for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
-  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
-    return c->inst[(size_t)a].fac.nnz > c->inst[(size_t)b].fac.nnz;
-  });
+  const size_t giveUpOff = ar.add(nullptr, sizeof(int));
+  // descriptors: one per instance (its only workgroup, or a pair's lead), then one
+  // per pair for the partner
+  std::vector<int> pairOf;  // instance of each pair
+  for (size_t i = 0; i < c->inst.size(); ++i)
+    if (c->inst[i].paired) pairOf.push_back((int)i);
+  const int ni = (int)c->inst.size(), np = (int)pairOf.size();
+  const size_t instOff = ar.add(nullptr, sizeof(InstDev) * (size_t)(ni + np));
+  // Launch order (workgroup b runs descriptor order[b]), longest first by factor
+  // entries.  The pairs come first, each lead and its partner 8 workgroups apart
+  // (16-workgroup groups: up to 8 leads, 8 partners), so that with the
+  // dispatcher's round-robin over the 8 XCDs a pair shares one L2; the groups'
+  // unused places take the heaviest single instances.
+  std::vector<int> order, solo;
+  for (int i = 0; i < ni; ++i)
+    if (!c->inst[(size_t)i].paired) solo.push_back(i);
+  auto heavier = [&](int a, int b) { return c->inst[(size_t)a].fac.nnz > c->inst[(size_t)b].fac.nnz; };
+  std::stable_sort(solo.begin(), solo.end(), heavier);
+  std::vector<int> pq(np);
+  for (int p = 0; p < np; ++p) pq[(size_t)p] = p;
+  std::stable_sort(pq.begin(), pq.end(), [&](int a, int b) { return heavier(pairOf[(size_t)a], pairOf[(size_t)b]); });
+  size_t si = 0;
+  for (int p0 = 0; p0 < np; p0 += 8) {
+    const int q = std::min(8, np - p0);
+    for (int half = 0; half < 2; ++half) {
+      for (int j = 0; j < q; ++j) {
+        const int p = pq[(size_t)(p0 + j)];
+        order.push_back(half == 0 ? pairOf[(size_t)p] : ni + p);
+      }
+      for (int j = q; j < 8 && si < solo.size(); ++j) order.push_back(solo[si++]);
+    }
+  }
+  while (si < solo.size()) order.push_back(solo[si++]);
   const size_t orderOff = ar.vec(order);
 
   HIPCHECK(c, hipSetDevice(c->device));
@@ -900,7 +1010,8 @@ int upload(pdx_ctx* c) {
       c->linvStale = false;
     }
   }
-  c->hInst.assign(c->inst.size(), InstDev{});
+  c->hInst.assign((size_t)(ni + np), InstDev{});
+  int* dGiveUp = (int*)(B + giveUpOff);
   for (size_t i = 0; i < c->inst.size(); ++i) {
     Instance& I = c->inst[i];
     const Off& o = offs[i];
@@ -973,28 +1084,29 @@ int upload(pdx_ctx* c) {
       sd[q]->col = (const int*)(B + fo[q][6]);
       sd[q]->val = B + fo[q][7];
     }
-    BlockSweepDev* bd[2] = {&d.bfwd, &d.bbwd};
-    const size_t* bo[2] = {o.bf, o.bb};
-    const HostBlockSweep* bsw[2] = {&I.bf, &I.bb};
-    for (int q = 0; q < 2; ++q) {
-      bd[q]->nstep = bsw[q]->nstep;
-      bd[q]->cap = c->chainCap;
-      bd[q]->stepTask = (const int*)(B + bo[q][0]);
-      bd[q]->stepRun = (const int*)(B + bo[q][1]);
-      bd[q]->stepDense = (const int*)(B + bo[q][2]);
-      bd[q]->taskInfo = (const int*)(B + bo[q][3]);
-      bd[q]->taskSlot = (const int*)(B + bo[q][4]);
-      bd[q]->taskDense = (const int*)(B + bo[q][5]);
-      bd[q]->slotRank = (const int*)(B + bo[q][6]);
-      bd[q]->slotRunBeg = (const int*)(B + bo[q][7]);
-      bd[q]->slotRunEnd = (const int*)(B + bo[q][8]);
-      bd[q]->runCol = (const void*)(B + bo[q][9]);
-      bd[q]->runVal = B + bo[q][10];
-      bd[q]->dense = B + bo[q][11];
-      bd[q]->laneMeta = (const int*)(B + bo[q][12]);
-      bd[q]->laneDinv = B + bo[q][13];
-      bd[q]->laneDense = (const int*)(B + bo[q][14]);
-    }
+    auto fillSweep = [&](BlockSweepDev* bd, const size_t* bo, const HostBlockSweep& h) {
+      bd->nstep = h.nstep;
+      bd->cap = c->chainCap;
+      bd->stepTask = (const int*)(B + bo[0]);
+      bd->stepRun = (const int*)(B + bo[1]);
+      bd->stepDense = (const int*)(B + bo[2]);
+      bd->taskInfo = (const int*)(B + bo[3]);
+      bd->taskSlot = (const int*)(B + bo[4]);
+      bd->taskDense = (const int*)(B + bo[5]);
+      bd->slotRank = (const int*)(B + bo[6]);
+      bd->slotRunBeg = (const int*)(B + bo[7]);
+      bd->slotRunEnd = (const int*)(B + bo[8]);
+      bd->runCol = (const void*)(B + bo[9]);
+      bd->runVal = B + bo[10];
+      bd->dense = B + bo[11];
+      bd->laneMeta = (const int*)(B + bo[12]);
+      bd->laneDinv = B + bo[13];
+      bd->laneDense = (const int*)(B + bo[14]);
+    };
+    fillSweep(&d.bfwd, o.sch[0], I.bf);
+    fillSweep(&d.bbwd, o.sch[1], I.bb);
+    fillSweep(&d.bfwdS, o.sch[2], I.sf);
+    fillSweep(&d.bbwdS, o.sch[3], I.sb);
     d.partOff = (int)(kRedBytes + (c->ldsW ? (size_t)3 * I.fac.m * w : 0));
     d.denseOff = d.partOff + (int)(partBytes(I, w));
     d.stepOff = d.denseOff + (int)(denseBytes(I, w));
@@ -1005,6 +1117,29 @@ int upload(pdx_ctx* c) {
     d.istats = (int*)(B + o.ist);
     d.lstats = (long long*)(B + o.lst);
     d.dstats = (double*)(B + o.dst);
+    d.pairRole = 0;
+    d.own = nullptr;
+    d.nOwn = s.n;
+    if (I.paired) {
+      d.rA = I.fac.top[0];
+      d.rB = I.fac.top[0] + I.fac.top[1];
+      d.pair.flags = (int*)(B + o.pairFlags);
+      d.pair.giveUp = dGiveUp;
+      d.pair.red = (double*)(B + o.pairRed);
+      d.pair.xbuf = (void*)(B + o.xbuf);
+      // the partner: the second half's sweeps and vertices
+      const int p = (int)(std::find(pairOf.begin(), pairOf.end(), (int)i) - pairOf.begin());
+      InstDev& e = c->hInst[(size_t)(ni + p)];
+      e = d;
+      e.pairRole = 2;
+      e.own = (const int*)(B + o.ownPart);
+      e.nOwn = o.nOwnPart;
+      fillSweep(&e.bfwd, o.sch[4], I.pf);
+      fillSweep(&e.bbwd, o.sch[5], I.pb);  // the separator's sweeps as the lead's
+      d.pairRole = 1;
+      d.own = (const int*)(B + o.ownLead);
+      d.nOwn = o.nOwnLead;
+    }
     I.dx = d.x;
     I.dv = d.vel;
     I.dpin = (double*)d.pinpos;
@@ -1015,6 +1150,8 @@ int upload(pdx_ctx* c) {
   }
   std::memcpy(ar.host.data() + instOff, c->hInst.data(), sizeof(InstDev) * c->hInst.size());
   HIPCHECK(c, hipMemcpy(c->dbuf, ar.host.data(), ar.host.size(), hipMemcpyHostToDevice));
+  c->npairs = np;
+  c->dGiveUp = dGiveUp;
   c->dInst = (InstDev*)(B + instOff);
   c->dGridSlots = (double*)(B + slotOff);
   c->dOrder = (int*)(B + orderOff);
@@ -1130,11 +1267,154 @@ int launch(pdx_ctx* c, int nsub) {
     return PDX_OK;
   }
   HIPCHECK(c, hipEventRecord(c->ev0, c->stream));
-  HIPCHECK(c, launchSubstep(c->dInst, c->dOrder, (int)c->inst.size(), nsub, c->block, c->lds,
+  HIPCHECK(c, launchSubstep(c->dInst, c->dOrder, (int)c->hInst.size(), nsub, c->block, c->lds,
                             c->prec == PDX_PREC_MIXED32, c->mode == PDX_SOLVE_FAST, c->idx16, c->ldsW, c->stream, 0, 0));
   HIPCHECK(c, hipEventRecord(c->ev1, c->stream));
   c->lastLaunches = 1;
   return PDX_OK;
+}
+
+// Host replay of an instance's packed block schedules (fast mode), in the order
+// and with the visibility the kernel has: one right-hand side, FP64, the device's
+// step / run / task / segment structure.  Phases (a split instance): forward over
+// the first half seeing only it, over the second half seeing only it, the
+// separator forward and backward seeing everything, then backward over each half
+// seeing it and the separator; one workgroup: forward and backward over all ranks.
+// A read of a rank the phase does not hold, or of a value not final yet in the
+// sweep's sense, or an index outside its array is an error (returned in `why`).
+// relErr: max |x - x_direct| / max |x_direct| against a plain LDL^T solve.
+bool replaySchedules(const Instance& I, std::string& why, double& relErr) {
+  const Factor& f = I.fac;
+  const int m = f.m;
+  const int rA = I.paired ? f.top[0] : m, rB = I.paired ? f.top[0] + f.top[1] : m;
+  std::vector<double> b((size_t)m), W((size_t)m);
+  for (int k = 0; k < m; ++k) b[(size_t)k] = std::sin(0.37 * k + 1.0) + 0.01 * (k % 7);
+  W = b;
+  std::vector<int> state((size_t)m, 0);  // 0 right-hand side, 1 forward final, 2 backward final
+  auto region = [&](int k) { return k < rA ? 0 : (k < rB ? 1 : 2); };
+  auto phase = [&](const HostBlockSweep& h, bool fwd, int lo, int hi, bool seeA, bool seeB, bool seeS,
+                   const char* name) -> bool {
+    const int need = fwd ? 1 : 2;
+    auto visible = [&](int k) {
+      const int g = region(k);
+      return g == 0 ? seeA : (g == 1 ? seeB : seeS);
+    };
+    auto fault = [&](const std::string& what) {
+      why = std::string(name) + ": " + what;
+      return false;
+    };
+    const size_t nrun = h.runCol.size() / kRun;
+    if (h.stepTask.size() != (size_t)h.nstep + 1 || h.stepRun.size() != (size_t)h.nstep + 1 ||
+        h.stepDense.size() != (size_t)h.nstep + 1)
+      return fault("step tables of the wrong length");
+    std::vector<int> done;
+    for (int st = 0; st < h.nstep; ++st) {
+      const int r0 = h.stepRun[(size_t)st], r1 = h.stepRun[(size_t)st + 1];
+      if (r0 < 0 || r1 < r0 || (size_t)r1 > nrun) return fault("run range out of bounds");
+      std::vector<double> part((size_t)(r1 - r0));
+      for (int r = r0; r < r1; ++r) {
+        double acc = 0.0;
+        for (int q = 0; q < kRun; ++q) {
+          const int col = h.runCol[(size_t)r * kRun + q];
+          const double v = h.runVal[(size_t)r * kRun + q];
+          if (col < 0 || col >= m) return fault("run column out of range");
+          if (v == 0.0) continue;  // padding
+          if (!visible(col)) return fault("reads rank " + std::to_string(col) + " of another workgroup");
+          if (state[(size_t)col] != need) return fault("reads rank " + std::to_string(col) + " before it is final");
+          acc += v * W[(size_t)col];
+        }
+        part[(size_t)(r - r0)] = acc;
+      }
+      const int t0 = h.stepTask[(size_t)st], t1 = h.stepTask[(size_t)st + 1];
+      if (t0 < 0 || t1 < t0 || (size_t)t1 > h.taskInfo.size()) return fault("task range out of bounds");
+      const int d0 = h.stepDense[(size_t)st];
+      done.clear();
+      for (int t = t0; t < t1; ++t) {
+        double a[64];
+        int kk[64], info[64], doff[64];
+        for (int ln = 0; ln < 64; ++ln) {
+          const size_t mi = ((size_t)t * 64 + ln) * 4;
+          if (mi + 3 >= h.laneMeta.size()) return fault("lane metadata out of bounds");
+          kk[ln] = h.laneMeta[mi];
+          info[ln] = h.laneMeta[mi + 3];
+          doff[ln] = h.laneDense[(size_t)t * 64 + ln];
+          a[ln] = 0.0;
+          const int k = kk[ln];
+          if (k < 0) continue;
+          if (k < lo || k >= hi) return fault("finishes rank " + std::to_string(k) + " outside the phase");
+          if (state[(size_t)k] != need - 1) return fault("finishes rank " + std::to_string(k) + " twice");
+          double x = W[(size_t)k];
+          if (!fwd) x = x * f.dinv[(size_t)k];
+          const int qb = h.laneMeta[mi + 1], qe = h.laneMeta[mi + 2];
+          if (qb < 0 || qe < qb || qe > r1 - r0) return fault("lane's runs outside its step");
+          for (int q = qb; q < qe; ++q) x -= part[(size_t)q];
+          a[ln] = x;
+        }
+        double y[64];
+        for (int ln = 0; ln < 64; ++ln) {
+          y[ln] = a[ln];
+          if (kk[ln] < 0) continue;
+          const int L = info[ln] & 0x7f, o = (info[ln] >> 7) & 0x7f;
+          if (L <= 1) continue;
+          if (ln - o < 0 || ln - o + L > 64) return fault("segment outside its task");
+          double acc = 0.0;
+          for (int s2 = 0; s2 < L; ++s2) {
+            const size_t di = (size_t)d0 + (size_t)doff[ln] + (size_t)s2 * L + o;
+            if (di >= h.dense.size()) return fault("segment inverse out of bounds");
+            acc += h.dense[di] * a[ln - o + s2];
+          }
+          y[ln] = acc;
+        }
+        for (int ln = 0; ln < 64; ++ln)
+          if (kk[ln] >= 0) {
+            W[(size_t)kk[ln]] = y[ln];
+            done.push_back(kk[ln]);
+          }
+      }
+      for (int k : done) state[(size_t)k] = need;  // final for the next step
+    }
+    for (int k = lo; k < hi; ++k)
+      if (state[(size_t)k] != need) return fault("leaves rank " + std::to_string(k) + " unfinished");
+    return true;
+  };
+  bool ok;
+  if (I.paired) {
+    ok = phase(I.bf, true, 0, rA, true, false, false, "forward, first half") &&
+         phase(I.pf, true, rA, rB, false, true, false, "forward, second half") &&
+         phase(I.sf, true, rB, m, true, true, true, "forward, separator") &&
+         phase(I.sb, false, rB, m, true, true, true, "backward, separator") &&
+         phase(I.bb, false, 0, rA, true, false, true, "backward, first half") &&
+         phase(I.pb, false, rA, rB, false, true, true, "backward, second half");
+  } else {
+    ok = phase(I.bf, true, 0, m, true, true, true, "forward") && phase(I.bb, false, 0, m, true, true, true, "backward");
+  }
+  if (!ok) return false;
+  // the direct solve: L y = b (rows ascending), y /= D, L^T x = y
+  std::vector<double> x = b;
+  for (int k = 0; k < m; ++k)
+    for (int p = f.rp[(size_t)k]; p < f.rp[(size_t)k + 1]; ++p) x[(size_t)k] -= f.rx[(size_t)p] * x[(size_t)f.rc[(size_t)p]];
+  for (int k = 0; k < m; ++k) x[(size_t)k] *= f.dinv[(size_t)k];
+  for (int k = m - 1; k >= 0; --k)
+    for (int p = f.cp[(size_t)k]; p < f.cp[(size_t)k + 1]; ++p) x[(size_t)k] -= f.cx[(size_t)p] * x[(size_t)f.ci[(size_t)p]];
+  double num = 0.0, den = 0.0;
+  for (int k = 0; k < m; ++k) {
+    num = std::max(num, std::fabs(W[(size_t)k] - x[(size_t)k]));
+    den = std::max(den, std::fabs(x[(size_t)k]));
+  }
+  relErr = den > 0.0 ? num / den : num;
+  return true;
+}
+
+// After a launch of split instances: did a pair stop waiting for its partner?
+int checkPairs(pdx_ctx* c) {
+  if (c->npairs == 0 || !c->uploaded || !c->dGiveUp) return PDX_OK;
+  int gaveUp = 0;
+  HIPCHECK(c, hipMemcpy(&gaveUp, c->dGiveUp, sizeof(int), hipMemcpyDeviceToHost));
+  if (!gaveUp) return PDX_OK;
+  // the pairs' counts no longer agree: the next step re-uploads (counts zero)
+  c->dirty = true;
+  return fail(c, PDX_EHIP, "two workgroups of a split instance stopped waiting for each other "
+                           "(the launch's results are incomplete)");
 }
 
 }  // namespace
@@ -1220,9 +1500,25 @@ int pdx_factor(pdx_ctx* c) {
   return ensureReady(c);
 }
 
+// Stepping after a differentiable rollout: instances may split over two
+// workgroups again (pdx_set_split).
+static int leaveRollMode(pdx_ctx* c) {
+  if (!c->rollMode) return PDX_OK;
+  c->rollMode = false;
+  if (c->splitMin > 0) {
+    const int rc = pullState(c);
+    if (rc) return rc;
+    for (auto& I : c->inst) I.packed = false;
+    c->dirty = true;
+  }
+  return PDX_OK;
+}
+
 int pdx_step_async(pdx_ctx* c, int nsub) {
   if (!c || nsub < 0) return PDX_EINVAL;
-  int rc = ensureReady(c);
+  int rc = leaveRollMode(c);
+  if (rc) return rc;
+  rc = ensureReady(c);
   if (rc) return rc;
   if (nsub == 0) return PDX_OK;
   return launch(c, nsub);
@@ -1235,7 +1531,7 @@ int pdx_sync(pdx_ctx* c) {
   float ms = 0.f;
   if (c->lastLaunches > 0 && hipEventElapsedTime(&ms, c->ev0, c->ev1) == hipSuccess)
     c->lastKernelMs = ms / c->lastLaunches;
-  return PDX_OK;
+  return checkPairs(c);
 }
 
 int pdx_step(pdx_ctx* c, int nsub) {
@@ -1246,7 +1542,9 @@ int pdx_step(pdx_ctx* c, int nsub) {
 
 int pdx_step_graph(pdx_ctx* c, int nsub, int replays) {
   if (!c || nsub <= 0 || replays < 0) return PDX_EINVAL;
-  int rc = ensureReady(c);
+  int rc = leaveRollMode(c);
+  if (rc) return rc;
+  rc = ensureReady(c);
   if (rc) return rc;
   if (cooperative(c)) {
     // cooperative launches are replayed directly (not captured)
@@ -1267,7 +1565,7 @@ int pdx_step_graph(pdx_ctx* c, int nsub, int replays) {
     }
   } gg;
   HIPCHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-  const hipError_t le = launchSubstep(c->dInst, c->dOrder, (int)c->inst.size(), nsub, c->block, c->lds,
+  const hipError_t le = launchSubstep(c->dInst, c->dOrder, (int)c->hInst.size(), nsub, c->block, c->lds,
                                       c->prec == PDX_PREC_MIXED32, c->mode == PDX_SOLVE_FAST, c->idx16, c->ldsW, c->stream,
                                       0, 0);
   const hipError_t ce = hipStreamEndCapture(c->stream, &gg.g);
@@ -1457,12 +1755,20 @@ int pdx_solve3(pdx_ctx* c, int i, const double* b, double* x) {
     HIPCHECK(c, launchGridSolve(c->dInst, i, c->dScratchB, c->dScratchX, c->waveRuns, g, c->block, c->lds,
                                 c->prec == PDX_PREC_MIXED32, c->idx16, c->stream));
   } else {
-    HIPCHECK(c, launchSolve(c->dInst, i, c->dScratchB, c->dScratchX, c->block, c->lds,
+    // a split instance solves on its two workgroups (the partner's descriptor
+    // follows the instances' in pair order)
+    int partner = -1;
+    if (I.paired) {
+      int p = 0;
+      for (int j = 0; j < i; ++j) p += c->inst[(size_t)j].paired ? 1 : 0;
+      partner = (int)c->inst.size() + p;
+    }
+    HIPCHECK(c, launchSolve(c->dInst, i, partner, c->dScratchB, c->dScratchX, c->block, c->lds,
                             c->prec == PDX_PREC_MIXED32, c->mode == PDX_SOLVE_FAST, c->idx16, c->ldsW, c->stream));
   }
   HIPCHECK(c, hipMemcpyAsync(x, c->dScratchX, (size_t)n3 * 8, hipMemcpyDeviceToHost, c->stream));
   HIPCHECK(c, hipStreamSynchronize(c->stream));
-  return PDX_OK;
+  return checkPairs(c);
 }
 
 int pdx_get_factor(pdx_ctx* c, int i, int* m, long long* nnz, int32_t* ptr, int32_t* rows, double* vals,
@@ -1577,8 +1883,21 @@ int pdx_rollout(pdx_ctx* c, int nsub, int every) {
   for (auto& I : c->inst)
     if (I.cfg.chebRho > 0.0)
       return fail(c, PDX_EUNSUPPORTED, "the differentiable rollout differentiates the plain iteration (no Chebyshev blend)");
+  if (!c->rollMode) {
+    // the differentiable rollout runs every instance on one workgroup
+    c->rollMode = true;
+    bool anyPaired = false;
+    for (auto& I : c->inst) anyPaired = anyPaired || I.paired;
+    if (anyPaired) {
+      const int prc = pullState(c);
+      if (prc) return prc;
+      for (auto& I : c->inst) I.packed = false;
+      c->dirty = true;
+    }
+  }
   int rc = ensureReady(c);
   if (rc) return rc;
+  if (c->hInst.size() != c->inst.size()) return fail(c, PDX_EUNSUPPORTED, "rollout: an instance is still split");
   const size_t ni = c->inst.size();
   c->roll.traj.resize(ni);
   c->roll.trajSub.resize(ni, 0);
@@ -1891,6 +2210,40 @@ int pdx_set_schedule(pdx_ctx* c, int block_threads, int chain_cap) {
   }
   c->block = block_threads;
   return PDX_OK;
+}
+
+int pdx_set_split(pdx_ctx* c, int min_vertices) {
+  if (!c || min_vertices < 0) return PDX_EINVAL;
+  if (min_vertices == c->splitMin) return PDX_OK;
+  const int rc = pullState(c);
+  if (rc) return rc;
+  c->splitMin = min_vertices;
+  for (auto& I : c->inst) I.packed = false;  // repack the same factors
+  c->dirty = true;
+  return PDX_OK;
+}
+
+int pdx_verify_schedule(pdx_ctx* c, int i, double* rel_err) {
+  if (!c || i < 0 || i >= (int)c->inst.size()) return PDX_EINVAL;
+  if (c->mode != PDX_SOLVE_FAST) return fail(c, PDX_EUNSUPPORTED, "only the fast mode has block schedules");
+  Instance& I = c->inst[(size_t)i];
+  if (!I.factored || !I.packed || numericStamp(I.scene, I.cfg.dt) != I.numStamp) {
+    bool changed = false;
+    const int rc = refreshFactors(c, changed);
+    if (rc) return rc;
+  }
+  std::string why;
+  double rel = 0.0;
+  if (!replaySchedules(I, why, rel)) return fail(c, PDX_EUNSUPPORTED, "schedule replay: " + why);
+  if (rel_err) *rel_err = rel;
+  return PDX_OK;
+}
+
+int pdx_split_count(const pdx_ctx* c) {
+  if (!c) return PDX_EINVAL;
+  int n = 0;
+  for (const auto& I : c->inst) n += I.paired ? 1 : 0;
+  return n;
 }
 
 int pdx_set_ordering(pdx_ctx* c, int ordering) {

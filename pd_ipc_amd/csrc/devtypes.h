@@ -68,6 +68,17 @@ struct BlockSweepDev {
                           // forward [s * L + t] = X(t, s), backward [s * L + t] = X(s, t)
 };
 
+// Exchange area of an instance split over two workgroups (a pair).  Each
+// workgroup counts the synchronisation points it has passed in flags[role]; a
+// workgroup that waits longer than a bound for its partner sets *giveUp (read by
+// the host after the launch) and both stop.
+struct PairDev {
+  int* flags;      // lead's count at flags[0], partner's at flags[32] (separate cache lines)
+  int* giveUp;     // one per context
+  double* red;     // [2 buffers][2 roles][4]: scale, displacement, residual maxima
+  void* xbuf;      // 3m Real: the halves' solution values handed across (rank space)
+};
+
 struct InstDev {
   int n, E, B, m;
   int maxIter, wantResidual, hasBendRhs, ldsW;
@@ -144,6 +155,16 @@ struct InstDev {
   double* gkap;            // dLoss/dkappa_e (E)
   double* gbend;           // dLoss/dk_s (B)
   double* gpin;            // dLoss/dq [substep][pin][3]
+  // Split over two workgroups (nested-dissection order, FP32 solves, W in LDS):
+  // ranks [0, rA) are the first half of the top-level split, [rA, rB) the second,
+  // [rB, m) its separator.  The lead (pairRole 1) owns the vertices of the first
+  // half and of the separator and sweeps bfwd/bbwd over the first half and
+  // bfwdS/bbwdS over the separator; the partner (pairRole 2) owns the second half
+  // and sweeps bfwd/bbwd over it.  pairRole 0: one workgroup, own == null.
+  int pairRole, nOwn, rA, rB;
+  const int* own;     // owned vertices, ascending
+  BlockSweepDev bfwdS, bbwdS;
+  PairDev pair;
   // statistics
   int* istats;        // [0] iterations used, [1] converged, [2] early exits
   long long* lstats;  // [0] total iterations, [1] substeps, [2..6] phase ticks

@@ -455,11 +455,12 @@ int analyzeAndFactor(const Scene& s, const Matrix& A, Factor& f, int ordering) {
   if (ordering == kOrderNd) {
     // the three components of a vertex share its place in the vertex order
     std::vector<int> vo;
-    ndOrder(A, s.x, kNdLeaf, vo);
+    ndOrder(A, s.x, kNdLeaf, vo, f.top);
     f.perm3.resize((size_t)n3);
     for (int k = 0; k < n; ++k)
       for (int d = 0; d < 3; ++d) f.perm3[(size_t)3 * k + d] = 3 * vo[(size_t)k] + d;
   } else {
+    f.top[0] = f.top[1] = 0;
     std::vector<int> cp((size_t)n3 + 1, 0), ri;
     ri.reserve((size_t)3 * A.rows.size());
     for (int vc = 0; vc < n; ++vc)

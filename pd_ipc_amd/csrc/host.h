@@ -51,8 +51,12 @@ void bendingRhs(const Scene& s, std::vector<double>& out);
 void amdOrder(int n, const std::vector<int>& colptr, const std::vector<int>& rowidx,
               std::vector<int>& perm);
 // Nested-dissection vertex order of A's graph by coordinate bisection of the
-// positions x (3n); parts of at most `leaf` vertices stay in sorted order.
-void ndOrder(const Matrix& A, const std::vector<double>& x, int leaf, std::vector<int>& order);
+// positions x (3n); parts of at most `leaf` vertices stay in sorted order.  The
+// order lists the first half of the top-level split, then the second, then the
+// separator; when `top` is given it receives the two halves' sizes (0, 0 when
+// the graph was a single leaf).  No matrix entry couples the two halves.
+void ndOrder(const Matrix& A, const std::vector<double>& x, int leaf, std::vector<int>& order,
+             int* top = nullptr);
 
 // Level-scheduled sweep of a unit triangular factor (rank space).
 struct Sweep {
@@ -78,6 +82,10 @@ struct Factor {
   std::vector<double> dinv;
   int64_t nnz = 0;
   Sweep fwd, bwd;
+  // nested-dissection order: ranks [0, top[0]) and [top[0], top[0] + top[1]) are the
+  // two halves of the top-level split (no entry couples them), the rest its
+  // separator; zero in the AMD order
+  int top[2] = {0, 0};
 };
 
 // Fill-reducing orders of analyzeAndFactor.

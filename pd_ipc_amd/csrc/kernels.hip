@@ -623,18 +623,117 @@ __device__ PDX_INLINE void blockSweep(const BlockSweepDev& sw, WP W, PDX_LDS Rea
   }
 }
 
-// Copies the step tables (run, dense and task offsets) of both sweeps into LDS.
+// Copies the step tables (run, dense and task offsets) of the workgroup's sweeps
+// into LDS, one after the other: forward, backward, and (the lead of a pair) the
+// separator's forward and backward.  Sweeps without steps copy nothing.
 __device__ inline void loadStepTables(const InstDev& I) {
   PDX_LDS int* dst = ldsAt<int>(I.stepOff);
-  const BlockSweepDev* sws[2] = {&I.bfwd, &I.bbwd};
-  for (int q = 0; q < 2; ++q) {
+  const BlockSweepDev* sws[4] = {&I.bfwd, &I.bbwd, &I.bfwdS, &I.bbwdS};
+  for (int q = 0; q < 4; ++q) {
     const int n1 = sws[q]->nstep + 1;
-    for (int i = threadIdx.x; i < n1; i += blockDim.x) {
-      dst[i] = sws[q]->stepRun[i];
-      dst[n1 + i] = sws[q]->stepDense[i];
-      dst[2 * n1 + i] = sws[q]->stepTask[i];
-    }
+    if (n1 > 1)
+      for (int i = threadIdx.x; i < n1; i += blockDim.x) {
+        dst[i] = sws[q]->stepRun[i];
+        dst[n1 + i] = sws[q]->stepDense[i];
+        dst[2 * n1 + i] = sws[q]->stepTask[i];
+      }
     dst += 3 * n1;
+  }
+}
+
+// LDS step tables of sweep q (0 forward, 1 backward, 2 / 3 separator forward /
+// backward), in loadStepTables' layout.
+__device__ inline const PDX_LDS int* stepTables(const InstDev& I, int q) {
+  int off = 0;
+  if (q > 0) off += 3 * (I.bfwd.nstep + 1);
+  if (q > 1) off += 3 * (I.bbwd.nstep + 1);
+  if (q > 2) off += 3 * (I.bfwdS.nstep + 1);
+  return ldsAt<int>(I.stepOff) + off;
+}
+
+// ---- one instance on two workgroups (PairDev) ----------------------------------
+// Vertex loops of a pair run over the workgroup's own vertices (I.own); a
+// single-workgroup instance has own == null and takes every vertex.
+__device__ inline int ownCount(const InstDev& I) { return I.own ? I.nOwn : I.n; }
+__device__ inline int ownVertex(const InstDev& I, int i) { return I.own ? I.own[i] : i; }
+
+constexpr long long kPairSpin = 1LL << 25;  // polls (s_sleep 2 apart) before a wait gives up, ~2 s
+
+// Synchronisation point `target` (the count of points passed) of a pair.  Every
+// wave releases its global stores at agent scope, thread 0 publishes the count
+// and, with `wait`, polls until the partner has published the same point and
+// acquires (agent scope), so the partner's stores before its point are visible
+// to the whole workgroup after the return.  The wait is bounded: past kPairSpin
+// polls, or when the partner gave up, it sets the context's give-up flag and
+// returns false, and the caller leaves the kernel.
+__device__ bool pairSync(const InstDev& I, int target, bool wait, int* okp) {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int* mine = I.pair.flags + (I.pairRole == 1 ? 0 : 32);
+    int* other = I.pair.flags + (I.pairRole == 1 ? 32 : 0);
+    __hip_atomic_store(mine, target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    int ok = 1;
+    if (wait) {
+      long long polls = 0;
+      while (__hip_atomic_load(other, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        if (__hip_atomic_load(I.pair.giveUp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ||
+            ++polls > kPairSpin) {
+          __hip_atomic_store(I.pair.giveUp, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          ok = 0;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+      }
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    *okp = ok;
+  }
+  __syncthreads();
+  return *okp != 0;
+}
+
+// The stop test of one iteration (Integrator.cpp:673-751) on the maxima of both
+// halves of a pair.
+__device__ inline bool stopTest(const InstDev& I, double scale, double diff, double residual) {
+  const double rel = (scale > 0.0) ? diff / scale : diff;
+  const bool absHit = (I.absTol > 0.0) && (diff <= I.absTol);
+  const bool relHit = (I.relTol > 0.0) && (rel <= I.relTol);
+  const bool resHit = (I.resTol > 0.0) && (residual <= I.resThresh);
+  return (I.resTol > 0.0) ? resHit : (absHit || relHit);
+}
+
+// Copies W[3 lo, 3 hi) between the workgroup's LDS and the pair's exchange buffer.
+template <typename Real, typename WP>
+__device__ inline void pairPut(const InstDev& I, WP W, int lo, int hi) {
+  Real* xb = static_cast<Real*>(I.pair.xbuf);
+  for (int i = 3 * lo + threadIdx.x; i < 3 * hi; i += blockDim.x) xb[i] = W[i];
+}
+template <typename Real, typename WP>
+__device__ inline void pairGet(const InstDev& I, WP W, int lo, int hi) {
+  const Real* xb = static_cast<const Real*>(I.pair.xbuf);
+  for (int i = 3 * lo + threadIdx.x; i < 3 * hi; i += blockDim.x) W[i] = xb[i];
+}
+// Each workgroup of a pair hands over what the other needs to sweep the
+// separator (before a pairSync) and takes the other's (after it): the lead its
+// half's forward values and the separator's right-hand sides (it gathers the
+// separator's vertices), the partner its half's forward values.
+template <typename Real, typename WP>
+__device__ inline void pairExchange(const InstDev& I, WP W) {
+  if (I.pairRole == 1) {
+    pairPut<Real>(I, W, 0, I.rA);
+    pairPut<Real>(I, W, I.rB, I.m);
+  } else {
+    pairPut<Real>(I, W, I.rA, I.rB);
+  }
+}
+template <typename Real, typename WP>
+__device__ inline void pairTake(const InstDev& I, WP W) {
+  if (I.pairRole == 1) {
+    pairGet<Real>(I, W, I.rA, I.rB);
+  } else {
+    pairGet<Real>(I, W, 0, I.rA);
+    pairGet<Real>(I, W, I.rB, I.m);
   }
 }
 
@@ -885,7 +984,9 @@ __device__ inline void putW(WP W, int r, double a, double b, double c) {
 // Fills W with the first right-hand side of a substep (iteration 0 at x_n).
 template <typename Real, bool kExact, typename WP>
 __device__ PDX_INLINE void gatherFirst(const InstDev& I, WP W) {
-  for (int v = threadIdx.x; v < I.n; v += blockDim.x) {
+  const int nv = ownCount(I);
+  for (int i = threadIdx.x; i < nv; i += blockDim.x) {
+    const int v = ownVertex(I, i);
     const int r = I.vrank[v];
     if (kExact) {
       const Acc3 b = rhsExact(I, v);
@@ -907,9 +1008,18 @@ __device__ PDX_INLINE void runForward(const InstDev& I, WP W, SweepTimes& tst) {
   const Real* dinv = static_cast<const Real*>(I.dinv);
   auto none = [](int, Real, Real, Real) {};
   if (kBlocked)
-    blockSweep<Real, true, kIdx16>(I.bfwd, W, ldsAt<Real>(I.partOff), ldsAt<Real>(I.denseOff), ldsAt<int>(I.stepOff), none, tst, I.ldsW != 0);
+    blockSweep<Real, true, kIdx16>(I.bfwd, W, ldsAt<Real>(I.partOff), ldsAt<Real>(I.denseOff), stepTables(I, 0), none, tst, I.ldsW != 0);
   else
     sweep<Real, true>(I.fwd, W, dinv, none);
+}
+
+// The lead of a pair: the separator's forward and backward sweeps, once both
+// halves' forward values are in its W.
+template <typename Real, bool kIdx16, typename WP>
+__device__ PDX_INLINE void runSeparator(const InstDev& I, WP W, SweepTimes& tst) {
+  auto none = [](int, Real, Real, Real) {};
+  blockSweep<Real, true, kIdx16>(I.bfwdS, W, ldsAt<Real>(I.partOff), ldsAt<Real>(I.denseOff), stepTables(I, 2), none, tst, I.ldsW != 0);
+  blockSweep<Real, false, kIdx16>(I.bbwdS, W, ldsAt<Real>(I.partOff), ldsAt<Real>(I.denseOff), stepTables(I, 3), none, tst, I.ldsW != 0);
 }
 
 template <typename Real, bool kExact, bool kBlocked, bool kIdx16, typename WP>
@@ -933,7 +1043,7 @@ __device__ PDX_INLINE void runBackward(const InstDev& I, WP W, SweepTimes& tst) 
   (void)pinned;
   (void)pinpos;
   if (kBlocked)
-    blockSweep<Real, false, kIdx16>(I.bbwd, W, ldsAt<Real>(I.partOff), ldsAt<Real>(I.denseOff), ldsAt<int>(I.stepOff) + 3 * (I.bfwd.nstep + 1), put, tst, I.ldsW != 0);
+    blockSweep<Real, false, kIdx16>(I.bbwd, W, ldsAt<Real>(I.partOff), ldsAt<Real>(I.denseOff), stepTables(I, 1), put, tst, I.ldsW != 0);
   else
     sweep<Real, false>(I.bwd, W, dinv, put);
 }
@@ -1001,6 +1111,14 @@ __device__ void substepBody(const InstDev& I, WP W, int nsub, int subBase, int f
   const bool fixedIter = kRoll && (flags & kRollFixed) != 0;
   const bool wantRes = I.wantResidual && !fixedIter;
   const size_t n3 = (size_t)3 * n;
+  // an instance split over two workgroups (FP32 solves, blocked sweeps, no rollout)
+  constexpr bool kPairable = !kExact && kBlocked && !kRoll;
+  const bool paired = kPairable && I.pairRole != 0;
+  const int nv = ownCount(I);
+  int* pairOk = (int*)(red + 31);
+  int seq = 0;  // synchronisation points of the pair passed in this launch
+  if (paired) seq = __hip_atomic_load(I.pair.flags + (I.pairRole == 1 ? 0 : 32), __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT);
 
   if (kBlocked) loadStepTables(I);  // visible after the first barrier below
   long long tk[4] = {0, 0, 0, 0};
@@ -1012,7 +1130,8 @@ __device__ void substepBody(const InstDev& I, WP W, int nsub, int subBase, int f
     const long long tp0 = tick();
     // predict (Integrator.cpp:447-463) and the displacement scale
     double lscale = 0.0;
-    for (int v = tid; v < n; v += bs) {
+    for (int i = tid; i < nv; i += bs) {
+      const int v = ownVertex(I, i);
       for (int d = 0; d < 3; ++d) {
         const double p = I.x[3 * v + d];
         I.xbs[3 * v + d] = p;
@@ -1039,13 +1158,38 @@ __device__ void substepBody(const InstDev& I, WP W, int nsub, int subBase, int f
     for (int k = 0; k < I.maxIter; ++k) {
       if (tid == 0) tk[0] = tick();
       runForward<Real, kBlocked, kIdx16>(I, W, swt);
+      if constexpr (kPairable) {
+        if (paired) {
+          // The halves hand each other their forward values and both sweep the
+          // separator (the same arithmetic, so the same bits).  The maxima of the
+          // previous iteration travel with this exchange: its stop decision is
+          // taken here, before anything but W changed.
+          pairExchange<Real>(I, W);
+          if (!pairSync(I, ++seq, true, pairOk)) return;
+          if (k > 0) {
+            const double* other = I.pair.red + 8 * ((k - 1) & 1) + 4 * (2 - I.pairRole);
+            const double* mine = I.pair.red + 8 * ((k - 1) & 1) + 4 * (I.pairRole - 1);
+            const double sc = fmax(mine[0], other[0]), df = fmax(mine[1], other[1]);
+            const double rsd = fmax(mine[2], other[2]);
+            lastRes = rsd;
+            if (stopTest(I, sc, df, rsd)) {
+              conv = 1;
+              break;
+            }
+          }
+          pairTake<Real>(I, W);
+          __syncthreads();
+          runSeparator<Real, kIdx16>(I, W, swt);
+        }
+      }
       if (tid == 0) tk[1] = tick();
       runBackward<Real, kExact, kBlocked, kIdx16>(I, W, swt);
       if (kExact && kBlocked) __syncthreads();  // the sweep stored x; the gather reads it
       if (!kExact) {
         // x += delta (pinned: x = q) for every vertex, then a barrier so that the
-        // gather below sees every neighbour's new position
-        for (int v = tid; v < n; v += bs) {
+        // gather below sees every neighbour's new position (a pair: both halves')
+        for (int i = tid; i < nv; i += bs) {
+          const int v = ownVertex(I, i);
           const int r = I.vrank[v];
           if (I.pinned[v]) {
             I.x[3 * v] = I.pinpos[3 * v];
@@ -1058,6 +1202,9 @@ __device__ void substepBody(const InstDev& I, WP W, int nsub, int subBase, int f
           }
         }
         __syncthreads();
+        if constexpr (kPairable) {
+          if (paired && !pairSync(I, ++seq, true, pairOk)) return;
+        }
       }
       if (tid == 0) tk[2] = tick();
       const bool more = (k + 1 < I.maxIter);
@@ -1101,7 +1248,8 @@ __device__ void substepBody(const InstDev& I, WP W, int nsub, int subBase, int f
             }
           }
       }
-      for (int v = cheb ? n : tid; v < n; v += bs) {
+      for (int i = cheb ? nv : tid; i < nv; i += bs) {
+        const int v = ownVertex(I, i);
         const double x0 = I.x[3 * v], x1 = I.x[3 * v + 1], x2 = I.x[3 * v + 2];
         if (tapeK) {
           tapeK[3 * v] = x0;
@@ -1143,12 +1291,31 @@ __device__ void substepBody(const InstDev& I, WP W, int nsub, int subBase, int f
       }
       // criterion (Integrator.cpp:673-751); two slot sets alternate so one barrier suffices
       double* rs = red + 16 * ((k & 1) ^ 1);
-      const double diff = blockMax(ldiff, rs);
+      double diff = blockMax(ldiff, rs);
       if (tid == 0) tk[3] = tick();
       double residual = -1.0;
       if (I.wantResidual) {
         __syncthreads();
         residual = blockMax(lres, rs);
+      }
+      if constexpr (kPairable) {
+        if (paired) {
+          // both workgroups take the stop decision on the maxima of both halves:
+          // at the next iteration's exchange, or here after the last iteration
+          double* mine = I.pair.red + 8 * (k & 1) + 4 * (I.pairRole - 1);
+          const double* other = I.pair.red + 8 * (k & 1) + 4 * (2 - I.pairRole);
+          if (tid == 0) {
+            mine[0] = scale;
+            mine[1] = diff;
+            mine[2] = residual;
+          }
+          used = k + 1;
+          if (k + 1 < I.maxIter) continue;
+          if (!pairSync(I, ++seq, true, pairOk)) return;
+          scale = fmax(scale, other[0]);
+          diff = fmax(diff, other[1]);
+          residual = fmax(residual, other[2]);
+        }
       }
       if (tid == 0) {
         tacc[1] += tk[1] - tk[0];
@@ -1170,7 +1337,8 @@ __device__ void substepBody(const InstDev& I, WP W, int nsub, int subBase, int f
     }
     // velocity + damping + re-pin (Integrator.cpp:774-791)
     const long long tv0 = tick();
-    for (int v = tid; v < n; v += bs) {
+    for (int i = tid; i < nv; i += bs) {
+      const int v = ownVertex(I, i);
       const bool pinnedV = I.pinned[v] != 0;
       for (int d = 0; d < 3; ++d) {
         if (pinnedV) {
@@ -1181,7 +1349,7 @@ __device__ void substepBody(const InstDev& I, WP W, int nsub, int subBase, int f
         }
       }
     }
-    if (tid == 0) {
+    if (tid == 0 && I.pairRole != 2) {  // a pair's lead keeps the instance's statistics
       tacc[3] += tick() - tv0;
       I.istats[0] = used;
       I.istats[1] = conv;
@@ -1192,7 +1360,7 @@ __device__ void substepBody(const InstDev& I, WP W, int nsub, int subBase, int f
     }
     __syncthreads();
   }
-  if (tid == 0) {
+  if (tid == 0 && I.pairRole != 2) {
     for (int q = 0; q < 5; ++q) I.lstats[2 + q] += tacc[q];
     I.lstats[7] += swt.stream;
     I.lstats[8] += swt.wait1;
@@ -1863,12 +2031,14 @@ __global__ void __launch_bounds__(512) denseSolveKernel(const InstDev* __restric
 template <typename Real, bool kBlocked, bool kIdx16, typename WP>
 __device__ void solveBody(const InstDev& I, WP W, const double* __restrict__ b, double* __restrict__ xout);
 
-// Stand-alone global solve x = L^{-1} b (EigenDirectSolver::solve) on one instance.
+// Stand-alone global solve x = L^{-1} b (EigenDirectSolver::solve) on one instance:
+// one workgroup, or two for a split instance (workgroup 1 runs the partner's
+// descriptor inst2).
 template <typename Real, bool kBlocked, bool kIdx16, bool kWL>
-__global__ void __launch_bounds__(512) solveKernel(const InstDev* __restrict__ insts, int inst,
+__global__ void __launch_bounds__(512) solveKernel(const InstDev* __restrict__ insts, int inst, int inst2,
                                                     const double* __restrict__ b,
                                                     double* __restrict__ xout) {
-  const InstDev& I = insts[inst];
+  const InstDev& I = insts[blockIdx.x == 0 ? inst : inst2];
   if (kWL)
     solveBody<Real, kBlocked, kIdx16>(I, ldsAt<Real>(32 * sizeof(double)), b, xout);
   else
@@ -1878,7 +2048,9 @@ __global__ void __launch_bounds__(512) solveKernel(const InstDev* __restrict__ i
 template <typename Real, bool kBlocked, bool kIdx16, typename WP>
 __device__ void solveBody(const InstDev& I, WP W, const double* __restrict__ b, double* __restrict__ xout) {
   if (kBlocked) loadStepTables(I);
-  for (int v = threadIdx.x; v < I.n; v += blockDim.x) {
+  const int nv = ownCount(I);
+  for (int i = threadIdx.x; i < nv; i += blockDim.x) {
+    const int v = ownVertex(I, i);
     const int r = I.vrank[v];
     putW<Real>(W, r, b[3 * v], b[3 * v + 1], b[3 * v + 2]);
   }
@@ -1887,20 +2059,36 @@ __device__ void solveBody(const InstDev& I, WP W, const double* __restrict__ b, 
   SweepTimes tst;
   auto none = [](int, Real, Real, Real) {};
   if (kBlocked)
-    blockSweep<Real, true, kIdx16>(I.bfwd, W, ldsAt<Real>(I.partOff), ldsAt<Real>(I.denseOff), ldsAt<int>(I.stepOff), none, tst, I.ldsW != 0);
+    blockSweep<Real, true, kIdx16>(I.bfwd, W, ldsAt<Real>(I.partOff), ldsAt<Real>(I.denseOff), stepTables(I, 0), none, tst, I.ldsW != 0);
   else
     sweep<Real, true>(I.fwd, W, dinv, none);
-  const int* rankv = I.rankv;
-  auto put = [=](int k, Real a0, Real a1, Real a2) {
-    const int v = rankv[k];
-    xout[3 * v] = (double)a0;
-    xout[3 * v + 1] = (double)a1;
-    xout[3 * v + 2] = (double)a2;
-  };
+  if constexpr (kBlocked) {
+    if (I.pairRole != 0) {
+      // the halves meet at the separator, as in the substep (pairSync)
+      extern __shared__ double smem[];
+      int* ok = (int*)(smem + 31);
+      int seq = __hip_atomic_load(I.pair.flags + (I.pairRole == 1 ? 0 : 32), __ATOMIC_RELAXED,
+                                  __HIP_MEMORY_SCOPE_AGENT);
+      pairExchange<Real>(I, W);
+      if (!pairSync(I, ++seq, true, ok)) return;
+      pairTake<Real>(I, W);
+      __syncthreads();
+      runSeparator<Real, kIdx16>(I, W, tst);
+    }
+  }
   if (kBlocked)
-    blockSweep<Real, false, kIdx16>(I.bbwd, W, ldsAt<Real>(I.partOff), ldsAt<Real>(I.denseOff), ldsAt<int>(I.stepOff) + 3 * (I.bfwd.nstep + 1), put, tst, I.ldsW != 0);
+    blockSweep<Real, false, kIdx16>(I.bbwd, W, ldsAt<Real>(I.partOff), ldsAt<Real>(I.denseOff), stepTables(I, 1), none, tst, I.ldsW != 0);
   else
-    sweep<Real, false>(I.bwd, W, dinv, put);
+    sweep<Real, false>(I.bwd, W, dinv, none);
+  __syncthreads();
+  // the solution of the workgroup's own vertices, from rank space
+  for (int i = threadIdx.x; i < nv; i += blockDim.x) {
+    const int v = ownVertex(I, i);
+    const int r = I.vrank[v];
+    xout[3 * v] = (double)W[3 * r];
+    xout[3 * v + 1] = (double)W[3 * r + 1];
+    xout[3 * v + 2] = (double)W[3 * r + 2];
+  }
 }
 
 }  // namespace
@@ -2024,15 +2212,18 @@ int gridBlocksMax(int block, size_t lds, bool fp32, bool idx16) {
   return perCu * cus;
 }
 
-hipError_t launchSolve(const InstDev* d, int inst, const double* b, double* x, int block,
+hipError_t launchSolve(const InstDev* d, int inst, int inst2, const double* b, double* x, int block,
                        size_t lds, bool fp32, bool blocked, bool idx16, bool wl, hipStream_t s) {
-#define PDX_L(R, E, B, X16)                                                                            \
-  if (fp32 == (sizeof(R) == 4) && blocked == B && (idx16 && blocked) == X16) {                         \
-    if (wl)                                                                                            \
-      hipLaunchKernelGGL((solveKernel<R, B, X16, true>), dim3(1), dim3(block), lds, s, d, inst, b, x);  \
-    else                                                                                               \
-      hipLaunchKernelGGL((solveKernel<R, B, X16, false>), dim3(1), dim3(block), lds, s, d, inst, b, x); \
-    return hipGetLastError();                                                                          \
+  const int g = inst2 >= 0 ? 2 : 1;
+#define PDX_L(R, E, B, X16)                                                                     \
+  if (fp32 == (sizeof(R) == 4) && blocked == B && (idx16 && blocked) == X16) {                  \
+    if (wl)                                                                                     \
+      hipLaunchKernelGGL((solveKernel<R, B, X16, true>), dim3(g), dim3(block), lds, s, d, inst, \
+                         inst2, b, x);                                                          \
+    else                                                                                        \
+      hipLaunchKernelGGL((solveKernel<R, B, X16, false>), dim3(g), dim3(block), lds, s, d, inst, \
+                         inst2, b, x);                                                          \
+    return hipGetLastError();                                                                   \
   }
   PDX_KERNEL_TABLE(PDX_L)
 #undef PDX_L

@@ -41,6 +41,9 @@ struct NdState {
     return false;
   }
 
+  int depth = 0;
+  int top[2] = {0, 0};  // vertices of the two halves of the top-level split
+
   void split(std::vector<int>& vs) {
     if ((int)vs.size() <= leaf) {
       out->insert(out->end(), vs.begin(), vs.end());
@@ -86,15 +89,21 @@ struct NdState {
     std::vector<int> s(sep);
     vs.clear();
     vs.shrink_to_fit();
+    if (depth == 0) {
+      top[0] = (int)L.size();
+      top[1] = (int)R.size();
+    }
+    ++depth;
     split(L);
     split(R);
+    --depth;
     out->insert(out->end(), s.begin(), s.end());
   }
 };
 
 }  // namespace
 
-void ndOrder(const Matrix& A, const std::vector<double>& x, int leaf, std::vector<int>& order) {
+void ndOrder(const Matrix& A, const std::vector<double>& x, int leaf, std::vector<int>& order, int* top) {
   const int n = A.n;
   order.clear();
   order.reserve((size_t)n);
@@ -107,6 +116,10 @@ void ndOrder(const Matrix& A, const std::vector<double>& x, int leaf, std::vecto
   std::vector<int> all((size_t)n);
   std::iota(all.begin(), all.end(), 0);
   st.split(all);
+  if (top) {
+    top[0] = st.top[0];
+    top[1] = st.top[1];
+  }
 }
 
 }  // namespace pdx

@@ -128,6 +128,9 @@ def load_library(path=LIB_PATH):
         "pdx_get_schedule_stats": (I, [P, I, P]),
         "pdx_set_schedule": (I, [P, I, I]),
         "pdx_set_ordering": (I, [P, I]),
+        "pdx_set_split": (I, [P, I]),
+        "pdx_split_count": (I, [P]),
+        "pdx_verify_schedule": (I, [P, I, C.POINTER(D)]),
         "pdx_bytes_per_iteration": (D, [P]),
         "pdx_bytes_per_substep": (D, [P]),
         "pdx_grid_counts": (I, [I, I, I, I, C.POINTER(I), C.POINTER(I), C.POINTER(I)]),
@@ -487,6 +490,22 @@ class Engine:
         'nd' (nested dissection: fewer dependent sweep steps; fast mode only)."""
         code = {"amd": PDX_ORDER_AMD, "nd": PDX_ORDER_ND}[ordering]
         self._check(self.lib.pdx_set_ordering(self.h, code), "set_ordering")
+
+    def set_split(self, min_vertices):
+        """Instances of at least `min_vertices` vertices run on two workgroups (mixed32,
+        fast solve, nested-dissection order, batch launch); 0 turns it off."""
+        self._check(self.lib.pdx_set_split(self.h, int(min_vertices)), "set_split")
+
+    def split_count(self):
+        return self.lib.pdx_split_count(self.h)
+
+    def verify_schedule(self, i=0):
+        """Host replay of the instance's sweep schedules in the kernel's phase order;
+        raises PdxError on a dependency or bounds violation, else returns the max
+        relative difference from a direct solve."""
+        r = C.c_double()
+        self._check(self.lib.pdx_verify_schedule(self.h, i, C.byref(r)), "verify_schedule")
+        return r.value
 
     def set_launch(self, mode, grid_blocks=0):
         """mode: 'batch' (one workgroup per instance) or 'grid' (the whole GPU on
