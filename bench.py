@@ -469,6 +469,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-dropin", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape (512 threads, chain cap 32)")
+    ap.add_argument("--split", type=int, default=None,
+                    help="split threshold of the mixed32 batch in factor entries (pdx_set_split); the "
+                         "autotuner searches thresholds unless --no-tune")
     ap.add_argument("--workload", default="batch", choices=["batch", "adjoint", "dropin", "large"])
     ap.add_argument("--dropin-substeps", type=int, default=0, help="cap on the drop-in configs' substeps")
     ap.add_argument("--large-substeps", type=int, default=10)
@@ -505,8 +508,12 @@ def main():
     order = args.ordering if args.solve == "fast" else "amd"
     mixed, tf32 = build_batch(cloths, cfg, "mixed32", args.solve, order)
     log(f"mixed32 factor+upload {tf32:.1f}s")
+    if args.split is not None:
+        mixed.set_split(args.split)
     tune32 = mixed.autotune(1) if not args.no_tune else None
-    log(f"mixed32 autotune: {tune32}")
+    mixed_shape = tune32[:2] if tune32 is not None else (512, 32)
+    log(f"mixed32 autotune: {tune32}, split threshold {mixed.split_threshold()} "
+        f"({mixed.split_count()} instances on two workgroups)")
     fp64, tf64 = build_batch(cloths, cfg, "fp64", args.solve)
     log(f"fp64 factor+upload {tf64:.1f}s")
     tune64 = fp64.autotune(1) if not args.no_tune else None
@@ -550,7 +557,9 @@ def main():
         "config": {"workload": "512-instance cloth mix, sides 16-96, 40 PD iters, h=1/600",
                    "instances": ninst, "vertices": nverts, "pd_iterations": iters,
                    "seed": args.seed, "solve": args.solve, "ordering": order,
-                   "schedule": None if tune32 is None else {"threads": tune32[0], "chain_cap": tune32[1]}},
+                   "schedule": {"threads": mixed_shape[0], "chain_cap": mixed_shape[1],
+                                "split_min_factor_entries": mixed.split_threshold(),
+                                "split_instances": mixed.split_count()}},
         "max_pos_error_m": max_err,
         "max_pos_error_basis": f"max over all {len(cloths)} instances of max |x_mixed32 - x_fp64| after "
                                f"{args.error_substeps} substeps from the initial state (untimed run): mixed32 "

@@ -214,8 +214,9 @@ int pdx_set_schedule(pdx_ctx* ctx, int block_threads, int chain_cap);
  * the next pdx_factor / pdx_step (counted in analyze_count and factorize_count). */
 int pdx_set_ordering(pdx_ctx* ctx, int ordering);
 
-/* Splits every instance of at least min_vertices vertices over two workgroups
- * (0, the default, keeps one workgroup per instance).  No counterpart in the
+/* Splits every instance whose factor has at least min_factor_entries strictly-lower
+ * entries (pdx_stats.factor_nnz) over two workgroups (0, the default, keeps one
+ * workgroup per instance; 1 splits every instance that can be).  No counterpart in the
  * reference, whose solve is one CPU call per cloth (EigenDirectSolver.cpp:411-447).
  * Applies to the batch launch of PDX_PREC_MIXED32 / PDX_SOLVE_FAST contexts in
  * PDX_ORDER_ND whose factor has a top-level split and W in LDS: the two
@@ -226,9 +227,11 @@ int pdx_set_ordering(pdx_ctx* ctx, int ordering);
  * returns PDX_EHIP.  Results are bit for bit those of one workgroup (the
  * schedules cut the same blocks either way).  The differentiable rollout runs
  * every instance on one workgroup.  Re-packs the schedules (no new factorisation). */
-int pdx_set_split(pdx_ctx* ctx, int min_vertices);
+int pdx_set_split(pdx_ctx* ctx, long long min_factor_entries);
 /* Number of instances currently split over two workgroups (after pdx_factor or a step). */
 int pdx_split_count(const pdx_ctx* ctx);
+/* The current split threshold (factor entries; 0 = no split; -1 for a NULL context). */
+long long pdx_get_split(const pdx_ctx* ctx);
 /* Host check of an instance's fast-mode sweep schedules (no device use): replays
  * them in FP64 in the kernel's phase order - for a split instance, each half seeing
  * only what its workgroup holds - and fails with PDX_EUNSUPPORTED (message in
@@ -279,7 +282,10 @@ int pdx_set_launch(pdx_ctx* ctx, int mode, int grid_blocks);
 /* Autotuner: times `probe_substeps` substeps of the whole context under each
  * launch shape (threads per workgroup 256 / 512 x chain cap 8 / 16 / 24 / 32 in
  * fast mode; in the dense solve mode, 16, 32, 64, ... workgroups, reported in
- * best_block with best_cap 0) and keeps the fastest.  State and statistics are
+ * best_block with best_cap 0) and keeps the fastest; then, for a FP32-solve fast
+ * batch, the split threshold (pdx_set_split: none, or the factor size of the 1st,
+ * 2nd, 4th, 8th, ..., 64th largest instance that can split; pdx_get_split reads
+ * the choice).  State and statistics are
  * restored, so the probes leave no trace.  No counterpart in the reference (its
  * CPU path has no launch schedule). */
 int pdx_autotune(pdx_ctx* ctx, int probe_substeps, int* best_block, int* best_cap, double* best_ms);

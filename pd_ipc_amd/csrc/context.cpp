@@ -464,9 +464,9 @@ struct pdx_ctx {
   size_t linvBytes = 0;
   bool linvStale = true;
   bool denseLds = false;  // W staged in each workgroup's LDS for the product
-  // instances of at least splitMin vertices run on two workgroups (0 = never);
+  // instances whose factor has at least splitMin entries run on two workgroups (0 = never);
   // the differentiable rollout (rollMode) packs every instance for one workgroup
-  int splitMin = 0;
+  long long splitMin = 0;
   bool rollMode = false;
   int npairs = 0;
   int* dGiveUp = nullptr;      // the pairs' give-up flag (device)
@@ -577,10 +577,12 @@ double blockBytes(const HostBlockSweep& s, size_t w) {
 
 // Whether an instance runs on two workgroups: the fast FP32-solve batch form with
 // W in LDS, a nested-dissection factor with a top-level split, no Chebyshev
-// blend, outside the differentiable rollout, and at least splitMin vertices.
+// blend, outside the differentiable rollout, and at least splitMin factor entries
+// (the entries, which both sweeps stream and which grow with bending's wider
+// coupling, predict an instance's time far better than its vertex count).
 bool wantPair(const pdx_ctx* c, const Instance& I, bool wCanFit) {
   return c->splitMin > 0 && !c->rollMode && c->mode == PDX_SOLVE_FAST && c->prec == PDX_PREC_MIXED32 &&
-         c->launchMode == 0 && wCanFit && I.cfg.chebRho <= 0.0 && I.scene.n >= c->splitMin &&
+         c->launchMode == 0 && wCanFit && I.cfg.chebRho <= 0.0 && I.fac.nnz >= c->splitMin &&
          I.fac.top[0] > 0 && I.fac.top[1] > 0 && I.fac.top[0] + I.fac.top[1] < I.fac.m;
 }
 
@@ -1731,28 +1733,23 @@ int pdx_get_stats(pdx_ctx* c, int i, pdx_stats* s) {
   return PDX_OK;
 }
 
-int pdx_solve3(pdx_ctx* c, int i, const double* b, double* x) {
-  if (!c || !b || !x || i < 0 || i >= (int)c->inst.size()) return PDX_EINVAL;
-  int rc = ensureReady(c);
-  if (rc) return rc;
+}  // extern "C"
+
+namespace {
+
+// One global solve of instance i on the context's stream: b (3n, host) staged in
+// dB, the solution written to dX and copied back to x (host).
+int solveThrough(pdx_ctx* c, int i, const double* b, double* x, double* dB, double* dX) {
   Instance& I = c->inst[(size_t)i];
   const int n3 = 3 * I.scene.n;
-  if (c->scratchN < n3) {
-    if (c->dScratchB) hipFree(c->dScratchB);
-    if (c->dScratchX) hipFree(c->dScratchX);
-    HIPCHECK(c, hipMalloc(&c->dScratchB, (size_t)n3 * 8));
-    HIPCHECK(c, hipMalloc(&c->dScratchX, (size_t)n3 * 8));
-    c->scratchN = n3;
-  }
-  HIPCHECK(c, hipMemcpyAsync(c->dScratchB, b, (size_t)n3 * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(c, hipMemcpyAsync(dB, b, (size_t)n3 * 8, hipMemcpyHostToDevice, c->stream));
   if (c->mode == PDX_SOLVE_DENSE) {
-    HIPCHECK(c, launchDenseSolve(c->dInst, i, c->dScratchB, c->dScratchX, I.scene.n, I.fac.m,
-                                 c->prec == PDX_PREC_MIXED32, c->stream));
+    HIPCHECK(c, launchDenseSolve(c->dInst, i, dB, dX, I.scene.n, I.fac.m, c->prec == PDX_PREC_MIXED32, c->stream));
   } else if (c->launchMode == 1) {
     if (c->lds < gridLdsNeed(c))
       return fail(c, PDX_EINVAL, "grid form: the launch's LDS does not hold a slice per wavefront");
     const int g = gridLaunchBlocks(c);
-    HIPCHECK(c, launchGridSolve(c->dInst, i, c->dScratchB, c->dScratchX, c->waveRuns, g, c->block, c->lds,
+    HIPCHECK(c, launchGridSolve(c->dInst, i, dB, dX, c->waveRuns, g, c->block, c->lds,
                                 c->prec == PDX_PREC_MIXED32, c->idx16, c->stream));
   } else {
     // a split instance solves on its two workgroups (the partner's descriptor
@@ -1763,12 +1760,33 @@ int pdx_solve3(pdx_ctx* c, int i, const double* b, double* x) {
       for (int j = 0; j < i; ++j) p += c->inst[(size_t)j].paired ? 1 : 0;
       partner = (int)c->inst.size() + p;
     }
-    HIPCHECK(c, launchSolve(c->dInst, i, partner, c->dScratchB, c->dScratchX, c->block, c->lds,
-                            c->prec == PDX_PREC_MIXED32, c->mode == PDX_SOLVE_FAST, c->idx16, c->ldsW, c->stream));
+    HIPCHECK(c, launchSolve(c->dInst, i, partner, dB, dX, c->block, c->lds, c->prec == PDX_PREC_MIXED32,
+                            c->mode == PDX_SOLVE_FAST, c->idx16, c->ldsW, c->stream));
   }
-  HIPCHECK(c, hipMemcpyAsync(x, c->dScratchX, (size_t)n3 * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipMemcpyAsync(x, dX, (size_t)n3 * 8, hipMemcpyDeviceToHost, c->stream));
   HIPCHECK(c, hipStreamSynchronize(c->stream));
   return checkPairs(c);
+}
+
+}  // namespace
+
+extern "C" {
+
+int pdx_solve3(pdx_ctx* c, int i, const double* b, double* x) {
+  if (!c || !b || !x || i < 0 || i >= (int)c->inst.size()) return PDX_EINVAL;
+  int rc = ensureReady(c);
+  if (rc) return rc;
+  const int n3 = 3 * c->inst[(size_t)i].scene.n;
+  if (c->scratchN < n3) {
+    if (c->dScratchB) hipFree(c->dScratchB);
+    if (c->dScratchX) hipFree(c->dScratchX);
+    c->dScratchB = c->dScratchX = nullptr;
+    c->scratchN = 0;
+    HIPCHECK(c, hipMalloc(&c->dScratchB, (size_t)n3 * 8));
+    HIPCHECK(c, hipMalloc(&c->dScratchX, (size_t)n3 * 8));
+    c->scratchN = n3;
+  }
+  return solveThrough(c, i, b, x, c->dScratchB, c->dScratchX);
 }
 
 int pdx_get_factor(pdx_ctx* c, int i, int* m, long long* nnz, int32_t* ptr, int32_t* rows, double* vals,
@@ -2212,12 +2230,12 @@ int pdx_set_schedule(pdx_ctx* c, int block_threads, int chain_cap) {
   return PDX_OK;
 }
 
-int pdx_set_split(pdx_ctx* c, int min_vertices) {
-  if (!c || min_vertices < 0) return PDX_EINVAL;
-  if (min_vertices == c->splitMin) return PDX_OK;
+int pdx_set_split(pdx_ctx* c, long long min_factor_entries) {
+  if (!c || min_factor_entries < 0) return PDX_EINVAL;
+  if (min_factor_entries == c->splitMin) return PDX_OK;
   const int rc = pullState(c);
   if (rc) return rc;
-  c->splitMin = min_vertices;
+  c->splitMin = min_factor_entries;
   for (auto& I : c->inst) I.packed = false;  // repack the same factors
   c->dirty = true;
   return PDX_OK;
@@ -2368,12 +2386,50 @@ int pdx_autotune(pdx_ctx* c, int probe_substeps, int* best_block, int* best_cap,
   }
   if ((rc = pdx_set_schedule(c, bb, bc))) return rc;
   if ((rc = ensureReady(c))) return rc;
+  // Split instances (pdx_set_split), where the context allows it: thresholds at
+  // the factor sizes of the 8th, 16th, ... largest instances that can split, and
+  // none; the shape found above is kept.
+  if (fast && c->prec == PDX_PREC_MIXED32 && c->launchMode == 0 && !c->rollMode) {
+    std::vector<long long> sizes;
+    for (const auto& I : c->inst)
+      if (I.fac.top[0] > 0 && I.fac.top[1] > 0 && I.cfg.chebRho <= 0.0) sizes.push_back(I.fac.nnz);
+    std::sort(sizes.rbegin(), sizes.rend());
+    if (!sizes.empty()) {
+      std::vector<long long> cand{c->splitMin};
+      if (c->splitMin != 0) cand.push_back(0);
+      for (size_t k : {1, 2, 4, 8, 16, 24, 32, 48, 64})
+        if (k <= sizes.size()) cand.push_back(sizes[k - 1]);
+      std::sort(cand.begin(), cand.end());
+      cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+      long long bestSplit = c->splitMin;
+      double splitMs = 1e300;
+      for (long long th : cand) {
+        if ((rc = pdx_set_split(c, th))) return rc;
+        if ((rc = ensureReady(c))) {
+          if (rc == PDX_EUNSUPPORTED) continue;
+          return rc;
+        }
+        if ((rc = restore())) return rc;
+        if ((rc = launch(c, probe_substeps))) return rc;
+        if ((rc = pdx_sync(c))) return rc;
+        if (c->lastKernelMs < splitMs) {
+          splitMs = c->lastKernelMs;
+          bestSplit = th;
+        }
+      }
+      if ((rc = pdx_set_split(c, bestSplit))) return rc;
+      if ((rc = ensureReady(c))) return rc;
+      bestMs = std::min(bestMs, splitMs);
+    }
+  }
   if ((rc = restore())) return rc;
   if (best_block) *best_block = bb;
   if (best_cap) *best_cap = bc;
   if (best_ms) *best_ms = bestMs;
   return PDX_OK;
 }
+
+long long pdx_get_split(const pdx_ctx* c) { return c ? c->splitMin : -1; }
 
 double pdx_last_kernel_ms(const pdx_ctx* c) { return c ? c->lastKernelMs : 0.0; }
 

@@ -659,15 +659,32 @@ __device__ inline int ownVertex(const InstDev& I, int i) { return I.own ? I.own[
 
 constexpr long long kPairSpin = 1LL << 25;  // polls (s_sleep 2 apart) before a wait gives up, ~2 s
 
+// The XCD (its L2) the calling workgroup runs on.
+__device__ inline int xccId() { return __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20); }  // HW_REG_XCC_ID[3:0]
+
+// Waits until this wave's global stores have been performed (vmcnt 0); the
+// signal fences keep the compiler from moving memory operations across it.
+__device__ inline void storesPerformed() {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0), expcnt and lgkmcnt not waited
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
 // Synchronisation point `target` (the count of points passed) of a pair.  Every
-// wave releases its global stores at agent scope, thread 0 publishes the count
-// and, with `wait`, polls until the partner has published the same point and
-// acquires (agent scope), so the partner's stores before its point are visible
-// to the whole workgroup after the return.  The wait is bounded: past kPairSpin
-// polls, or when the partner gave up, it sets the context's give-up flag and
-// returns false, and the caller leaves the kernel.
-__device__ bool pairSync(const InstDev& I, int target, bool wait, int* okp) {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+// wave releases its global stores, thread 0 publishes the count and, with
+// `wait`, polls until the partner has published the same point and acquires
+// (agent scope), so the partner's stores before its point are visible to the
+// whole workgroup after the return.  The release is at agent scope (the L2 is
+// written back for other XCDs) unless both workgroups share an XCD
+// (`sameL2`): then their stores meet in that L2 and waiting until they are
+// performed suffices.  The wait is bounded: past kPairSpin polls, or when the
+// partner gave up, it sets the context's give-up flag and returns false, and
+// the caller leaves the kernel.
+__device__ PDX_INLINE bool pairSync(const InstDev& I, int target, bool wait, int* okp, bool sameL2 = false) {
+  if (sameL2)
+    storesPerformed();
+  else
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
   __syncthreads();
   if (threadIdx.x == 0) {
     int* mine = I.pair.flags + (I.pairRole == 1 ? 0 : 32);
@@ -1117,8 +1134,14 @@ __device__ void substepBody(const InstDev& I, WP W, int nsub, int subBase, int f
   const int nv = ownCount(I);
   int* pairOk = (int*)(red + 31);
   int seq = 0;  // synchronisation points of the pair passed in this launch
-  if (paired) seq = __hip_atomic_load(I.pair.flags + (I.pairRole == 1 ? 0 : 32), __ATOMIC_RELAXED,
-                                      __HIP_MEMORY_SCOPE_AGENT);
+  bool sameL2 = false;  // both workgroups of the pair on one XCD (known after the first point)
+  if (paired) {
+    seq = __hip_atomic_load(I.pair.flags + (I.pairRole == 1 ? 0 : 32), __ATOMIC_RELAXED,
+                            __HIP_MEMORY_SCOPE_AGENT);
+    if (tid == 0)
+      __hip_atomic_store(I.pair.flags + (I.pairRole == 1 ? 16 : 48), xccId(), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+  }
 
   if (kBlocked) loadStepTables(I);  // visible after the first barrier below
   long long tk[4] = {0, 0, 0, 0};
@@ -1165,7 +1188,9 @@ __device__ void substepBody(const InstDev& I, WP W, int nsub, int subBase, int f
           // previous iteration travel with this exchange: its stop decision is
           // taken here, before anything but W changed.
           pairExchange<Real>(I, W);
-          if (!pairSync(I, ++seq, true, pairOk)) return;
+          if (!pairSync(I, ++seq, true, pairOk, sameL2)) return;
+          sameL2 = __hip_atomic_load(I.pair.flags + 16, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+                   __hip_atomic_load(I.pair.flags + 48, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           if (k > 0) {
             const double* other = I.pair.red + 8 * ((k - 1) & 1) + 4 * (2 - I.pairRole);
             const double* mine = I.pair.red + 8 * ((k - 1) & 1) + 4 * (I.pairRole - 1);
@@ -1203,7 +1228,7 @@ __device__ void substepBody(const InstDev& I, WP W, int nsub, int subBase, int f
         }
         __syncthreads();
         if constexpr (kPairable) {
-          if (paired && !pairSync(I, ++seq, true, pairOk)) return;
+          if (paired && !pairSync(I, ++seq, true, pairOk, sameL2)) return;
         }
       }
       if (tid == 0) tk[2] = tick();
@@ -1311,7 +1336,7 @@ __device__ void substepBody(const InstDev& I, WP W, int nsub, int subBase, int f
           }
           used = k + 1;
           if (k + 1 < I.maxIter) continue;
-          if (!pairSync(I, ++seq, true, pairOk)) return;
+          if (!pairSync(I, ++seq, true, pairOk, sameL2)) return;
           scale = fmax(scale, other[0]);
           diff = fmax(diff, other[1]);
           residual = fmax(residual, other[2]);

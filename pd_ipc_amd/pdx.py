@@ -128,8 +128,9 @@ def load_library(path=LIB_PATH):
         "pdx_get_schedule_stats": (I, [P, I, P]),
         "pdx_set_schedule": (I, [P, I, I]),
         "pdx_set_ordering": (I, [P, I]),
-        "pdx_set_split": (I, [P, I]),
+        "pdx_set_split": (I, [P, C.c_longlong]),
         "pdx_split_count": (I, [P]),
+        "pdx_get_split": (C.c_longlong, [P]),
         "pdx_verify_schedule": (I, [P, I, C.POINTER(D)]),
         "pdx_bytes_per_iteration": (D, [P]),
         "pdx_bytes_per_substep": (D, [P]),
@@ -491,13 +492,17 @@ class Engine:
         code = {"amd": PDX_ORDER_AMD, "nd": PDX_ORDER_ND}[ordering]
         self._check(self.lib.pdx_set_ordering(self.h, code), "set_ordering")
 
-    def set_split(self, min_vertices):
-        """Instances of at least `min_vertices` vertices run on two workgroups (mixed32,
-        fast solve, nested-dissection order, batch launch); 0 turns it off."""
-        self._check(self.lib.pdx_set_split(self.h, int(min_vertices)), "set_split")
+    def set_split(self, min_factor_entries):
+        """Instances whose factor has at least `min_factor_entries` entries run on two
+        workgroups (mixed32, fast solve, nested-dissection order, batch launch); 0
+        turns it off, 1 splits every instance that can be."""
+        self._check(self.lib.pdx_set_split(self.h, int(min_factor_entries)), "set_split")
 
     def split_count(self):
         return self.lib.pdx_split_count(self.h)
+
+    def split_threshold(self):
+        return self.lib.pdx_get_split(self.h)
 
     def verify_schedule(self, i=0):
         """Host replay of the instance's sweep schedules in the kernel's phase order;

@@ -54,8 +54,8 @@ def test_split_equals_one_workgroup(residual_tol, rel_tol):
     _say("one workgroup per instance, residual tol", residual_tol, "rel tol", rel_tol)
     a = _engine(cloths, cfg, 0)
     a.step_graph(1, 12)
-    _say("split instances of >= 1000 vertices")
-    b = _engine(cloths, cfg, 1000)
+    _say("split instances of >= 21000 factor entries")
+    b = _engine(cloths, cfg, 21000)
     assert b.split_count() == 4
     b.step_graph(1, 12)
     early = 0
@@ -92,7 +92,7 @@ def test_split_around_a_rollout():
     cloths = _cloths()[:3]
     cfg = SimConfig.fixed_iterations(8, scenes.H600)
     res = []
-    for split in (0, 1000):
+    for split in (0, 21000):
         _say("rollout + adjoint + step, split", split)
         e = _engine(cloths, cfg, split)
         e.step(3)
@@ -123,7 +123,7 @@ def test_split_full_mix_equals_one_workgroup():
     for c in cloths:
         a.add(c, cfg)
         b.add(c, cfg)
-    for e, split in ((a, 0), (b, 5000)):
+    for e, split in ((a, 0), (b, 300000)):
         e.set_ordering("nd")
         e.set_split(split)
         e.factor()
@@ -134,3 +134,25 @@ def test_split_full_mix_equals_one_workgroup():
     print("kernel ms per launch: one workgroup", a.last_kernel_ms(), "split", b.last_kernel_ms())
     for i in range(len(cloths)):
         assert _same(a.state(i)[0], b.state(i)[0]), i
+
+
+def test_autotune_chooses_split_and_leaves_no_trace():
+    """The autotuner also times split thresholds; its probes are undone (no substep
+    counted, the state as before) and stepping afterwards agrees with an untuned
+    context to the FP32 form's rounding."""
+    cloths = _cloths()
+    cfg = SimConfig.fixed_iterations(12, scenes.H600)
+    _say("autotune with split thresholds")
+    a = _engine(cloths, cfg, 0)
+    b = _engine(cloths, cfg, 0)
+    blk, cap, ms = a.autotune(1)
+    th = a.split_threshold()
+    print("chosen", blk, cap, ms, "split threshold", th, "split instances", a.split_count())
+    sizes = sorted((len(a.factor_arrays(i)["rows"]) for i in range(len(cloths))), reverse=True)
+    assert th == 0 or th in sizes
+    for i in range(len(cloths)):
+        assert a.stats(i)["substeps"] == 0 and a.stats(i)["factorize_count"] == 1
+    a.step(4)
+    b.step(4)
+    for i in range(len(cloths)):
+        assert np.abs(a.state(i)[0] - b.state(i)[0]).max() <= 1e-6

@@ -39,7 +39,7 @@ def test_split_only_where_it_applies():
     e.add(big, CFG)
     e.add(small, CFG)
     e.set_ordering("nd")
-    e.set_split(1000)
+    e.set_split(10000)
     e.verify_schedule(0)
     e.verify_schedule(1)
     assert e.split_count() == 1
