@@ -187,6 +187,17 @@ int pdx_get_stats(pdx_ctx* ctx, int instance, pdx_stats* out);
 /* --- global solve (EigenDirectSolver::solve, EigenDirectSolver.cpp:299-357) ---- */
 /* x = L^{-1} b for one instance, all three components, on the GPU. */
 int pdx_solve3(pdx_ctx* ctx, int instance, const double* b, double* x);
+/* pdx_solve3 without allocating: no counterpart in the reference.  For callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  b and x are staged through a static
+ * device array of the library (393,216 + 393,216 FP64 values, so at most 131,072
+ * vertices; PDX_EINVAL beyond that), held on every device the library runs on
+ * until the process ends, instead of pdx_solve3's per-context buffers.  The
+ * context must already be factored and uploaded (pdx_factor or a step, and no
+ * change since): PDX_ENOTREADY otherwise, as an upload would allocate.  Calls
+ * that overlap on one device share the array and race; same results as
+ * pdx_solve3 bit for bit. */
+int pdx_solve3_noalloc(pdx_ctx* ctx, int instance, const double* b, double* x);
 /* Host copy of the factor (rank space): ptr[m+1], rows[nnz], vals[nnz], dinv[m],
  * rank_vertex[m], perm3[3m]; any pointer may be NULL.  *m / *nnz receive sizes. */
 int pdx_get_factor(pdx_ctx* ctx, int instance, int* m, long long* nnz, int32_t* ptr,

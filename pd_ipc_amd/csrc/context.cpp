@@ -31,6 +31,8 @@ hipError_t launchGrid(const InstDev* d, int inst, int nsub, double* gslots, int 
 hipError_t launchGridSolve(const InstDev* d, int inst, const double* b, double* x, int waveRuns, int blocks,
                            int block, size_t lds, bool fp32, bool idx16, hipStream_t s);
 int gridBlocksMax(int block, size_t lds, bool fp32, bool idx16);
+size_t noallocDofs();
+hipError_t noallocScratch(int device, double** ptr);
 hipError_t launchDense(const InstDev* d, int inst, int nsub, double* gslots, int blocks, int block, size_t lds,
                        bool fp32, hipStream_t s);
 int denseBlocksMax(int block, size_t lds, bool fp32);
@@ -1787,6 +1789,22 @@ int pdx_solve3(pdx_ctx* c, int i, const double* b, double* x) {
     c->scratchN = n3;
   }
   return solveThrough(c, i, b, x, c->dScratchB, c->dScratchX);
+}
+
+int pdx_solve3_noalloc(pdx_ctx* c, int i, const double* b, double* x) {
+  if (!c || !b || !x || i < 0 || i >= (int)c->inst.size()) return PDX_EINVAL;
+  // no upload here (it would allocate): the context must be factored and current
+  if (!c->uploaded || c->dirty)
+    return fail(c, PDX_ENOTREADY, "pdx_solve3_noalloc: factor the context first (pdx_factor or a step); "
+                                  "it allocates nothing, so it does not upload");
+  const size_t n3 = (size_t)3 * c->inst[(size_t)i].scene.n;
+  if (n3 > noallocDofs())
+    return fail(c, PDX_EINVAL, "pdx_solve3_noalloc: at most " + std::to_string(noallocDofs() / 3) +
+                                   " vertices (its static staging); use pdx_solve3");
+  double* s = nullptr;
+  HIPCHECK(c, hipSetDevice(c->device));
+  HIPCHECK(c, noallocScratch(c->device, &s));
+  return solveThrough(c, i, b, x, s, s + noallocDofs());
 }
 
 int pdx_get_factor(pdx_ctx* c, int i, int* m, long long* nnz, int32_t* ptr, int32_t* rows, double* vals,

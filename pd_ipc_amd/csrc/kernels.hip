@@ -20,6 +20,8 @@
 #include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "devtypes.h"
 
 namespace pdx {
@@ -2253,6 +2255,40 @@ hipError_t launchSolve(const InstDev* d, int inst, int inst2, const double* b, d
   PDX_KERNEL_TABLE(PDX_L)
 #undef PDX_L
   return hipErrorInvalidValue;
+}
+
+// Staging of the allocation-free solve (pdx_solve3_noalloc): kNoAllocDofs FP64
+// right-hand-side values and as many solution values.  Every device the library
+// runs on holds its own copy until the process ends; calls that overlap on one
+// device share it and race.
+constexpr size_t kNoAllocDofs = (size_t)3 * 131072;
+static __device__ __align__(256) double scratch[2 * kNoAllocDofs];
+
+size_t noallocDofs() { return kNoAllocDofs; }
+
+// The device address of `scratch` on `device`, looked up once per device (with
+// that device current) and kept: host code uses this address, never the
+// array's name, which on the host is only a stand-in.
+hipError_t noallocScratch(int device, double** ptr) {
+  constexpr int kMaxDevices = 64;
+  static std::mutex mu;
+  static double* addr[kMaxDevices] = {};
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[device]) {
+    int cur = 0;
+    hipError_t e = hipGetDevice(&cur);
+    if (e != hipSuccess) return e;
+    if ((e = hipSetDevice(device)) != hipSuccess) return e;
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    const hipError_t back = hipSetDevice(cur);
+    if (e != hipSuccess) return e;
+    if (back != hipSuccess) return back;
+    addr[device] = static_cast<double*>(p);
+  }
+  *ptr = addr[device];
+  return hipSuccess;
 }
 
 hipError_t setLdsLimit(size_t bytes) {

@@ -112,6 +112,7 @@ def load_library(path=LIB_PATH):
         "pdx_set_config": (I, [P, I, C.POINTER(_Config)]),
         "pdx_get_stats": (I, [P, I, C.POINTER(_Stats)]),
         "pdx_solve3": (I, [P, I, P, P]),
+        "pdx_solve3_noalloc": (I, [P, I, P, P]),
         "pdx_get_factor": (I, [P, I, C.POINTER(I), C.POINTER(C.c_longlong), P, P, P, P, P, P]),
         "pdx_get_topology": (I, [P, I, C.POINTER(I), P, P, P, P, P, P, P]),
         "pdx_pin_count": (I, [P, I]),
