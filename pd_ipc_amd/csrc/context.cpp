@@ -1627,6 +1627,7 @@ int pdx_set_pin_positions(pdx_ctx* c, int i, const double* q) {
 int pdx_set_edge_kappa(pdx_ctx* c, int i, const double* kappa) {
   if (!c || !kappa || i < 0 || i >= (int)c->inst.size()) return PDX_EINVAL;
   Instance& I = c->inst[(size_t)i];
+  if (std::memcmp(I.scene.kappa.data(), kappa, I.scene.kappa.size() * 8) == 0) return PDX_OK;  // unchanged
   const int rc = pullState(c);
   if (rc) return rc;
   std::memcpy(I.scene.kappa.data(), kappa, I.scene.kappa.size() * 8);
@@ -1637,6 +1638,7 @@ int pdx_set_edge_kappa(pdx_ctx* c, int i, const double* kappa) {
 int pdx_set_bend_k(pdx_ctx* c, int i, const double* k) {
   if (!c || !k || i < 0 || i >= (int)c->inst.size()) return PDX_EINVAL;
   Instance& I = c->inst[(size_t)i];
+  if (std::memcmp(I.scene.bk.data(), k, I.scene.bk.size() * 8) == 0) return PDX_OK;  // unchanged
   const int rc = pullState(c);
   if (rc) return rc;
   std::memcpy(I.scene.bk.data(), k, I.scene.bk.size() * 8);
