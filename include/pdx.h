@@ -61,8 +61,9 @@ enum {
 
 enum {
   PDX_LAUNCH_BATCH = 0, /* one workgroup per instance, persistent over the substeps */
-  PDX_LAUNCH_GRID = 1   /* every CU on one instance: cooperative launch, grid barrier per sweep step
-                           (fast solve mode, exactly one instance) */
+  PDX_LAUNCH_GRID = 1   /* every CU on one instance: cooperative launch; sweep steps paced by
+                           per-step completion counters, grid barriers only around the vertex
+                           phases (fast solve mode, exactly one instance) */
 };
 
 typedef struct pdx_ctx pdx_ctx;

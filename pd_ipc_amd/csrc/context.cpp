@@ -776,7 +776,7 @@ int upload(pdx_ctx* c) {
         bstart, bst, sa, sb, sc, bk, vrank, rankv, dinv, W, ist, lst, dst, recE, rkE, recC, rkC, brR, brK;
     size_t f[9], b[9];
     size_t sch[6][15];  // block schedules bf, bb, sf, sb, pf, pb
-    size_t ownLead = 0, ownPart = 0, pairFlags = 0, pairRed = 0, xbuf = 0;
+    size_t ownLead = 0, ownPart = 0, pairFlags = 0, pairRed = 0, xbuf = 0, flow = 0, flowPart = 0;
     int nOwnLead = 0, nOwnPart = 0;
   };
   std::vector<Off> offs(c->inst.size());
@@ -910,6 +910,11 @@ int upload(pdx_ctx* c) {
       o.xbuf = ar.add(nullptr, (size_t)3 * I.fac.m * w);
     }
     o.W = c->ldsW ? 0 : ar.add(nullptr, (size_t)3 * I.fac.m * w);
+    // the grid form's step counters, one 256-byte line each (kFlowStride in the
+    // kernels), and its run partials of one step
+    o.flow = ar.add(nullptr, c->launchMode == 1 ? (size_t)256 * 2 * std::max(1, I.bf.nstep + I.bb.nstep) : 256);
+    o.flowPart = ar.add(nullptr, c->launchMode == 1 ? (size_t)std::max(1, std::max(I.bf.maxRuns, I.bb.maxRuns)) * 3 * w
+                                                    : 16);
     std::vector<int> ist(4, 0);
     std::vector<long long> lst(16, 0);
     std::vector<double> dst(2, -1.0);
@@ -1124,6 +1129,9 @@ int upload(pdx_ctx* c) {
     d.pairRole = 0;
     d.own = nullptr;
     d.nOwn = s.n;
+    d.flow = (int*)(B + o.flow);
+    d.flowPart = (void*)(B + o.flowPart);
+    d.pair.giveUp = dGiveUp;  // also the grid form's flow waits
     if (I.paired) {
       d.rA = I.fac.top[0];
       d.rB = I.fac.top[0] + I.fac.top[1];
@@ -1411,14 +1419,14 @@ bool replaySchedules(const Instance& I, std::string& why, double& relErr) {
 
 // After a launch of split instances: did a pair stop waiting for its partner?
 int checkPairs(pdx_ctx* c) {
-  if (c->npairs == 0 || !c->uploaded || !c->dGiveUp) return PDX_OK;
+  if ((c->npairs == 0 && c->launchMode != 1) || !c->uploaded || !c->dGiveUp) return PDX_OK;
   int gaveUp = 0;
   HIPCHECK(c, hipMemcpy(&gaveUp, c->dGiveUp, sizeof(int), hipMemcpyDeviceToHost));
   if (!gaveUp) return PDX_OK;
   // the pairs' counts no longer agree: the next step re-uploads (counts zero)
   c->dirty = true;
-  return fail(c, PDX_EHIP, "two workgroups of a split instance stopped waiting for each other "
-                           "(the launch's results are incomplete)");
+  return fail(c, PDX_EHIP, "workgroups stopped waiting for each other (a split instance's pair or the "
+                           "grid form's sweep steps): the launch's results are incomplete");
 }
 
 }  // namespace

@@ -165,6 +165,13 @@ struct InstDev {
   const int* own;     // owned vertices, ascending
   BlockSweepDev bfwdS, bbwdS;
   PairDev pair;
+  // grid form: per step of the forward then the backward sweep, the tasks done
+  // and the streaming wavefronts done, cumulative over the sweeps of one launch
+  // (zeroed at its start): [forward done | forward streamed | backward done |
+  // backward streamed], one 256-byte line per counter; and the run partials of
+  // the step in flight (3 Real per run, the most runs of a step)
+  int* flow;
+  void* flowPart;
   // statistics
   int* istats;        // [0] iterations used, [1] converged, [2] early exits
   long long* lstats;  // [0] total iterations, [1] substeps, [2..6] phase ticks

@@ -1637,13 +1637,12 @@ __global__ void __launch_bounds__(512) adjointKernel(const InstDev* __restrict__
 
 // ---------------------------------------------------------------------------
 // Cooperative form for one large instance: the whole grid (one workgroup per CU,
-// all co-resident) advances a single cloth.  Vertex phases are grid-strided; a
-// chain-blocked sweep step is taken by every wavefront of the grid, each
-// streaming the runs of whole tasks into its own slice of LDS and finishing
-// them, followed by one grid barrier; reductions go through per-workgroup
-// slots and every workgroup reduces them in the same order, so all take the
-// same stop decision.  W lives in HBM.  Same arithmetic per row as the
-// single-workgroup fast sweep: run partials summed in run order.
+// all co-resident) advances a single cloth.  Vertex phases are grid-strided and
+// end in a grid barrier; the chain-blocked sweeps are paced by step counters
+// (flowSweep), not by barriers; reductions go through per-workgroup slots and
+// every workgroup reduces them in the same order, so all take the same stop
+// decision.  W lives in HBM.  Same arithmetic per row as the single-workgroup
+// fast sweep: run partials summed in run order.
 
 namespace cg = cooperative_groups;
 
@@ -1682,37 +1681,136 @@ __device__ inline void gridMax2(cg::grid_group& grid, double a, double b, double
   rb = blockMax(mb, red + 24);
 }
 
+// W of the grid form as the flow sweeps read it: relaxed agent-scope atomic loads,
+// coherent across the XCDs' L2 caches without a write-back per task.
+template <typename Real>
+struct AgentW {
+  Real* p;
+  __device__ Real operator[](int i) const { return __hip_atomic_load(p + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+};
+template <typename Real>
+__device__ inline void agentStore(Real* p, Real v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+constexpr long long kFlowSpin = 1LL << 26;  // polls before a flow wait gives up (~2 s at the back-off cap)
+constexpr int kFlowSolo = 2;  // runs per lane up to which a one-task step streams and finishes on one wavefront
+constexpr int kFlowStride = 64;             // ints between two step counters (one 256-byte line each)
+
+// Waits (the whole wavefront) until *counter >= target.  Most of the grid's
+// wavefronts wait on some counter at any time, so the polls back off (s_sleep 1,
+// 2, then 4 apart) to keep the counters' memory channels free for the work.
+// Bounded: past kFlowSpin polls, or when another wavefront gave up (checked every
+// 64 polls), it sets *giveUp and returns false.
+__device__ inline bool flowWait(const int* counter, int target, int* giveUp) {
+  long long polls = 0;
+  int nap = 1;
+  for (;;) {
+    const int v = __builtin_amdgcn_readfirstlane(__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    if (v >= target) return true;
+    ++polls;
+    if ((polls & 63) == 0 || polls > kFlowSpin) {
+      const int g = __builtin_amdgcn_readfirstlane(__hip_atomic_load(giveUp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+      if (g != 0 || polls > kFlowSpin) {
+        __hip_atomic_store(giveUp, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return false;
+      }
+    }
+    switch (nap) {  // s_sleep takes an immediate
+      case 1: __builtin_amdgcn_s_sleep(1); break;
+      case 2: __builtin_amdgcn_s_sleep(2); break;
+      default: __builtin_amdgcn_s_sleep(4); break;
+    }
+    if (nap < 4) nap *= 2;
+  }
+}
+
+// A chain-blocked sweep of the grid form without grid barriers.  Every step has
+// two phases, each paced by a counter (cumulative over the sweeps of the launch,
+// so step st's targets are rep times its sizes):
+//  * streaming: the step's runs are dealt to the first S = ceil(runs / 64) of the
+//    grid's wavefronts, one run per lane at a time; each turns its runs into
+//    partial sums in `part` (global, step-relative run index) and counts itself
+//    in streamed[st].  A streamer first waits until every task of step st-1 is
+//    done (done[st-1]; step 0 waits on prev / prevTarget when given), which by
+//    induction completes every earlier step.
+//  * finishing: the step's tasks go to the wavefronts round-robin (global index);
+//    a task waits until all S streamers counted (or, for a step without runs,
+//    until step st-1 is done), subtracts its rows' partials in run order,
+//    finishes its chain segments and counts itself in done[st].
+// A step of one task with at most kFlowSolo runs per lane skips the hand-off:
+// its wavefront streams the runs into its own LDS slice (Pw) and finishes.
+// A wavefront goes on to the next step as soon as it has nothing left in this
+// one.  W and the partials cross workgroups as agent-scope atomics (AgentW /
+// agentStore), and a wavefront counts itself only once its stores are
+// performed.  The arithmetic of each row is the one-workgroup sweep's.  Returns
+// false when a wait gave up.
 template <typename Real, bool kFwd, bool kIdx16, typename Store>
-__device__ void gridSweep(cg::grid_group& grid, const BlockSweepDev& sw, PDX_GLOBAL Real* W, PDX_LDS Real* Pw,
-                          Store store) {
+__device__ bool flowSweep(const BlockSweepDev& sw, Real* Wp, Real* part, PDX_LDS Real* Pw, Store store, int* done,
+                          int* streamed, int rep, const int* prev, int prevTarget, int* giveUp) {
   const int lane = threadIdx.x & (kWave - 1);
   const int nw = blockDim.x / kWave;
   const int gw = blockIdx.x * nw + (int)(threadIdx.x / kWave);
   const int GW = gridDim.x * nw;
+  const AgentW<Real> W{Wp};
+  const AgentW<Real> P{part};
   const Real* rval = static_cast<const Real*>(sw.runVal);
   const PDX_GLOBAL Real* gdense = gview<Real>(sw.dense);
   const PDX_GLOBAL v4i* gmeta = gview<v4i>(sw.laneMeta);
   const PDX_GLOBAL Real* gldinv = gview<Real>(sw.laneDinv);
   for (int st = 0; st < sw.nstep; ++st) {
     const int tb = sw.stepTask[st], te = sw.stepTask[st + 1];
-    const int rbase = sw.stepRun[st], dbase = sw.stepDense[st];
+    const int rbase = sw.stepRun[st], nr = sw.stepRun[st + 1] - rbase, dbase = sw.stepDense[st];
+    const bool solo = te - tb == 1 && nr <= kFlowSolo * kWave;
+    const int S = solo ? 0 : min(GW, (nr + kWave - 1) / kWave);
+    const int* prevOn = st > 0 ? done + (size_t)(st - 1) * kFlowStride : prev;
+    const int prevFor = st > 0 ? rep * (sw.stepTask[st] - sw.stepTask[st - 1]) : prevTarget;
+    int* streamedSt = streamed + (size_t)st * kFlowStride;
+    if (gw < S) {
+      // streaming: runs gw*64 + lane, then S*64 further; the first one loads before the wait
+      RunRaw<Real, kIdx16> raw;
+      int r = gw * kWave + lane;
+      loadRun<Real, kIdx16>(sw.runCol, rval, rbase + (r < nr ? r : 0), raw);
+      if (prevOn && !flowWait(prevOn, prevFor, giveUp)) return false;
+      for (; r < nr; r += S * kWave) {
+        if (r != gw * kWave + lane) loadRun<Real, kIdx16>(sw.runCol, rval, rbase + r, raw);
+        Real q[3];
+        runPartial<Real, kIdx16>(raw, W, q);
+        agentStore(part + 3 * r + 0, q[0]);
+        agentStore(part + 3 * r + 1, q[1]);
+        agentStore(part + 3 * r + 2, q[2]);
+      }
+      storesPerformed();
+      if (lane == 0) __hip_atomic_fetch_add(streamedSt, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    bool waited = false;
     for (int t = tb + gw; t < te; t += GW) {
       const v4i mt = gmeta[(size_t)t * kWave + lane];
       const Real dv = kFwd ? Real(0) : gldinv[(size_t)t * kWave + lane];
       const int doff = gview<int>(sw.laneDense)[(size_t)t * kWave + lane];
       const int k = mt.x;
       const int info = mt.w;
-      const int used = (info >> 14) & 0x7f;
       const bool act = k >= 0;
-      const int rb = __builtin_amdgcn_readlane(mt.y, 0);
-      const int re = __builtin_amdgcn_readlane(mt.z, used - 1);
-      for (int r = rb + lane; r < re; r += kWave) {
-        RunRaw<Real, kIdx16> raw;
-        loadRun<Real, kIdx16>(sw.runCol, rval, rbase + r, raw);
-        runPartial<Real, kIdx16>(raw, W, Pw + 3 * (r - rb));
+      if (solo) {
+        // the step's only task streams its own runs (<= kFlowSolo per lane) into LDS
+        RunRaw<Real, kIdx16> raw[kFlowSolo];
+#pragma unroll
+        for (int u = 0; u < kFlowSolo; ++u) {
+          const int r = lane + u * kWave;
+          loadRun<Real, kIdx16>(sw.runCol, rval, rbase + (r < nr ? r : 0), raw[u]);
+        }
+        if (prevOn && !flowWait(prevOn, prevFor, giveUp)) return false;
+#pragma unroll
+        for (int u = 0; u < kFlowSolo; ++u)
+          if (lane + u * kWave < nr) runPartial<Real, kIdx16>(raw[u], W, Pw + 3 * (lane + u * kWave));
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+      } else if (!waited) {
+        const bool ok = S > 0 ? flowWait(streamedSt, rep * S, giveUp)
+                              : (prevOn ? flowWait(prevOn, prevFor, giveUp) : true);
+        if (!ok) return false;
+        waited = true;
       }
-      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-      __builtin_amdgcn_wave_barrier();
       Real a0 = 0, a1 = 0, a2 = 0;
       if (act) {
         a0 = W[3 * k + 0];
@@ -1723,21 +1821,30 @@ __device__ void gridSweep(cg::grid_group& grid, const BlockSweepDev& sw, PDX_GLO
           a1 = a1 * dv;
           a2 = a2 * dv;
         }
-        subtractPartials<Real>(Pw - 3 * rb, mt.y, mt.z, a0, a1, a2);
+        if (solo)
+          subtractPartials<Real>(Pw, mt.y, mt.z, a0, a1, a2);
+        else
+          subtractPartials<Real>(P, mt.y, mt.z, a0, a1, a2);
       }
       segmentFinish<Real>(gdense + dbase + doff, info, lane, act, a0, a1, a2);
       if (act) {
-        W[3 * k + 0] = a0;
-        W[3 * k + 1] = a1;
-        W[3 * k + 2] = a2;
+        agentStore(Wp + 3 * k + 0, a0);
+        agentStore(Wp + 3 * k + 1, a1);
+        agentStore(Wp + 3 * k + 2, a2);
         store(k, a0, a1, a2);
       }
-      // the next task of this wavefront reuses the partial slice
-      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-      __builtin_amdgcn_wave_barrier();
+      // the task's stores are performed before it counts as complete
+      storesPerformed();
+      if (lane == 0)
+        __hip_atomic_fetch_add(done + (size_t)st * kFlowStride, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (solo) {
+        // the LDS slice is this wavefront's; the next task may overwrite it
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+      }
     }
-    grid.sync();
   }
+  return true;
 }
 
 constexpr int kDR = 8;  // rows per wavefront pass of the dense product (4: FP64 config 2 9.32 ms vs 8.72)
@@ -1845,6 +1952,13 @@ __global__ void __launch_bounds__(512) gridSubstepKernel(const InstDev* __restri
   const int gs = gridDim.x * blockDim.x;
   const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
   int buf = 0;
+  // the flow sweeps' step counters start at zero in every launch (ordered before
+  // the first sweep by the grid barrier after the first right-hand side)
+  const int nF = I.bfwd.nstep, nB = I.bbwd.nstep;
+  if (!kDense && blockIdx.x == 0)
+    for (int i = threadIdx.x; i < 2 * (nF + nB); i += blockDim.x) I.flow[(size_t)i * kFlowStride] = 0;
+  int rep = 0;  // sweeps of this launch
+  const int lastF = nF > 0 ? I.bfwd.stepTask[nF] - I.bfwd.stepTask[nF - 1] : 0;
   for (int sub = 0; sub < nsub; ++sub) {
     // predict (Integrator.cpp:447-463), displacement scale, first right-hand side
     double lscale = 0.0;
@@ -1912,8 +2026,17 @@ __global__ void __launch_bounds__(512) gridSubstepKernel(const InstDev* __restri
           gridDense<Real>(grid, I, W, update);
         }
       } else {
-        gridSweep<Real, true, kIdx16>(grid, I.bfwd, W, Pw, none);
-        gridSweep<Real, false, kIdx16>(grid, I.bbwd, W, Pw, put);
+        // sweeps paced by step counters, one grid barrier once the backward sweep is done
+        ++rep;
+        int* doneF = I.flow;
+        int* doneB = I.flow + (size_t)2 * nF * kFlowStride;
+        Real* part = static_cast<Real*>(I.flowPart);
+        if (flowSweep<Real, true, kIdx16>(I.bfwd, (Real*)W, part, Pw, none, doneF, doneF + (size_t)nF * kFlowStride,
+                                          rep, nullptr, 0, I.pair.giveUp))
+          flowSweep<Real, false, kIdx16>(I.bbwd, (Real*)W, part, Pw, put, doneB, doneB + (size_t)nB * kFlowStride, rep,
+                                          nF > 0 ? doneF + (size_t)(nF - 1) * kFlowStride : nullptr, rep * lastF,
+                                          I.pair.giveUp);
+        grid.sync();
       }
       if (!kExact && !kDense) {
         const PDX_GLOBAL Real* U = W;
@@ -2015,6 +2138,9 @@ __global__ void __launch_bounds__(512) gridSolveKernel(const InstDev* __restrict
   const int gt = blockIdx.x * blockDim.x + threadIdx.x;
   const int gs = gridDim.x * blockDim.x;
   for (int v = gt; v < I.n; v += gs) putW<Real>(W, I.vrank[v], b[3 * v], b[3 * v + 1], b[3 * v + 2]);
+  const int nF = I.bfwd.nstep, nB = I.bbwd.nstep;
+  if (blockIdx.x == 0)
+    for (int i = threadIdx.x; i < 2 * (nF + nB); i += blockDim.x) I.flow[(size_t)i * kFlowStride] = 0;
   grid.sync();
   auto none = [](int, Real, Real, Real) {};
   const int* rankv = I.rankv;
@@ -2024,8 +2150,14 @@ __global__ void __launch_bounds__(512) gridSolveKernel(const InstDev* __restrict
     xout[3 * v + 1] = (double)a1;
     xout[3 * v + 2] = (double)a2;
   };
-  gridSweep<Real, true, kIdx16>(grid, I.bfwd, W, Pw, none);
-  gridSweep<Real, false, kIdx16>(grid, I.bbwd, W, Pw, put);
+  const int lastF = nF > 0 ? I.bfwd.stepTask[nF] - I.bfwd.stepTask[nF - 1] : 0;
+  int* doneF = I.flow;
+  int* doneB = I.flow + (size_t)2 * nF * kFlowStride;
+  Real* part = static_cast<Real*>(I.flowPart);
+  if (flowSweep<Real, true, kIdx16>(I.bfwd, (Real*)W, part, Pw, none, doneF, doneF + (size_t)nF * kFlowStride, 1,
+                                    nullptr, 0, I.pair.giveUp))
+    flowSweep<Real, false, kIdx16>(I.bbwd, (Real*)W, part, Pw, put, doneB, doneB + (size_t)nB * kFlowStride, 1,
+                                    nF > 0 ? doneF + (size_t)(nF - 1) * kFlowStride : nullptr, lastF, I.pair.giveUp);
 }
 
 // Dense solve mode, x = Linv b: W = P b (rank space) ...
