@@ -36,20 +36,28 @@ def _run(cmd):
 def build(verbose=False):
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
-    objs = []
+    objs, procs = [], []
+    # every source compiles in its own process, all at once
     for src in SOURCES:
         path = os.path.join(CSRC, src)
         obj = os.path.join(objdir, src + ".o")
-        cmd = [HIPCC] + COMMON + ["-c", "-o", obj]
         if src.endswith(".hip"):
             cmd = [HIPCC] + COMMON + DEVICE + ["-x", "hip", "-c", "-o", obj]
         else:
             cmd = [HIPCC] + COMMON + ["-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-x", "c++", "-c", "-o", obj]
         cmd.append(path)
-        out = _run(cmd)
-        if verbose and out:
-            print(out)
+        procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
         objs.append(obj)
+    failed = None
+    for cmd, p in procs:
+        out, _ = p.communicate()
+        if p.returncode != 0:
+            sys.stderr.write(" ".join(cmd) + "\n" + out)
+            failed = failed or cmd[-1]
+        elif verbose and out:
+            print(out)
+    if failed:
+        raise RuntimeError(f"build step failed: {failed}")
     tmp = OUT + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs + ["-lpthread"])
     os.replace(tmp, OUT)

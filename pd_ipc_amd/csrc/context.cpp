@@ -138,6 +138,12 @@ struct HostBlockSweep {
   std::vector<int> laneDense;                     // 64 lanes x offset of the lane's inverse block
   std::vector<int> runCol;                        // kRun per run
   std::vector<double> runVal, dense;
+  // level form (grid): per-task runs, phase B block products, the long chains'
+  // inverse panels and the R range (BlockSweepDev)
+  bool levels = false;
+  std::vector<int> taskRun, stepB, bTask;
+  std::vector<double> xpanel;
+  int rTotal = 0;
 };
 
 // Block schedule of one sweep (fast mode).  Ranks are cut into contiguous
@@ -377,6 +383,257 @@ void packBlocked(const Factor& f, bool fwd, int cap, int runCap, int stepRunCap,
     closeStep();
   }
   out.nstep = steps;
+}
+
+// Inverse X = B^-1 of the unit lower triangle B of ranks [s0, s0 + nr) (the
+// factor's own entries among them), FP64, row-major nr x nr, unit diagonal
+// included: row t of X is e_t minus the factor's entries of row t times the rows
+// of X before it (B X = I, taken row by row).
+void chainInverse(const Factor& f, int s0, int nr, std::vector<double>& X) {
+  X.assign((size_t)nr * nr, 0.0);
+  for (int t = 0; t < nr; ++t) {
+    double* xt = X.data() + (size_t)t * nr;
+    xt[t] = 1.0;
+    const int k = s0 + t;
+    for (int p = f.rp[(size_t)k]; p < f.rp[(size_t)k + 1]; ++p) {
+      const int q = f.rc[(size_t)p] - s0;
+      if (q < 0 || q >= t) continue;
+      const double l = f.rx[(size_t)p];
+      const double* xq = X.data() + (size_t)q * nr;
+      for (int c = 0; c <= q; ++c) xt[c] -= l * xq[c];
+    }
+  }
+}
+
+// Level schedules of both sweeps for the grid form.  Blocks are whole
+// elimination-tree chains (a nested-dissection separator is one block), so a
+// sweep takes one step per level of the block tree instead of one per piece of
+// a chain.  A step has two phases:
+//  * phase A: tasks of up to 64 lanes, each task streaming its own runs (at most
+//    waveRuns, one wavefront's LDS slice) into partial sums.  A block of at most
+//    `cap` rows whose runs fit one task is finished inside the task, its rows in
+//    consecutive lanes, with its inverse rows in registers (as packBlocked's
+//    chain blocks).  A longer block (a long chain) gives each row a lane that
+//    subtracts its external partials and writes the result r to R.
+//  * phase B: each long chain's inverse applied to its r, y = X r forward and
+//    y = X^T r backward, 8 rows per task, the rows of X stored as panels: forward
+//    panel p holds rows 8p.. of X over columns [0, 8p + 8), backward panel p rows
+//    8p.. of X^T over columns [8p, the chain's end), both padded with zeros to a
+//    multiple of 4 columns.
+constexpr int kWaveLanes = 64;     // lanes of a wavefront
+constexpr long long kLevelWaves = 1024;  // wavefronts a level step's runs are spread over
+
+void packLevels(const Factor& f, int cap, int waveRuns, HostBlockSweep& outF, HostBlockSweep& outB) {
+  const int m = f.m;
+  std::vector<int> parent((size_t)m, -1), nchild((size_t)m, 0);
+  for (int j = 0; j < m; ++j)
+    if (f.cp[(size_t)j + 1] > f.cp[(size_t)j]) {
+      parent[(size_t)j] = f.ci[(size_t)f.cp[(size_t)j]];
+      nchild[(size_t)parent[(size_t)j]]++;
+    }
+  std::vector<int> bs{0};
+  for (int k = 1; k < m; ++k)
+    if (!(parent[(size_t)k - 1] == k && nchild[(size_t)k] == 1)) bs.push_back(k);
+  bs.push_back(m);
+  if (m == 0) bs.assign(1, 0);
+  const int nb = (int)bs.size() - 1;
+  std::vector<int> blk((size_t)m, 0);
+  for (int b = 0; b < nb; ++b)
+    for (int k = bs[(size_t)b]; k < bs[(size_t)b + 1]; ++k) blk[(size_t)k] = b;
+  std::vector<std::vector<double>> Xc((size_t)nb);  // long chains' inverses, formed once for both sweeps
+  auto round4 = [](int v) { return (v + 3) & ~3; };
+  for (int dir = 0; dir < 2; ++dir) {
+    const bool fwd = dir == 0;
+    HostBlockSweep& out = fwd ? outF : outB;
+    out = HostBlockSweep{};
+    out.levels = true;
+    out.maxRuns = 1;
+    // external entries of row k of block b: forward its columns before the block,
+    // backward the rows after the block in its column
+    auto extRange = [&](int k, int b, int& p0, int& p1) {
+      if (fwd) {
+        p0 = f.rp[(size_t)k];
+        p1 = p0;
+        while (p1 < f.rp[(size_t)k + 1] && f.rc[(size_t)p1] < bs[(size_t)b]) ++p1;
+      } else {
+        p1 = f.cp[(size_t)k + 1];
+        p0 = p1;
+        while (p0 > f.cp[(size_t)k] && f.ci[(size_t)p0 - 1] >= bs[(size_t)b + 1]) --p0;
+      }
+    };
+    auto rowRuns = [&](int k, int b) {
+      int p0, p1;
+      extRange(k, b, p0, p1);
+      return (p1 - p0 + kRun - 1) / kRun;
+    };
+    std::vector<int> blev((size_t)nb, 0);
+    int nlev = 0;
+    for (int q = 0; q < nb; ++q) {
+      const int b = fwd ? q : nb - 1 - q;
+      int lv = 0;
+      for (int k = bs[(size_t)b]; k < bs[(size_t)b + 1]; ++k) {
+        int p0, p1;
+        extRange(k, b, p0, p1);
+        for (int p = p0; p < p1; ++p) {
+          const int o = fwd ? f.rc[(size_t)p] : f.ci[(size_t)p];
+          lv = std::max(lv, blev[(size_t)blk[(size_t)o]] + 1);
+        }
+      }
+      blev[(size_t)b] = lv;
+      nlev = std::max(nlev, lv + 1);
+    }
+    if (nb == 0) nlev = 0;
+    std::vector<std::vector<int>> byLevel((size_t)nlev);
+    for (int b = 0; b < nb; ++b) byLevel[(size_t)blev[(size_t)b]].push_back(b);
+    std::vector<int> rpos((size_t)nb, -1);
+    out.stepTask.push_back(0);
+    out.stepRun.push_back(0);
+    out.stepDense.push_back(0);
+    out.stepB.push_back(0);
+    struct Item {
+      int block, len, runs, row;  // row >= 0: one row of a long chain
+    };
+    for (int l = 0; l < nlev; ++l) {
+      std::vector<Item> items;
+      std::vector<int> longChains;
+      for (int b : byLevel[(size_t)l]) {
+        const int s0 = bs[(size_t)b], len = bs[(size_t)b + 1] - s0;
+        int runs = 0;
+        for (int k = s0; k < s0 + len; ++k) runs += rowRuns(k, b);
+        if (len <= cap && runs <= waveRuns) {
+          items.push_back(Item{b, len, runs, -1});
+        } else {
+          longChains.push_back(b);
+          rpos[(size_t)b] = out.rTotal;
+          out.rTotal += round4(len);
+          for (int k = s0; k < s0 + len; ++k) items.push_back(Item{b, 1, rowRuns(k, b), k});
+        }
+      }
+      std::stable_sort(items.begin(), items.end(), [](const Item& a, const Item& b) {
+        return a.len != b.len ? a.len > b.len : a.runs > b.runs;
+      });
+      // a task's run budget spreads the step over about kLevelWaves wavefronts (at
+      // least one run per lane, at most the LDS slice)
+      long long stepRuns = 0;
+      for (const Item& it : items) stepRuns += it.runs;
+      const int budget = (int)std::min<long long>(waveRuns, std::max<long long>(kWaveLanes, (stepRuns + kLevelWaves - 1) / kLevelWaves));
+      std::vector<std::vector<Item>> tasks;
+      {
+        std::vector<Item> cur;
+        int lanes = 0, truns = 0;
+        for (const Item& it : items) {
+          if (!cur.empty() && (lanes + it.len > 64 || truns + it.runs > budget)) {
+            tasks.push_back(cur);
+            cur.clear();
+            lanes = truns = 0;
+          }
+          cur.push_back(it);
+          lanes += it.len;
+          truns += it.runs;
+        }
+        if (!cur.empty()) tasks.push_back(cur);
+      }
+      const int denseBase = out.stepDense.back();
+      for (const auto& t : tasks) {
+        int used = 0, tr = 0, maxLen = 1;
+        for (const Item& it : t) {
+          used += it.len;
+          tr += it.runs;
+          maxLen = std::max(maxLen, it.len);
+        }
+        out.maxTaskRuns = std::max(out.maxTaskRuns, tr);
+        const int runBase = (int)(out.runCol.size() / kRun);
+        out.taskRun.push_back(runBase);
+        out.taskInfo.push_back(used | (maxLen > 1 ? (1 << 16) : 0));
+        out.taskSlot.push_back((int)out.slotRank.size());
+        out.taskDense.push_back((int)out.dense.size() - denseBase);
+        const size_t meta0 = out.laneMeta.size();
+        const int common = (used << 14) | (maxLen << 21);
+        for (int ln = 0; ln < 64; ++ln) {
+          out.laneMeta.push_back(-1);
+          out.laneMeta.push_back(0);
+          out.laneMeta.push_back(0);
+          out.laneMeta.push_back(1 | common);
+          out.laneDinv.push_back(0.0);
+          out.laneDense.push_back(0);
+        }
+        int lane = 0, runs = 0;
+        for (const Item& it : t) {
+          const int b = it.block, s0 = bs[(size_t)b];
+          const int nr = it.row >= 0 ? 1 : it.len, k0 = it.row >= 0 ? it.row : s0;
+          const int doff = (int)out.dense.size() - denseBase;
+          for (int r = 0; r < nr; ++r, ++lane) {
+            const int k = k0 + r;
+            int p0, p1;
+            extRange(k, b, p0, p1);
+            out.slotRank.push_back(k);
+            out.slotRunBeg.push_back(runs);
+            for (int p = p0; p < p1; p += kRun) {
+              for (int q = 0; q < kRun; ++q) {
+                if (p + q < p1) {
+                  out.runCol.push_back(fwd ? f.rc[(size_t)(p + q)] : f.ci[(size_t)(p + q)]);
+                  out.runVal.push_back(fwd ? f.rx[(size_t)(p + q)] : f.cx[(size_t)(p + q)]);
+                } else {
+                  out.runCol.push_back(0);
+                  out.runVal.push_back(0.0);
+                }
+              }
+              ++runs;
+            }
+            out.slotRunEnd.push_back(runs);
+            const size_t mi = meta0 + 4 * (size_t)lane;
+            out.laneMeta[mi + 0] = k;
+            out.laneMeta[mi + 1] = out.slotRunBeg.back();
+            out.laneMeta[mi + 2] = runs;
+            out.laneMeta[mi + 3] = nr | (r << 7) | common | (it.row >= 0 ? (1 << 28) : 0);
+            out.laneDinv[meta0 / 4 + (size_t)lane] = f.dinv[(size_t)k];
+            out.laneDense[meta0 / 4 + (size_t)lane] =
+                it.row >= 0 ? rpos[(size_t)b] + (k - s0) : (nr > 1 ? doff : 0);
+          }
+          if (it.row < 0 && nr > 1) {
+            // a short chain: forward [s * nr + t] = X(t, s), backward X(s, t)
+            std::vector<double> X;
+            chainInverse(f, s0, nr, X);
+            for (int sc = 0; sc < nr; ++sc)
+              for (int tt = 0; tt < nr; ++tt)
+                out.dense.push_back(fwd ? X[(size_t)tt * nr + sc] : X[(size_t)sc * nr + tt]);
+          }
+        }
+      }
+      while ((out.dense.size() - (size_t)denseBase) % 4) out.dense.push_back(0.0);
+      out.maxDense = std::max(out.maxDense, (int)out.dense.size() - denseBase);
+      // phase B: the long chains' panels
+      for (int b : longChains) {
+        const int s0 = bs[(size_t)b], s = bs[(size_t)b + 1] - s0, spad = round4(s);
+        std::vector<double>& X = Xc[(size_t)b];
+        if (X.empty()) chainInverse(f, s0, s, X);
+        for (int r0 = 0; r0 < s; r0 += 8) {
+          const int nrows = std::min(8, s - r0);
+          const int c0 = fwd ? 0 : r0;
+          const int ld = fwd ? round4(std::min(s, r0 + 8)) : spad - r0;
+          const int xoff = (int)out.xpanel.size();
+          for (int q = 0; q < nrows; ++q)
+            for (int j = 0; j < ld; ++j) {
+              const int t = r0 + q, u = c0 + j;  // output row t, input column u
+              double v = 0.0;
+              if (u < s) v = fwd ? (u <= t ? X[(size_t)t * s + u] : 0.0) : (u >= t ? X[(size_t)u * s + t] : 0.0);
+              out.xpanel.push_back(v);
+            }
+          out.bTask.push_back(xoff);
+          out.bTask.push_back(ld);
+          out.bTask.push_back(rpos[(size_t)b] + c0);
+          out.bTask.push_back((s0 + r0) | (nrows << 24));
+        }
+      }
+      out.stepTask.push_back((int)out.taskInfo.size());
+      out.stepRun.push_back((int)(out.runCol.size() / kRun));
+      out.stepDense.push_back((int)out.dense.size());
+      out.stepB.push_back((int)(out.bTask.size() / 4));
+    }
+    out.taskRun.push_back((int)(out.runCol.size() / kRun));
+    out.nstep = nlev;
+    if (out.rTotal == 0) out.rTotal = 4;
+  }
 }
 
 struct Instance {
@@ -645,10 +902,9 @@ int refreshFactors(pdx_ctx* c, bool& changed) {
             }
           };
           if (c->launchMode == 1) {
-            // grid form: a task's runs fit one wavefront's LDS slice; steps are
-            // not staged in LDS
-            const int big = 1 << 29;
-            packAll(c->waveRuns, big, big);
+            // grid form: one step per level of whole chains, a task's runs within
+            // one wavefront's LDS slice
+            packLevels(I.fac, c->chainCap, c->waveRuns, I.bf, I.bb);
           } else {
             // per-step LDS caps: run partials <= 48 KB; FP64 stages the chain
             // blocks' inverses too (<= 32 KB), FP32 loads them into registers
@@ -751,9 +1007,11 @@ int upload(pdx_ctx* c) {
       if (I.cfg.chebRho > 0.0) return fail(c, PDX_EUNSUPPORTED, "the grid form has no Chebyshev blend");
     c->ldsW = false;
     c->lds = kRedBytes + (size_t)(c->block / 64) * c->waveRuns * 3 * w;
-    for (auto& I : c->inst)
+    for (auto& I : c->inst) {
       if (std::max(I.bf.maxTaskRuns, I.bb.maxTaskRuns) > c->waveRuns)
         return fail(c, PDX_EUNSUPPORTED, "a factor row streams more runs than one wavefront's LDS slice");
+      if (I.fac.m >= (1 << 24)) return fail(c, PDX_EUNSUPPORTED, "the grid form takes fewer than 2^24 vertices");
+    }
   }
   const bool dense = c->mode == PDX_SOLVE_DENSE;
   if (dense) {
@@ -775,7 +1033,7 @@ int upload(pdx_ctx* c) {
     size_t x, v, xbs, xhat, prev, mass, pinned, pinpos, brhs, ea, eb, rest, kappa, vstart, vedges, vedgesC,
         bstart, bst, sa, sb, sc, bk, vrank, rankv, dinv, W, ist, lst, dst, recE, rkE, recC, rkC, brR, brK;
     size_t f[9], b[9];
-    size_t sch[6][15];  // block schedules bf, bb, sf, sb, pf, pb
+    size_t sch[6][19];  // block schedules bf, bb, sf, sb, pf, pb
     size_t ownLead = 0, ownPart = 0, pairFlags = 0, pairRed = 0, xbuf = 0, flow = 0, flowPart = 0;
     int nOwnLead = 0, nOwnPart = 0;
   };
@@ -887,6 +1145,10 @@ int upload(pdx_ctx* c) {
       bo[q][12] = ar.vec(h.laneMeta);
       bo[q][13] = fp32 ? ar.vec(castVec<float>(h.laneDinv)) : ar.vec(h.laneDinv);
       bo[q][14] = ar.vec(h.laneDense);
+      bo[q][15] = ar.vec(h.taskRun);
+      bo[q][16] = ar.vec(h.stepB);
+      bo[q][17] = ar.vec(h.bTask);
+      bo[q][18] = fp32 ? ar.vec(castVec<float>(h.xpanel)) : ar.vec(h.xpanel);
       if (c->idx16) {
         std::vector<uint16_t> c16(h.runCol.begin(), h.runCol.end());
         bo[q][9] = ar.vec(c16);
@@ -909,12 +1171,12 @@ int upload(pdx_ctx* c) {
       o.pairRed = ar.add(nullptr, 16 * sizeof(double));
       o.xbuf = ar.add(nullptr, (size_t)3 * I.fac.m * w);
     }
-    o.W = c->ldsW ? 0 : ar.add(nullptr, (size_t)3 * I.fac.m * w);
+    // W in HBM: 3 Real per rank, 4 in the grid form's level sweeps
+    o.W = c->ldsW ? 0 : ar.add(nullptr, (size_t)(c->launchMode == 1 && !dense ? 4 : 3) * I.fac.m * w);
     // the grid form's step counters, one 256-byte line each (kFlowStride in the
-    // kernels), and its run partials of one step
+    // kernels), and R (zero, so its padding stays zero)
     o.flow = ar.add(nullptr, c->launchMode == 1 ? (size_t)256 * 2 * std::max(1, I.bf.nstep + I.bb.nstep) : 256);
-    o.flowPart = ar.add(nullptr, c->launchMode == 1 ? (size_t)std::max(1, std::max(I.bf.maxRuns, I.bb.maxRuns)) * 3 * w
-                                                    : 16);
+    o.flowPart = ar.add(nullptr, c->launchMode == 1 ? (size_t)3 * std::max(I.bf.rTotal, I.bb.rTotal) * w : 16);
     std::vector<int> ist(4, 0);
     std::vector<long long> lst(16, 0);
     std::vector<double> dst(2, -1.0);
@@ -1111,6 +1373,12 @@ int upload(pdx_ctx* c) {
       bd->laneMeta = (const int*)(B + bo[12]);
       bd->laneDinv = B + bo[13];
       bd->laneDense = (const int*)(B + bo[14]);
+      bd->taskRun = (const int*)(B + bo[15]);
+      bd->stepB = (const int*)(B + bo[16]);
+      bd->bTask = (const int*)(B + bo[17]);
+      bd->xpanel = B + bo[18];
+      bd->rTotal = h.rTotal;
+      bd->pad = 0;
     };
     fillSweep(&d.bfwd, o.sch[0], I.bf);
     fillSweep(&d.bbwd, o.sch[1], I.bb);
@@ -1295,9 +1563,17 @@ int launch(pdx_ctx* c, int nsub) {
 // A read of a rank the phase does not hold, or of a value not final yet in the
 // sweep's sense, or an index outside its array is an error (returned in `why`).
 // relErr: max |x - x_direct| / max |x_direct| against a plain LDL^T solve.
+bool replayLevels(const Instance& I, std::string& why, double& relErr, const std::vector<double>& b,
+                  std::vector<double>& W);
+
 bool replaySchedules(const Instance& I, std::string& why, double& relErr) {
   const Factor& f = I.fac;
   const int m = f.m;
+  if (I.bf.levels) {
+    std::vector<double> b((size_t)m), W;
+    for (int k = 0; k < m; ++k) b[(size_t)k] = std::sin(0.37 * k + 1.0) + 0.01 * (k % 7);
+    return replayLevels(I, why, relErr, b, W);
+  }
   const int rA = I.paired ? f.top[0] : m, rB = I.paired ? f.top[0] + f.top[1] : m;
   std::vector<double> b((size_t)m), W((size_t)m);
   for (int k = 0; k < m; ++k) b[(size_t)k] = std::sin(0.37 * k + 1.0) + 0.01 * (k % 7);
@@ -1402,6 +1678,154 @@ bool replaySchedules(const Instance& I, std::string& why, double& relErr) {
   }
   if (!ok) return false;
   // the direct solve: L y = b (rows ascending), y /= D, L^T x = y
+  std::vector<double> x = b;
+  for (int k = 0; k < m; ++k)
+    for (int p = f.rp[(size_t)k]; p < f.rp[(size_t)k + 1]; ++p) x[(size_t)k] -= f.rx[(size_t)p] * x[(size_t)f.rc[(size_t)p]];
+  for (int k = 0; k < m; ++k) x[(size_t)k] *= f.dinv[(size_t)k];
+  for (int k = m - 1; k >= 0; --k)
+    for (int p = f.cp[(size_t)k]; p < f.cp[(size_t)k + 1]; ++p) x[(size_t)k] -= f.cx[(size_t)p] * x[(size_t)f.ci[(size_t)p]];
+  double num = 0.0, den = 0.0;
+  for (int k = 0; k < m; ++k) {
+    num = std::max(num, std::fabs(W[(size_t)k] - x[(size_t)k]));
+    den = std::max(den, std::fabs(x[(size_t)k]));
+  }
+  relErr = den > 0.0 ? num / den : num;
+  return true;
+}
+
+// Host replay of the grid form's level schedules (packLevels) in the kernel's
+// order and visibility: per step, phase A tasks (their own runs, lanes, short
+// chains' inverse rows, long chains' rows to R), then phase B (panels of the long
+// chains' inverses applied to R).  A task may read only ranks final before its
+// step; phase B only R written by this step's phase A; every index is checked
+// against its array.  W returns the solution (rank space) of one right-hand
+// side b; relErr as replaySchedules.
+bool replayLevels(const Instance& I, std::string& why, double& relErr, const std::vector<double>& b,
+                  std::vector<double>& W) {
+  const Factor& f = I.fac;
+  const int m = f.m;
+  W = b;
+  std::vector<int> state((size_t)m, 0);  // 0 right-hand side, 1 forward final, 2 backward final
+  auto sweep = [&](const HostBlockSweep& h, bool fwd) -> bool {
+    const char* name = fwd ? "forward" : "backward";
+    auto fault = [&](const std::string& what) {
+      why = std::string(name) + ": " + what;
+      return false;
+    };
+    const int need = fwd ? 1 : 2;
+    const size_t nrun = h.runCol.size() / kRun;
+    const size_t ntask = h.taskInfo.size(), nb = h.bTask.size() / 4;
+    if (h.stepTask.size() != (size_t)h.nstep + 1 || h.stepB.size() != (size_t)h.nstep + 1 ||
+        h.stepDense.size() != (size_t)h.nstep + 1 || h.taskRun.size() != ntask + 1 ||
+        h.laneMeta.size() != ntask * 256 || h.laneDense.size() != ntask * 64 || h.rTotal % 4 != 0)
+      return fault("tables of the wrong length");
+    std::vector<double> R((size_t)h.rTotal, 0.0);
+    std::vector<int> rStep((size_t)h.rTotal, -1);  // step that wrote each R position
+    std::vector<int> done;
+    for (int st = 0; st < h.nstep; ++st) {
+      const int t0 = h.stepTask[(size_t)st], t1 = h.stepTask[(size_t)st + 1];
+      const int j0 = h.stepB[(size_t)st], j1 = h.stepB[(size_t)st + 1];
+      if (t0 < 0 || t1 < t0 || (size_t)t1 > ntask || j0 < 0 || j1 < j0 || (size_t)j1 > nb)
+        return fault("task range out of bounds");
+      const int d0 = h.stepDense[(size_t)st];
+      done.clear();
+      for (int t = t0; t < t1; ++t) {
+        const int r0 = h.taskRun[(size_t)t], r1 = h.taskRun[(size_t)t + 1];
+        if (r0 < 0 || r1 < r0 || (size_t)r1 > nrun) return fault("task runs out of bounds");
+        std::vector<double> part((size_t)(r1 - r0));
+        for (int r = r0; r < r1; ++r) {
+          double acc = 0.0;
+          for (int q = 0; q < kRun; ++q) {
+            const int col = h.runCol[(size_t)r * kRun + q];
+            const double v = h.runVal[(size_t)r * kRun + q];
+            if (col < 0 || col >= m) return fault("run column out of range");
+            if (v == 0.0) continue;  // padding
+            if (state[(size_t)col] != need) return fault("reads rank " + std::to_string(col) + " before it is final");
+            acc += v * W[(size_t)col];
+          }
+          part[(size_t)(r - r0)] = acc;
+        }
+        double a[64], y[64];
+        int kk[64], info[64], doff[64];
+        for (int ln = 0; ln < 64; ++ln) {
+          const size_t mi = ((size_t)t * 64 + ln) * 4;
+          kk[ln] = h.laneMeta[mi];
+          info[ln] = h.laneMeta[mi + 3];
+          doff[ln] = h.laneDense[(size_t)t * 64 + ln];
+          a[ln] = 0.0;
+          const int k = kk[ln];
+          if (k < 0) continue;
+          if (k >= m) return fault("lane rank out of range");
+          if (state[(size_t)k] != need - 1) return fault("finishes rank " + std::to_string(k) + " twice");
+          double x = W[(size_t)k];
+          if (!fwd) x = x * f.dinv[(size_t)k];
+          const int qb = h.laneMeta[mi + 1], qe = h.laneMeta[mi + 2];
+          if (qb < 0 || qe < qb || qe > r1 - r0) return fault("lane's runs outside its task");
+          for (int q = qb; q < qe; ++q) x -= part[(size_t)q];
+          a[ln] = x;
+        }
+        for (int ln = 0; ln < 64; ++ln) {
+          y[ln] = a[ln];
+          if (kk[ln] < 0) continue;
+          const int L = info[ln] & 0x7f, o = (info[ln] >> 7) & 0x7f;
+          if ((info[ln] >> 28) & 1) {
+            if (L != 1) return fault("a long chain's row in a segment");
+            if (doff[ln] < 0 || doff[ln] >= h.rTotal) return fault("R position out of range");
+            if (rStep[(size_t)doff[ln]] >= 0) return fault("R position written twice");
+            continue;
+          }
+          if (L <= 1) continue;
+          if (ln - o < 0 || ln - o + L > 64) return fault("segment outside its task");
+          double acc = 0.0;
+          for (int s2 = 0; s2 < L; ++s2) {
+            const size_t di = (size_t)d0 + (size_t)doff[ln] + (size_t)s2 * L + o;
+            if (di >= h.dense.size()) return fault("segment inverse out of bounds");
+            acc += h.dense[di] * a[ln - o + s2];
+          }
+          y[ln] = acc;
+        }
+        for (int ln = 0; ln < 64; ++ln) {
+          if (kk[ln] < 0) continue;
+          if ((info[ln] >> 28) & 1) {
+            R[(size_t)doff[ln]] = a[ln];
+            rStep[(size_t)doff[ln]] = st;
+          } else {
+            W[(size_t)kk[ln]] = y[ln];
+            done.push_back(kk[ln]);
+          }
+        }
+      }
+      for (int j = j0; j < j1; ++j) {
+        const int xoff = h.bTask[(size_t)j * 4], ld = h.bTask[(size_t)j * 4 + 1], rp = h.bTask[(size_t)j * 4 + 2];
+        const int k0 = h.bTask[(size_t)j * 4 + 3] & 0xffffff, nrows = h.bTask[(size_t)j * 4 + 3] >> 24;
+        if (nrows < 1 || nrows > 8 || ld < 4 || ld % 4 || xoff < 0 || xoff % 4 ||
+            (size_t)xoff + (size_t)nrows * ld > h.xpanel.size() || rp < 0 || rp % 4 || rp + ld > h.rTotal ||
+            k0 + nrows > m)
+          return fault("block product out of bounds");
+        for (int q = 0; q < nrows; ++q) {
+          const int k = k0 + q;
+          if (state[(size_t)k] != need - 1) return fault("block product finishes rank " + std::to_string(k) + " twice");
+          double acc = 0.0;
+          for (int u = 0; u < ld; ++u) {
+            const double xv = h.xpanel[(size_t)xoff + (size_t)q * ld + u];
+            if (xv == 0.0) continue;
+            if (rStep[(size_t)(rp + u)] != st) return fault("block product reads R not written in its step");
+            acc += xv * R[(size_t)(rp + u)];
+          }
+          W[(size_t)k] = acc;
+          done.push_back(k);
+        }
+      }
+      for (int k : done) {
+        if (state[(size_t)k] != need - 1) return fault("rank " + std::to_string(k) + " finished twice in a step");
+        state[(size_t)k] = need;
+      }
+    }
+    for (int k = 0; k < m; ++k)
+      if (state[(size_t)k] != need) return fault("leaves rank " + std::to_string(k) + " unfinished");
+    return true;
+  };
+  if (!sweep(I.bf, true) || !sweep(I.bb, false)) return false;
   std::vector<double> x = b;
   for (int k = 0; k < m; ++k)
     for (int p = f.rp[(size_t)k]; p < f.rp[(size_t)k + 1]; ++p) x[(size_t)k] -= f.rx[(size_t)p] * x[(size_t)f.rc[(size_t)p]];
@@ -2224,9 +2648,12 @@ int pdx_get_schedule_stats(pdx_ctx* c, int i, int64_t* out) {
     o[2] = chains;
     o[3] = chainRows;
     o[4] = (int64_t)h.runCol.size() / kRun;
-    o[5] = (int64_t)h.dense.size();
+    int64_t maxRuns = 0;
+    for (int st = 0; st < h.nstep; ++st)
+      maxRuns = std::max<int64_t>(maxRuns, h.stepRun[(size_t)st + 1] - h.stepRun[(size_t)st]);
+    o[5] = (int64_t)(h.dense.size() + h.xpanel.size());
     o[6] = maxTasks;
-    o[7] = h.maxRuns;
+    o[7] = maxRuns;
   }
   return PDX_OK;
 }

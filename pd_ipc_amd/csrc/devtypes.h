@@ -66,6 +66,20 @@ struct BlockSweepDev {
   const int* laneDense;   // per task, 64 lanes x offset of the segment's inverse block
   const void* dense;      // per segment of length L > 1, its inverse X = B^-1:
                           // forward [s * L + t] = X(t, s), backward [s * L + t] = X(s, t)
+  // Level form (the grid form's sweeps): blocks are whole elimination-tree chains
+  // and every block level is one step, in two phases.  Phase A tasks each stream
+  // their own runs ([taskRun[t], taskRun[t+1])), finish their lanes' rows and
+  // their short chains, and write each row of a long chain (lane info bit 28) to
+  // R instead of W, at component stride rTotal and position laneDense.  Phase B
+  // tasks ([stepB[st], stepB[st+1])) apply a long chain's inverse, 8 rows each:
+  // bTask = {offset of the panel in xpanel, row stride (a multiple of 4), R
+  // position of the panel's column 0, first rank | rows << 24}.
+  const int* taskRun;     // per task, its first run (ntask + 1)
+  const int* stepB;       // nstep + 1
+  const int* bTask;       // 4 per phase B task
+  const void* xpanel;     // the long chains' inverses, panels of 8 rows (Real, zero padded)
+  int rTotal;             // R values per component (each chain's range padded to 4)
+  int pad;
 };
 
 // Exchange area of an instance split over two workgroups (a pair).  Each
@@ -166,10 +180,10 @@ struct InstDev {
   BlockSweepDev bfwdS, bbwdS;
   PairDev pair;
   // grid form: per step of the forward then the backward sweep, the tasks done
-  // and the streaming wavefronts done, cumulative over the sweeps of one launch
-  // (zeroed at its start): [forward done | forward streamed | backward done |
-  // backward streamed], one 256-byte line per counter; and the run partials of
-  // the step in flight (3 Real per run, the most runs of a step)
+  // (both phases) and the phase A tasks done, cumulative over the sweeps of one
+  // launch (zeroed at its start): [forward done | forward phase A | backward done
+  // | backward phase A], one 256-byte line per counter; and R, the long chains'
+  // rows between the phases (3 x rTotal Real, component-major, zero padding)
   int* flow;
   void* flowPart;
   // statistics

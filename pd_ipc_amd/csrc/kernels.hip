@@ -1638,11 +1638,12 @@ __global__ void __launch_bounds__(512) adjointKernel(const InstDev* __restrict__
 // ---------------------------------------------------------------------------
 // Cooperative form for one large instance: the whole grid (one workgroup per CU,
 // all co-resident) advances a single cloth.  Vertex phases are grid-strided and
-// end in a grid barrier; the chain-blocked sweeps are paced by step counters
-// (flowSweep), not by barriers; reductions go through per-workgroup slots and
-// every workgroup reduces them in the same order, so all take the same stop
-// decision.  W lives in HBM.  Same arithmetic per row as the single-workgroup
-// fast sweep: run partials summed in run order.
+// end in a grid barrier; the sweeps take one step per level of whole
+// elimination-tree chains and are paced by step counters (levelSweep), not by
+// barriers; reductions go through per-workgroup slots and every workgroup
+// reduces them in the same order, so all take the same stop decision.  W lives
+// in HBM.  Rows subtract their run partials in run order as the one-workgroup
+// sweep does; long chains apply their inverse as a dense product.
 
 namespace cg = cooperative_groups;
 
@@ -1681,20 +1682,74 @@ __device__ inline void gridMax2(cg::grid_group& grid, double a, double b, double
   rb = blockMax(mb, red + 24);
 }
 
-// W of the grid form as the flow sweeps read it: relaxed agent-scope atomic loads,
-// coherent across the XCDs' L2 caches without a write-back per task.
-template <typename Real>
-struct AgentW {
-  Real* p;
-  __device__ Real operator[](int i) const { return __hip_atomic_load(p + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
-};
+// Stores that other workgroups read: relaxed agent-scope atomics, written through
+// to memory past the XCDs' L2 caches.
 template <typename Real>
 __device__ inline void agentStore(Real* p, Real v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// The grid form's sweeps keep W as 4 Real per rank (x, y, z, unused), so that a
+// column's three values come in one device-scope (sc1) load: 16 bytes FP32, 16 +
+// 8 bytes FP64, through a buffer resource over W.
+__device__ inline __amdgpu_buffer_rsrc_t bufferOver(const void* p, int bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, bytes, 0x00020000);
+}
+template <typename Real>
+__device__ inline void loadW4(__amdgpu_buffer_rsrc_t w, int c, Real& a0, Real& a1, Real& a2);
+template <>
+__device__ inline void loadW4<float>(__amdgpu_buffer_rsrc_t w, int c, float& a0, float& a1, float& a2) {
+  const v4f t = __builtin_bit_cast(v4f, __builtin_amdgcn_raw_buffer_load_b128(w, c * 16, 0, 16));
+  a0 = t.x;
+  a1 = t.y;
+  a2 = t.z;
+}
+template <>
+__device__ inline void loadW4<double>(__amdgpu_buffer_rsrc_t w, int c, double& a0, double& a1, double& a2) {
+  const v2d t = __builtin_bit_cast(v2d, __builtin_amdgcn_raw_buffer_load_b128(w, c * 32, 0, 16));
+  a0 = t.x;
+  a1 = t.y;
+  a2 = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(w, c * 32 + 16, 0, 16));
+}
+
+// runPartial over the grid form's W: the same sums in the same order, every
+// column's values loaded (all 16 columns issued) before the products.
+template <typename Real, bool kIdx16, typename PP>
+__device__ inline void runPartialW4(const RunRaw<Real, kIdx16>& b, __amdgpu_buffer_rsrc_t w, PP P) {
+  Real wv[kRun][3];
+#pragma unroll
+  for (int q = 0; q < kRun; ++q) {
+    int c;
+    if (kIdx16) {
+      const int x = b.c[q >> 3][(q >> 1) & 3];
+      c = (q & 1) ? ((unsigned)x >> 16) : (x & 0xffff);
+    } else {
+      c = b.c[q >> 2][q & 3];
+    }
+    loadW4<Real>(w, c, wv[q][0], wv[q][1], wv[q][2]);
+  }
+  Real p0[2] = {0, 0}, p1[2] = {0, 0}, p2[2] = {0, 0};
+#pragma unroll
+  for (int q = 0; q < kRun; ++q) {
+    const Real l = b.v[q];
+    p0[q & 1] = fmaT(l, wv[q][0], p0[q & 1]);
+    p1[q & 1] = fmaT(l, wv[q][1], p1[q & 1]);
+    p2[q & 1] = fmaT(l, wv[q][2], p2[q & 1]);
+  }
+  P[0] = p0[0] + p0[1];
+  P[1] = p1[0] + p1[1];
+  P[2] = p2[0] + p2[1];
+}
+
+// W of the grid form (4 Real per rank), written by the vertex phases.
+template <typename Real, typename WP>
+__device__ inline void putW4(WP W, int r, double a, double b, double c) {
+  W[4 * r] = (Real)a;
+  W[4 * r + 1] = (Real)b;
+  W[4 * r + 2] = (Real)c;
+}
+
 constexpr long long kFlowSpin = 1LL << 26;  // polls before a flow wait gives up (~2 s at the back-off cap)
-constexpr int kFlowSolo = 2;  // runs per lane up to which a one-task step streams and finishes on one wavefront
 constexpr int kFlowStride = 64;             // ints between two step counters (one 256-byte line each)
 
 // Waits (the whole wavefront) until *counter >= target.  Most of the grid's
@@ -1725,128 +1780,6 @@ __device__ inline bool flowWait(const int* counter, int target, int* giveUp) {
   }
 }
 
-// A chain-blocked sweep of the grid form without grid barriers.  Every step has
-// two phases, each paced by a counter (cumulative over the sweeps of the launch,
-// so step st's targets are rep times its sizes):
-//  * streaming: the step's runs are dealt to the first S = ceil(runs / 64) of the
-//    grid's wavefronts, one run per lane at a time; each turns its runs into
-//    partial sums in `part` (global, step-relative run index) and counts itself
-//    in streamed[st].  A streamer first waits until every task of step st-1 is
-//    done (done[st-1]; step 0 waits on prev / prevTarget when given), which by
-//    induction completes every earlier step.
-//  * finishing: the step's tasks go to the wavefronts round-robin (global index);
-//    a task waits until all S streamers counted (or, for a step without runs,
-//    until step st-1 is done), subtracts its rows' partials in run order,
-//    finishes its chain segments and counts itself in done[st].
-// A step of one task with at most kFlowSolo runs per lane skips the hand-off:
-// its wavefront streams the runs into its own LDS slice (Pw) and finishes.
-// A wavefront goes on to the next step as soon as it has nothing left in this
-// one.  W and the partials cross workgroups as agent-scope atomics (AgentW /
-// agentStore), and a wavefront counts itself only once its stores are
-// performed.  The arithmetic of each row is the one-workgroup sweep's.  Returns
-// false when a wait gave up.
-template <typename Real, bool kFwd, bool kIdx16, typename Store>
-__device__ bool flowSweep(const BlockSweepDev& sw, Real* Wp, Real* part, PDX_LDS Real* Pw, Store store, int* done,
-                          int* streamed, int rep, const int* prev, int prevTarget, int* giveUp) {
-  const int lane = threadIdx.x & (kWave - 1);
-  const int nw = blockDim.x / kWave;
-  const int gw = blockIdx.x * nw + (int)(threadIdx.x / kWave);
-  const int GW = gridDim.x * nw;
-  const AgentW<Real> W{Wp};
-  const AgentW<Real> P{part};
-  const Real* rval = static_cast<const Real*>(sw.runVal);
-  const PDX_GLOBAL Real* gdense = gview<Real>(sw.dense);
-  const PDX_GLOBAL v4i* gmeta = gview<v4i>(sw.laneMeta);
-  const PDX_GLOBAL Real* gldinv = gview<Real>(sw.laneDinv);
-  for (int st = 0; st < sw.nstep; ++st) {
-    const int tb = sw.stepTask[st], te = sw.stepTask[st + 1];
-    const int rbase = sw.stepRun[st], nr = sw.stepRun[st + 1] - rbase, dbase = sw.stepDense[st];
-    const bool solo = te - tb == 1 && nr <= kFlowSolo * kWave;
-    const int S = solo ? 0 : min(GW, (nr + kWave - 1) / kWave);
-    const int* prevOn = st > 0 ? done + (size_t)(st - 1) * kFlowStride : prev;
-    const int prevFor = st > 0 ? rep * (sw.stepTask[st] - sw.stepTask[st - 1]) : prevTarget;
-    int* streamedSt = streamed + (size_t)st * kFlowStride;
-    if (gw < S) {
-      // streaming: runs gw*64 + lane, then S*64 further; the first one loads before the wait
-      RunRaw<Real, kIdx16> raw;
-      int r = gw * kWave + lane;
-      loadRun<Real, kIdx16>(sw.runCol, rval, rbase + (r < nr ? r : 0), raw);
-      if (prevOn && !flowWait(prevOn, prevFor, giveUp)) return false;
-      for (; r < nr; r += S * kWave) {
-        if (r != gw * kWave + lane) loadRun<Real, kIdx16>(sw.runCol, rval, rbase + r, raw);
-        Real q[3];
-        runPartial<Real, kIdx16>(raw, W, q);
-        agentStore(part + 3 * r + 0, q[0]);
-        agentStore(part + 3 * r + 1, q[1]);
-        agentStore(part + 3 * r + 2, q[2]);
-      }
-      storesPerformed();
-      if (lane == 0) __hip_atomic_fetch_add(streamedSt, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    bool waited = false;
-    for (int t = tb + gw; t < te; t += GW) {
-      const v4i mt = gmeta[(size_t)t * kWave + lane];
-      const Real dv = kFwd ? Real(0) : gldinv[(size_t)t * kWave + lane];
-      const int doff = gview<int>(sw.laneDense)[(size_t)t * kWave + lane];
-      const int k = mt.x;
-      const int info = mt.w;
-      const bool act = k >= 0;
-      if (solo) {
-        // the step's only task streams its own runs (<= kFlowSolo per lane) into LDS
-        RunRaw<Real, kIdx16> raw[kFlowSolo];
-#pragma unroll
-        for (int u = 0; u < kFlowSolo; ++u) {
-          const int r = lane + u * kWave;
-          loadRun<Real, kIdx16>(sw.runCol, rval, rbase + (r < nr ? r : 0), raw[u]);
-        }
-        if (prevOn && !flowWait(prevOn, prevFor, giveUp)) return false;
-#pragma unroll
-        for (int u = 0; u < kFlowSolo; ++u)
-          if (lane + u * kWave < nr) runPartial<Real, kIdx16>(raw[u], W, Pw + 3 * (lane + u * kWave));
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-      } else if (!waited) {
-        const bool ok = S > 0 ? flowWait(streamedSt, rep * S, giveUp)
-                              : (prevOn ? flowWait(prevOn, prevFor, giveUp) : true);
-        if (!ok) return false;
-        waited = true;
-      }
-      Real a0 = 0, a1 = 0, a2 = 0;
-      if (act) {
-        a0 = W[3 * k + 0];
-        a1 = W[3 * k + 1];
-        a2 = W[3 * k + 2];
-        if (!kFwd) {
-          a0 = a0 * dv;
-          a1 = a1 * dv;
-          a2 = a2 * dv;
-        }
-        if (solo)
-          subtractPartials<Real>(Pw, mt.y, mt.z, a0, a1, a2);
-        else
-          subtractPartials<Real>(P, mt.y, mt.z, a0, a1, a2);
-      }
-      segmentFinish<Real>(gdense + dbase + doff, info, lane, act, a0, a1, a2);
-      if (act) {
-        agentStore(Wp + 3 * k + 0, a0);
-        agentStore(Wp + 3 * k + 1, a1);
-        agentStore(Wp + 3 * k + 2, a2);
-        store(k, a0, a1, a2);
-      }
-      // the task's stores are performed before it counts as complete
-      storesPerformed();
-      if (lane == 0)
-        __hip_atomic_fetch_add(done + (size_t)st * kFlowStride, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (solo) {
-        // the LDS slice is this wavefront's; the next task may overwrite it
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-      }
-    }
-  }
-  return true;
-}
-
 constexpr int kDR = 8;  // rows per wavefront pass of the dense product (4: FP64 config 2 9.32 ms vs 8.72)
 
 template <typename Real>
@@ -1861,6 +1794,193 @@ struct Vec16<float> {
   typedef float T __attribute__((ext_vector_type(4)));
   static constexpr int n = 4;
 };
+
+constexpr int kLB = 8;  // rows per phase B task of a level sweep (the panels' height)
+
+// A level sweep of the grid form (one step per level of whole elimination-tree
+// chains), without grid barriers.  Every step has two phases, each paced by
+// counters cumulative over the sweeps of the launch (step st's targets are rep
+// times its task counts):
+//  * phase A: task t goes to wavefront t mod GW (t counted from the step's first
+//    task).  It streams its own runs, two per lane at a time, into its LDS slice
+//    (the first pair loads before the wait), after waiting until every task of
+//    step st-1 is done (done[st-1], which by induction completes every earlier
+//    step; step 0 waits on prev / prevTarget when given).  Each lane subtracts its
+//    row's partials in run order; short chains finish with their inverse rows as
+//    in the one-workgroup sweep; a long chain's row goes to R.  Counts itself in
+//    doneA[st] and done[st].
+//  * phase B: task j goes to wavefront (nA + j) mod GW, waits until every phase
+//    A task of the step is done, and applies 8 rows of a long chain's inverse to
+//    R: each lane reads 16 bytes of each row and of R's three components at a
+//    time (R through device-scope loads, as it was written by other
+//    workgroups), a row's sum is lane-strided partial sums then a butterfly, so
+//    the bits do not depend on the grid size.  Counts itself in done[st].
+// W and R cross workgroups as device-scope stores and loads, and a wavefront
+// counts itself only once its stores are performed.  Returns false when a wait
+// gave up.
+template <typename Real, bool kFwd, bool kIdx16, typename Store>
+__device__ bool levelSweep(const BlockSweepDev& sw, int m, Real* Wp, Real* Rp, PDX_LDS Real* Pw, Store store, int* done,
+                           int* doneA, int rep, const int* prev, int prevTarget, int* giveUp) {
+  typedef typename Vec16<Real>::T VT;
+  constexpr int V = Vec16<Real>::n;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int nw = blockDim.x / kWave;
+  const int gw = blockIdx.x * nw + (int)(threadIdx.x / kWave);
+  const int GW = gridDim.x * nw;
+  const __amdgpu_buffer_rsrc_t wsrc = bufferOver(Wp, 4 * m * (int)sizeof(Real));
+  const Real* rval = static_cast<const Real*>(sw.runVal);
+  const PDX_GLOBAL Real* gdense = gview<Real>(sw.dense);
+  const PDX_GLOBAL Real* gx = gview<Real>(sw.xpanel);
+  const PDX_GLOBAL v4i* gmeta = gview<v4i>(sw.laneMeta);
+  const PDX_GLOBAL Real* gldinv = gview<Real>(sw.laneDinv);
+  const int rT = sw.rTotal;
+  // device-scope (sc1) 16-byte loads of R through a buffer resource over its 3 x rTotal values
+  const __amdgpu_buffer_rsrc_t rsrc =
+      __builtin_amdgcn_make_buffer_rsrc(Rp, (short)0, 3 * rT * (int)sizeof(Real), 0x00020000);
+  for (int st = 0; st < sw.nstep; ++st) {
+    const int ta = sw.stepTask[st], tb = sw.stepTask[st + 1];
+    const int ja = sw.stepB[st], jb = sw.stepB[st + 1];
+    const int nA = tb - ta, nB = jb - ja;
+    const int* prevOn = st > 0 ? done + (size_t)(st - 1) * kFlowStride : prev;
+    const int prevFor =
+        st > 0 ? rep * (sw.stepTask[st] - sw.stepTask[st - 1] + sw.stepB[st] - sw.stepB[st - 1]) : prevTarget;
+    const int dbase = sw.stepDense[st];
+    int* doneSt = done + (size_t)st * kFlowStride;
+    int* doneASt = doneA + (size_t)st * kFlowStride;
+    bool waited = false;
+    for (int t = ta + gw; t < tb; t += GW) {
+      const v4i mt = gmeta[(size_t)t * kWave + lane];
+      const Real dv = kFwd ? Real(0) : gldinv[(size_t)t * kWave + lane];
+      const int doff = gview<int>(sw.laneDense)[(size_t)t * kWave + lane];
+      const int r0 = sw.taskRun[t], nr = sw.taskRun[t + 1] - r0;
+      for (int q0 = 0; q0 < nr; q0 += 2 * kWave) {
+        RunRaw<Real, kIdx16> raw[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          const int q = q0 + u * kWave + lane;
+          loadRun<Real, kIdx16>(sw.runCol, rval, r0 + (q < nr ? q : 0), raw[u]);
+        }
+        if (!waited) {
+          if (prevOn && !flowWait(prevOn, prevFor, giveUp)) return false;
+          waited = true;
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          const int q = q0 + u * kWave + lane;
+          if (q < nr) runPartialW4<Real, kIdx16>(raw[u], wsrc, Pw + 3 * q);
+        }
+      }
+      if (!waited) {
+        if (prevOn && !flowWait(prevOn, prevFor, giveUp)) return false;
+        waited = true;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      const int k = mt.x;
+      const int info = mt.w;
+      const bool act = k >= 0;
+      const bool toR = (info >> 28) & 1;
+      Real a0 = 0, a1 = 0, a2 = 0;
+      if (act) {
+        loadW4<Real>(wsrc, k, a0, a1, a2);
+        if (!kFwd) {
+          a0 = a0 * dv;
+          a1 = a1 * dv;
+          a2 = a2 * dv;
+        }
+        subtractPartials<Real>(Pw, mt.y, mt.z, a0, a1, a2);
+      }
+      segmentFinish<Real>(gdense + dbase + (toR ? 0 : doff), info, lane, act, a0, a1, a2);
+      if (act) {
+        if (toR) {
+          agentStore(Rp + doff, a0);
+          agentStore(Rp + rT + doff, a1);
+          agentStore(Rp + 2 * rT + doff, a2);
+        } else {
+          agentStore(Wp + 4 * k + 0, a0);
+          agentStore(Wp + 4 * k + 1, a1);
+          agentStore(Wp + 4 * k + 2, a2);
+          store(k, a0, a1, a2);
+        }
+      }
+      // the task's stores are performed before it counts as complete
+      storesPerformed();
+      if (lane == 0) {
+        __hip_atomic_fetch_add(doneASt, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_add(doneSt, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      // the LDS slice is this wavefront's; the next task may overwrite it
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+    }
+    int j = gw - nA % GW;
+    if (j < 0) j += GW;
+    bool waitedB = false;
+    for (; j < nB; j += GW) {
+      const int* bt = sw.bTask + 4 * (size_t)(ja + j);
+      const int xoff = __builtin_amdgcn_readfirstlane(bt[0]);
+      const int ld = __builtin_amdgcn_readfirstlane(bt[1]);
+      const int rp = __builtin_amdgcn_readfirstlane(bt[2]);
+      const int kr = __builtin_amdgcn_readfirstlane(bt[3]);
+      const int k0 = kr & 0xffffff, nrows = kr >> 24;
+      const PDX_GLOBAL VT* row[kLB];
+#pragma unroll
+      for (int q = 0; q < kLB; ++q)  // rows past the panel repeat its last row (their sums are dropped)
+        row[q] = (const PDX_GLOBAL VT*)(gx + xoff + (size_t)min(q, nrows - 1) * ld);
+      if (!waitedB) {
+        if (!flowWait(doneASt, rep * nA, giveUp)) return false;
+        waitedB = true;
+      }
+      Real acc[kLB][3];
+#pragma unroll
+      for (int q = 0; q < kLB; ++q) acc[q][0] = acc[q][1] = acc[q][2] = Real(0);
+      const int nv = ld / V;
+      for (int jv = lane; jv < nv; jv += kWave) {
+        VT lv[kLB];
+#pragma unroll
+        for (int q = 0; q < kLB; ++q) lv[q] = row[q][jv];
+        const int off = (rp + jv * V) * (int)sizeof(Real);
+        const VT w0 = __builtin_bit_cast(VT, __builtin_amdgcn_raw_buffer_load_b128(rsrc, off, 0, 16));
+        const VT w1 = __builtin_bit_cast(VT, __builtin_amdgcn_raw_buffer_load_b128(rsrc, off + rT * (int)sizeof(Real), 0, 16));
+        const VT w2 =
+            __builtin_bit_cast(VT, __builtin_amdgcn_raw_buffer_load_b128(rsrc, off + 2 * rT * (int)sizeof(Real), 0, 16));
+#pragma unroll
+        for (int u = 0; u < V; ++u)
+#pragma unroll
+          for (int q = 0; q < kLB; ++q) {
+            const Real l = lv[q][u];
+            acc[q][0] = fmaT(l, w0[u], acc[q][0]);
+            acc[q][1] = fmaT(l, w1[u], acc[q][1]);
+            acc[q][2] = fmaT(l, w2[u], acc[q][2]);
+          }
+      }
+#pragma unroll
+      for (int q = 0; q < kLB; ++q)
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+#pragma unroll
+          for (int o = kWave / 2; o >= 1; o >>= 1) acc[q][d] += __shfl_xor(acc[q][d], o);
+      Real y0 = 0, y1 = 0, y2 = 0;
+#pragma unroll
+      for (int q = 0; q < kLB; ++q)
+        if (lane == q) {
+          y0 = acc[q][0];
+          y1 = acc[q][1];
+          y2 = acc[q][2];
+        }
+      if (lane < nrows) {
+        const int k = k0 + lane;
+        agentStore(Wp + 4 * k + 0, y0);
+        agentStore(Wp + 4 * k + 1, y1);
+        agentStore(Wp + 4 * k + 2, y2);
+        store(k, y0, y1, y2);
+      }
+      storesPerformed();
+      if (lane == 0) __hip_atomic_fetch_add(doneSt, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+  return true;
+}
 
 // Dense solve: out_r = sum_j Linv[r][j] W_j for the three right-hand sides, for
 // every rank r, rows spread over the grid's wavefronts (global wavefront gw of GW),
@@ -1951,14 +2071,21 @@ __global__ void __launch_bounds__(512) gridSubstepKernel(const InstDev* __restri
   const int gt = blockIdx.x * blockDim.x + threadIdx.x;
   const int gs = gridDim.x * blockDim.x;
   const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  // W: 3 Real per rank for the dense product, 4 for the level sweeps
+  auto wput = [=](int r, double a, double b, double c) {
+    if (kDense)
+      putW<Real>(W, r, a, b, c);
+    else
+      putW4<Real>(W, r, a, b, c);
+  };
   int buf = 0;
-  // the flow sweeps' step counters start at zero in every launch (ordered before
+  // the level sweeps' step counters start at zero in every launch (ordered before
   // the first sweep by the grid barrier after the first right-hand side)
   const int nF = I.bfwd.nstep, nB = I.bbwd.nstep;
   if (!kDense && blockIdx.x == 0)
     for (int i = threadIdx.x; i < 2 * (nF + nB); i += blockDim.x) I.flow[(size_t)i * kFlowStride] = 0;
   int rep = 0;  // sweeps of this launch
-  const int lastF = nF > 0 ? I.bfwd.stepTask[nF] - I.bfwd.stepTask[nF - 1] : 0;
+  const int lastF = nF > 0 ? I.bfwd.stepTask[nF] - I.bfwd.stepTask[nF - 1] + I.bfwd.stepB[nF] - I.bfwd.stepB[nF - 1] : 0;
   for (int sub = 0; sub < nsub; ++sub) {
     // predict (Integrator.cpp:447-463), displacement scale, first right-hand side
     double lscale = 0.0;
@@ -1976,14 +2103,14 @@ __global__ void __launch_bounds__(512) gridSubstepKernel(const InstDev* __restri
       const int r = I.vrank[v];
       if (kExact) {
         const Acc3 b = rhsExact(I, v);
-        putW<Real>(W, r, b.x, b.y, b.z);
+        wput(r, b.x, b.y, b.z);
       } else if (I.pinned[v]) {
-        putW<Real>(W, r, I.pinpos[3 * v] - I.x[3 * v], I.pinpos[3 * v + 1] - I.x[3 * v + 1],
+        wput(r, I.pinpos[3 * v] - I.x[3 * v], I.pinpos[3 * v + 1] - I.x[3 * v + 1],
              I.pinpos[3 * v + 2] - I.x[3 * v + 2]);
       } else {
         Acc3 rh;
         (void)residualAndRhs(I, v, &rh);
-        putW<Real>(W, r, rh.x, rh.y, rh.z);
+        wput(r, rh.x, rh.y, rh.z);
       }
     }
     double scale = gridMax(grid, lscale, red, gslots, buf);
@@ -1995,12 +2122,24 @@ __global__ void __launch_bounds__(512) gridSubstepKernel(const InstDev* __restri
     auto none = [](int, Real, Real, Real) {};
     double* x = I.x;
     const int* rankv = I.rankv;
+    const uint8_t* pinnedP = I.pinned;
+    const double* pinposP = I.pinpos;
+    // the backward sweep's rows finish the iterate: the solution itself (exact
+    // right-hand side), or x + the FP32 update (pinned rows take q)
     auto put = [=](int k, Real a0, Real a1, Real a2) {
+      const int v = rankv[k];
       if (kExact) {
-        const int v = rankv[k];
         x[3 * v] = (double)a0;
         x[3 * v + 1] = (double)a1;
         x[3 * v + 2] = (double)a2;
+      } else if (pinnedP[v]) {
+        x[3 * v] = pinposP[3 * v];
+        x[3 * v + 1] = pinposP[3 * v + 1];
+        x[3 * v + 2] = pinposP[3 * v + 2];
+      } else {
+        x[3 * v] += (double)a0;
+        x[3 * v + 1] += (double)a1;
+        x[3 * v + 2] += (double)a2;
       }
     };
     for (int k = 0; k < I.maxIter; ++k) {
@@ -2026,32 +2165,17 @@ __global__ void __launch_bounds__(512) gridSubstepKernel(const InstDev* __restri
           gridDense<Real>(grid, I, W, update);
         }
       } else {
-        // sweeps paced by step counters, one grid barrier once the backward sweep is done
+        // level sweeps paced by step counters; the backward rows update x as they
+        // finish, and one grid barrier follows
         ++rep;
         int* doneF = I.flow;
         int* doneB = I.flow + (size_t)2 * nF * kFlowStride;
-        Real* part = static_cast<Real*>(I.flowPart);
-        if (flowSweep<Real, true, kIdx16>(I.bfwd, (Real*)W, part, Pw, none, doneF, doneF + (size_t)nF * kFlowStride,
-                                          rep, nullptr, 0, I.pair.giveUp))
-          flowSweep<Real, false, kIdx16>(I.bbwd, (Real*)W, part, Pw, put, doneB, doneB + (size_t)nB * kFlowStride, rep,
-                                          nF > 0 ? doneF + (size_t)(nF - 1) * kFlowStride : nullptr, rep * lastF,
-                                          I.pair.giveUp);
-        grid.sync();
-      }
-      if (!kExact && !kDense) {
-        const PDX_GLOBAL Real* U = W;
-        for (int v = gt; v < n; v += gs) {
-          const int r = I.vrank[v];
-          if (I.pinned[v]) {
-            I.x[3 * v] = I.pinpos[3 * v];
-            I.x[3 * v + 1] = I.pinpos[3 * v + 1];
-            I.x[3 * v + 2] = I.pinpos[3 * v + 2];
-          } else {
-            I.x[3 * v] += (double)U[3 * r];
-            I.x[3 * v + 1] += (double)U[3 * r + 1];
-            I.x[3 * v + 2] += (double)U[3 * r + 2];
-          }
-        }
+        Real* R = static_cast<Real*>(I.flowPart);
+        if (levelSweep<Real, true, kIdx16>(I.bfwd, I.m, (Real*)W, R, Pw, none, doneF, doneF + (size_t)nF * kFlowStride,
+                                           rep, nullptr, 0, I.pair.giveUp))
+          levelSweep<Real, false, kIdx16>(I.bbwd, I.m, (Real*)W, R, Pw, put, doneB, doneB + (size_t)nB * kFlowStride, rep,
+                                           nF > 0 ? doneF + (size_t)(nF - 1) * kFlowStride : nullptr, rep * lastF,
+                                           I.pair.giveUp);
         grid.sync();
       }
       const bool more = (k + 1 < I.maxIter);
@@ -2070,16 +2194,16 @@ __global__ void __launch_bounds__(512) gridSubstepKernel(const InstDev* __restri
           if (I.wantResidual && !pinnedV && I.mass[v] > 0.0) lres = fmax(lres, residualAndRhs(I, v, nullptr));
           if (more) {
             const Acc3 b = rhsExact(I, v);
-            putW<Real>(W, r, b.x, b.y, b.z);
+            wput(r, b.x, b.y, b.z);
           }
         } else {
           if (pinnedV) {
-            if (more) putW<Real>(W, r, I.pinpos[3 * v] - x0, I.pinpos[3 * v + 1] - x1, I.pinpos[3 * v + 2] - x2);
+            if (more) wput(r, I.pinpos[3 * v] - x0, I.pinpos[3 * v + 1] - x1, I.pinpos[3 * v + 2] - x2);
           } else if (more || I.wantResidual) {
             Acc3 rh;
             const double rv = residualAndRhs(I, v, more ? &rh : nullptr);
             if (I.wantResidual && I.mass[v] > 0.0) lres = fmax(lres, rv);
-            if (more) putW<Real>(W, r, rh.x, rh.y, rh.z);
+            if (more) wput(r, rh.x, rh.y, rh.z);
           }
         }
       }
@@ -2137,7 +2261,7 @@ __global__ void __launch_bounds__(512) gridSolveKernel(const InstDev* __restrict
   PDX_GLOBAL Real* W = (PDX_GLOBAL Real*)I.Wg;
   const int gt = blockIdx.x * blockDim.x + threadIdx.x;
   const int gs = gridDim.x * blockDim.x;
-  for (int v = gt; v < I.n; v += gs) putW<Real>(W, I.vrank[v], b[3 * v], b[3 * v + 1], b[3 * v + 2]);
+  for (int v = gt; v < I.n; v += gs) putW4<Real>(W, I.vrank[v], b[3 * v], b[3 * v + 1], b[3 * v + 2]);
   const int nF = I.bfwd.nstep, nB = I.bbwd.nstep;
   if (blockIdx.x == 0)
     for (int i = threadIdx.x; i < 2 * (nF + nB); i += blockDim.x) I.flow[(size_t)i * kFlowStride] = 0;
@@ -2150,14 +2274,14 @@ __global__ void __launch_bounds__(512) gridSolveKernel(const InstDev* __restrict
     xout[3 * v + 1] = (double)a1;
     xout[3 * v + 2] = (double)a2;
   };
-  const int lastF = nF > 0 ? I.bfwd.stepTask[nF] - I.bfwd.stepTask[nF - 1] : 0;
+  const int lastF = nF > 0 ? I.bfwd.stepTask[nF] - I.bfwd.stepTask[nF - 1] + I.bfwd.stepB[nF] - I.bfwd.stepB[nF - 1] : 0;
   int* doneF = I.flow;
   int* doneB = I.flow + (size_t)2 * nF * kFlowStride;
-  Real* part = static_cast<Real*>(I.flowPart);
-  if (flowSweep<Real, true, kIdx16>(I.bfwd, (Real*)W, part, Pw, none, doneF, doneF + (size_t)nF * kFlowStride, 1,
-                                    nullptr, 0, I.pair.giveUp))
-    flowSweep<Real, false, kIdx16>(I.bbwd, (Real*)W, part, Pw, put, doneB, doneB + (size_t)nB * kFlowStride, 1,
-                                    nF > 0 ? doneF + (size_t)(nF - 1) * kFlowStride : nullptr, lastF, I.pair.giveUp);
+  Real* R = static_cast<Real*>(I.flowPart);
+  if (levelSweep<Real, true, kIdx16>(I.bfwd, I.m, (Real*)W, R, Pw, none, doneF, doneF + (size_t)nF * kFlowStride, 1,
+                                     nullptr, 0, I.pair.giveUp))
+    levelSweep<Real, false, kIdx16>(I.bbwd, I.m, (Real*)W, R, Pw, put, doneB, doneB + (size_t)nB * kFlowStride, 1,
+                                     nF > 0 ? doneF + (size_t)(nF - 1) * kFlowStride : nullptr, lastF, I.pair.giveUp);
 }
 
 // Dense solve mode, x = Linv b: W = P b (rank space) ...
