@@ -344,8 +344,10 @@ def dropin_workload(args):
     context, against the FP64 oracle (the reference CPU path restated) on the
     same inputs: 40x40 distance only (10 iters x 600 substeps) and 100x100 with
     shear and bend, sigma 1e-3 m position noise (40 iters x 300 substeps).  Forms:
-    dense, fast (AMD order), fast_nd (nested-dissection order), exact; the oracle
-    runs in the reference's AMD order."""
+    dense (the factor's inverse as one product), sparse (the factor's sweeps; the
+    autotuner picks fill order, launch form and shape) and exact (bitwise, the
+    reference's order); the oracle runs in the reference's AMD order with 4
+    threads (the reference's default) and is timed once per length."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_ffi import Oracle
     out = {}
@@ -353,45 +355,50 @@ def dropin_workload(args):
         if args.dropin_substeps:
             nsub = min(nsub, args.dropin_substeps)
         res = {"substeps": nsub, "pd_iterations": cfg.max_iterations}
-        amd_perm = None
-        for solve in ("dense", "fast", "fast_nd", "exact"):
-            e = Engine("fp64", "fast" if solve == "fast_nd" else solve)
+        amd = Engine("fp64", "exact")
+        amd.add(c, cfg)
+        amd_perm = amd.factor_arrays(0)["perm3"]
+        amd.close()
+        oracle_runs = {}
+
+        def oracle(ns):
+            if ns not in oracle_runs:
+                o = Oracle(c, cfg, amd_perm, threads=4)
+                o.factor()
+                t1 = time.perf_counter()
+                o.step(ns)
+                oracle_runs[ns] = (o.positions(), time.perf_counter() - t1)
+            return oracle_runs[ns]
+
+        for solve in ("dense", "sparse", "exact"):
+            e = Engine("fp64", "fast" if solve == "sparse" else solve)
             e.add(c, cfg)
-            if solve == "fast_nd":
-                e.set_ordering("nd")
             t0 = time.perf_counter()
             e.factor()
             tf = time.perf_counter() - t0
             tune = e.autotune(2) if (solve != "exact" and not args.no_tune) else None
-            # the oracle runs in the reference's AMD order for every form
-            perm = e.factor_arrays(0)["perm3"] if solve != "fast_nd" else amd_perm
-            if amd_perm is None:
-                amd_perm = perm
             ns = nsub if solve != "exact" else min(nsub, 600 if c.n < 5000 else 2)
             t0 = time.perf_counter()
             e.step(ns)
             dt = time.perf_counter() - t0
             x, _ = e.state(0)
             st = e.stats(0)
-            o = Oracle(c, cfg, perm, threads=4)
-            o.factor()
-            t1 = time.perf_counter()
-            o.step(ns)
-            tc = time.perf_counter() - t1
-            xo = o.positions()
+            xo, tc = oracle(ns)
             rel = float(np.max(np.abs(x - xo)) / np.max(np.abs(xo)))
+            form, blocks, order = e.launch()
             sched = None
             if tune is not None:
                 sched = ({"grid_workgroups": tune[0]} if solve == "dense"
-                         else {"threads": tune[0], "chain_cap": tune[1]})
+                         else {"threads": tune[0], "chain_cap": tune[1], "launch": form,
+                               "grid_workgroups": blocks, "ordering": order})
             res[solve] = {"gpu_ms_per_substep": round(1e3 * dt / ns, 4), "substeps_timed": ns,
                           "max_rel_pos_diff_vs_oracle": rel, "bitwise_equal": bool(np.array_equal(x, xo)),
                           "factorize_count": st["factorize_count"], "factor_and_upload_s": round(tf, 3),
                           "schedule": sched,
                           "algorithmic_GB_s": round(e.bytes_per_substep() / (dt / ns) / 1e9, 1),
                           "cpu_oracle_ms_per_substep_4threads": round(1e3 * tc / ns, 4)}
-            log(f"{name} {solve}: GPU {1e3 * dt / ns:.3f} ms/substep, CPU oracle {1e3 * tc / ns:.3f} ms/substep, "
-                f"rel diff {rel:.3e}")
+            log(f"{name} {solve}: GPU {1e3 * dt / ns:.3f} ms/substep ({sched}), CPU oracle "
+                f"{1e3 * tc / ns:.3f} ms/substep, rel diff {rel:.3e}")
             e.close()
         out[name] = res
     return out
@@ -399,40 +406,72 @@ def dropin_workload(args):
 
 def large_workload(args):
     """512x512 cloth (262,144 vertices, distance + shear + bend, four corners
-    pinned), 40 PD iterations: FP64 in the one-workgroup form (W in HBM, the
-    reference's AMD order) and FP32-solve / FP64-residual in the cooperative grid
-    form (every CU on the one cloth, --ordering: nested dissection by default),
-    max position error of the FP32 form against FP64 after the run."""
+    pinned), 40 PD iterations, in FP64 and in FP32-solve / FP64-residual form:
+    the autotuner picks fill order, launch form (one workgroup, or the grid of
+    16-256 workgroups) and shape for each.  Max position error of the FP32 form
+    against FP64 after the run; the FP64 GPU path against the FP64 oracle (the
+    reference CPU path restated, AMD order, 4 threads) after its first substep,
+    which also times the oracle (the CPU baseline of this workload)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_ffi import Oracle
     c, cfg, nsub = scenes.config3()
     nsub = min(nsub, args.large_substeps)
     res = {"vertices": c.n, "substeps": nsub, "pd_iterations": cfg.max_iterations}
+    amd = Engine("fp64", "exact")
+    amd.add(c, cfg)
+    amd_perm = amd.factor_arrays(0)["perm3"]
+    amd.close()
+    cores = host_cores()
+    o = Oracle(c, cfg, amd_perm, threads=4)
+    t0 = time.perf_counter()
+    o.factor()
+    tfo = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    o.step(1)
+    tco = time.perf_counter() - t0
+    xo1 = o.positions()
+    del o
+    log(f"large: CPU oracle (4 threads) {1e3 * tco:.0f} ms/substep, factor {tfo:.1f} s")
     xs = {}
-    for prec, mode, g, order in (("fp64", "batch", 0, "amd"), ("mixed32", "grid", 32, args.ordering)):
+    for prec in ("fp64", "mixed32"):
         e = Engine(prec, "fast")
         e.add(c, cfg)
-        if mode == "grid":
-            e.set_launch("grid", g)
-        if order != "amd":
-            e.set_ordering(order)
         t0 = time.perf_counter()
         e.factor()
         tf = time.perf_counter() - t0
+        tune = e.autotune(1) if not args.no_tune else None
+        form, blocks, order = e.launch()
+        x0, v0 = e.state(0)
+        e.step(1)
+        x1 = e.state(0)[0]
+        rel1 = float(np.max(np.abs(x1 - xo1)) / np.max(np.abs(xo1)))
+        e.set_state(0, x0, v0)
         t0 = time.perf_counter()
         e.step(nsub)
         dt = time.perf_counter() - t0
         xs[prec] = e.state(0)[0]
         st = e.stats(0)
         bps = e.bytes_per_substep()
-        res[prec] = {"form": mode if mode == "batch" else f"grid ({g} workgroups)", "ordering": order,
+        achieved = bps / (dt / nsub) / 1e9
+        res[prec] = {"launch": form, "grid_workgroups": blocks, "ordering": order,
+                     "schedule": None if tune is None else {"threads": tune[0], "chain_cap": tune[1]},
                      "ms_per_substep": round(1e3 * dt / nsub, 2),
                      "cloth_substeps_per_s": round(nsub / dt, 4),
                      "algorithmic_GB_per_substep": round(bps / 1e9, 3),
-                     "achieved_GB_s": round(bps / (dt / nsub) / 1e9, 2),
+                     "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                                  "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5)},
                      "factor_nnz": st["factor_nnz"], "steps_per_sweep": [st["fwd_steps"], st["bwd_steps"]],
-                     "factor_and_upload_s": round(tf, 2)}
-        log(f"large {prec} {mode}: {1e3 * dt / nsub:.1f} ms/substep")
+                     "factor_and_upload_s": round(tf, 2),
+                     "max_rel_pos_diff_vs_oracle_after_1_substep": rel1}
+        log(f"large {prec} {form} ({blocks} workgroups, {order}): {1e3 * dt / nsub:.1f} ms/substep, "
+            f"rel diff vs oracle after 1 substep {rel1:.3e}")
         e.close()
     res["max_pos_error_m_mixed32_vs_fp64"] = float(np.max(np.abs(xs["mixed32"] - xs["fp64"])))
+    res["cpu_baseline"] = {"value": round(1.0 / tco, 4), "unit": "cloth-substeps/s", "cores": 4, "kind": "port",
+                           "ms_per_substep": round(1e3 * tco, 1), "factor_s": round(tfo, 2),
+                           "host_cores": cores,
+                           "sample": "FP64 oracle (reference CPU path restated, OpenMP 4 threads: the reference's "
+                                     "default), 1 substep of the 512x512 cloth"}
     return res
 
 

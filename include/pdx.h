@@ -244,6 +244,10 @@ int pdx_set_split(pdx_ctx* ctx, long long min_factor_entries);
 int pdx_split_count(const pdx_ctx* ctx);
 /* The current split threshold (factor entries; 0 = no split; -1 for a NULL context). */
 long long pdx_get_split(const pdx_ctx* ctx);
+/* The context's launch form (PDX_LAUNCH_*), grid workgroups (0 = all that fit)
+ * and fill order (PDX_ORDER_*), as set or as the autotuner chose them.  Any
+ * pointer may be NULL.  No counterpart in the reference. */
+int pdx_get_launch(const pdx_ctx* ctx, int* mode, int* grid_blocks, int* ordering);
 /* Host check of an instance's fast-mode sweep schedules (no device use): replays
  * them in FP64 in the kernel's phase order - for a split instance, each half seeing
  * only what its workgroup holds - and fails with PDX_EUNSUPPORTED (message in
@@ -292,14 +296,19 @@ int pdx_get_schedule_stats(pdx_ctx* ctx, int instance, int64_t* out16);
 int pdx_set_launch(pdx_ctx* ctx, int mode, int grid_blocks);
 
 /* Autotuner: times `probe_substeps` substeps of the whole context under each
- * launch shape (threads per workgroup 256 / 512 x chain cap 8 / 16 / 24 / 32 in
- * fast mode; in the dense solve mode, 16, 32, 64, ... workgroups, reported in
- * best_block with best_cap 0) and keeps the fastest; then, for a FP32-solve fast
- * batch, the split threshold (pdx_set_split: none, or the factor size of the 1st,
- * 2nd, 4th, 8th, ..., 64th largest instance that can split; pdx_get_split reads
- * the choice).  State and statistics are
- * restored, so the probes leave no trace.  No counterpart in the reference (its
- * CPU path has no launch schedule). */
+ * choice and keeps the fastest.  A context of one cloth in the fast mode first
+ * probes the fill order (PDX_ORDER_AMD / PDX_ORDER_ND; a change re-analyses and
+ * refactors, which factorize_count and analyze_count count) and the launch form
+ * (PDX_LAUNCH_BATCH, or PDX_LAUNCH_GRID with 16, 32, ..., 256 workgroups);
+ * pdx_get_launch reads the choice.  Then every context probes the launch shape
+ * (threads per workgroup 256 / 512 x chain cap 8 / 16 / 24 / 32 in fast mode; in
+ * the dense solve mode, 16, 32, 64, ... workgroups, reported in best_block with
+ * best_cap 0); then, for a FP32-solve fast batch, the split threshold
+ * (pdx_set_split: none, or the factor size of the 1st, 2nd, 4th, 8th, ..., 64th
+ * largest instance that can split; pdx_get_split reads the choice).  State and
+ * the step statistics are restored, so the probes leave no trace in the
+ * simulation.  No counterpart in the reference (its CPU path has no launch
+ * schedule). */
 int pdx_autotune(pdx_ctx* ctx, int probe_substeps, int* best_block, int* best_cap, double* best_ms);
 
 /* --- timing ---------------------------------------------------------------- */

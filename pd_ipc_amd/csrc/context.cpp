@@ -2816,9 +2816,51 @@ int pdx_autotune(pdx_ctx* c, int probe_substeps, int* best_block, int* best_cap,
     if (best_ms) *best_ms = bestMs;
     return PDX_OK;
   }
+  const bool fast = c->mode == PDX_SOLVE_FAST;
+  // A single cloth in the fast mode: the fill order (the current one first, then
+  // the other) and the launch form (one workgroup, or the grid with 16 .. 256
+  // workgroups) are probed at the current shape; the shape search below runs in
+  // the form chosen.  A change of order re-analyses and refactors the cloth
+  // (factorize_count and analyze_count count it).
+  if (fast && c->inst.size() == 1 && !c->rollMode && c->inst[0].cfg.chebRho <= 0.0) {
+    double formMs = 1e300;
+    int bo = c->ordering, bm = c->launchMode, bgb = c->gridBlocks;
+    const int orders[2] = {c->ordering, c->ordering == PDX_ORDER_AMD ? PDX_ORDER_ND : PDX_ORDER_AMD};
+    for (int ord : orders) {
+      // the state before the probes: a new analysis orders by these positions
+      if (c->uploaded && (rc = restore())) return rc;
+      if ((rc = pdx_set_ordering(c, ord))) return rc;
+      for (int form : {PDX_LAUNCH_BATCH, PDX_LAUNCH_GRID}) {
+        for (int g = 16; g <= 256; g *= 2) {
+          const int gb = form == PDX_LAUNCH_GRID ? g : 0;
+          if (c->uploaded && (rc = restore())) return rc;
+          if ((rc = pdx_set_launch(c, form, gb))) return rc;
+          if ((rc = ensureReady(c))) {
+            if (rc == PDX_EUNSUPPORTED) break;  // this form does not fit the cloth
+            return rc;
+          }
+          if (form == PDX_LAUNCH_GRID && gb > gridBlocksMax(c->block, c->lds, c->prec == PDX_PREC_MIXED32, c->idx16))
+            break;
+          if ((rc = restore())) return rc;
+          if ((rc = launch(c, probe_substeps))) return rc;
+          if ((rc = pdx_sync(c))) return rc;
+          if (c->lastKernelMs < formMs) {
+            formMs = c->lastKernelMs;
+            bo = ord;
+            bm = form;
+            bgb = gb;
+          }
+          if (form == PDX_LAUNCH_BATCH) break;
+        }
+      }
+    }
+    if ((rc = restore())) return rc;
+    if ((rc = pdx_set_ordering(c, bo))) return rc;
+    if ((rc = pdx_set_launch(c, bm, bgb))) return rc;
+    if ((rc = ensureReady(c))) return rc;
+  }
   const int blocks[2] = {256, 512};
   const int caps[4] = {8, 16, 24, 32};
-  const bool fast = c->mode == PDX_SOLVE_FAST;
   double bestMs = 1e300;
   int bb = c->block, bc = c->chainCap;
   for (int ci = 0; ci < (fast ? 4 : 1); ++ci) {
@@ -2885,6 +2927,14 @@ int pdx_autotune(pdx_ctx* c, int probe_substeps, int* best_block, int* best_cap,
 }
 
 long long pdx_get_split(const pdx_ctx* c) { return c ? c->splitMin : -1; }
+
+int pdx_get_launch(const pdx_ctx* c, int* mode, int* grid_blocks, int* ordering) {
+  if (!c) return PDX_EINVAL;
+  if (mode) *mode = c->launchMode;
+  if (grid_blocks) *grid_blocks = c->gridBlocks;
+  if (ordering) *ordering = c->ordering;
+  return PDX_OK;
+}
 
 double pdx_last_kernel_ms(const pdx_ctx* c) { return c ? c->lastKernelMs : 0.0; }
 

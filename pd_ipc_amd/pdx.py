@@ -132,6 +132,7 @@ def load_library(path=LIB_PATH):
         "pdx_set_split": (I, [P, C.c_longlong]),
         "pdx_split_count": (I, [P]),
         "pdx_get_split": (C.c_longlong, [P]),
+        "pdx_get_launch": (I, [P, C.POINTER(I), C.POINTER(I), C.POINTER(I)]),
         "pdx_verify_schedule": (I, [P, I, C.POINTER(D)]),
         "pdx_bytes_per_iteration": (D, [P]),
         "pdx_bytes_per_substep": (D, [P]),
@@ -518,6 +519,12 @@ class Engine:
         one instance, fast solve only)."""
         m = {"batch": 0, "grid": 1}[mode]
         self._check(self.lib.pdx_set_launch(self.h, m, grid_blocks), "set_launch")
+
+    def launch(self):
+        """(launch form 'batch' / 'grid', grid workgroups (0 = all), fill order 'amd' / 'nd')."""
+        m, g, o = C.c_int(), C.c_int(), C.c_int()
+        self._check(self.lib.pdx_get_launch(self.h, C.byref(m), C.byref(g), C.byref(o)), "get_launch")
+        return ("batch", "grid")[m.value], g.value, ("amd", "nd")[o.value]
 
     def schedule_stats(self, i=0):
         o = np.zeros(16, np.int64)
