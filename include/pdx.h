@@ -61,9 +61,10 @@ enum {
 
 enum {
   PDX_LAUNCH_BATCH = 0, /* one workgroup per instance, persistent over the substeps */
-  PDX_LAUNCH_GRID = 1   /* every CU on one instance: cooperative launch; sweep steps paced by
-                           per-step completion counters, grid barriers only around the vertex
-                           phases (fast solve mode, exactly one instance) */
+  PDX_LAUNCH_GRID = 1   /* the grid's workgroups on one instance: cooperative launch; one sweep
+                           step per level of whole elimination-tree chains, paced by per-step
+                           completion counters, grid barriers only around the vertex phases
+                           (fast solve mode, exactly one instance, fewer than 2^24 vertices) */
 };
 
 typedef struct pdx_ctx pdx_ctx;
