@@ -4,7 +4,8 @@
   path): the FP32-solve form in the nested-dissection order against the FP64
   batch in the reference's AMD order, every instance within 1e-5 m, and four
   sampled instances against the FP64 CPU oracle;
-* the 512 x 512 cloth: the FP32-solve grid form against the FP64 batch form;
+* the 512 x 512 cloth: the FP64 grid form against the CPU oracle after one
+  substep, the FP32-solve grid form against the FP64 one after three;
 * the differentiable batch, two 128 x 128 instances, 120 substeps, 20 iterations,
   checkpoints every 10: GPU adjoint gradients (FP64 and FP32 solves) against
   central differences of the GPU FP64 forward, and one instance against the FP64
@@ -104,30 +105,57 @@ def test_mix_512_full_size_through_graph():
 
 @pytest.mark.timeout(900)
 def test_large_cloth_512_grid_form():
+    """512 x 512 with shear and bending, four corners pinned, 40 iterations: the
+    grid form in FP64 (nested-dissection order) against the FP64 CPU oracle (the
+    reference path restated, AMD order) after one substep, within 1e-10
+    relative; the FP32-solve grid form against the FP64 grid form after 3
+    substeps, within 1e-5 m; one factorisation each."""
     c, cfg, _ = scenes.config3()
     nsub = 3
-    _say("512x512: factor mixed32 ND grid form")
-    e32 = Engine("mixed32", "fast")
-    e32.add(c, cfg)
-    e32.set_ordering("nd")
-    e32.set_launch("grid")
-    e32.factor()
-    _say("512x512: factor FP64 batch (one workgroup, AMD)")
-    e64 = Engine("fp64", "fast")
-    e64.add(c, cfg)
-    e64.factor()
-    _say("512x512: mixed32 grid form,", nsub, "substeps")
+    _say("512x512: AMD order for the oracle")
+    amd = Engine("fp64", "exact")
+    amd.add(c, cfg)
+    perm = amd.factor_arrays(0)["perm3"]
+    amd.close()
+    orc = _Bg(lambda: _oracle_positions(c, cfg, perm, 1))
+    engines = {}
+    for prec, g in (("fp64", 128), ("mixed32", 64)):
+        _say("512x512: factor", prec, "ND grid form,", g, "workgroups")
+        e = Engine(prec, "fast")
+        e.add(c, cfg)
+        e.set_ordering("nd")
+        e.set_launch("grid", g)
+        e.set_schedule(256, 32)
+        e.factor()
+        engines[prec] = e
+    e64, e32 = engines["fp64"], engines["mixed32"]
+    _say("512x512: FP64 grid form, 1 substep")
+    e64.step(1)
+    x1 = e64.state(0)[0].copy()
+    _say("512x512: FP64 grid form, 2 more substeps; mixed32 grid form,", nsub, "substeps")
+    e64.step(nsub - 1)
     e32.step(nsub)
-    _say("512x512: FP64 batch,", nsub, "substeps")
-    e64.step(nsub)
-    x32, _ = e32.state(0)
-    x64, _ = e64.state(0)
+    x64, x32 = e64.state(0)[0], e32.state(0)[0]
     err = np.abs(x32 - x64).max()
     print("512x512 max |x32 - x64|", err, "after", nsub, "substeps")
     assert np.all(np.isfinite(x32)) and err <= 1e-5
     for e in (e32, e64):
         s = e.stats(0)
         assert s["factorize_count"] == 1 and s["substeps"] == nsub and s["iterations_used"] == 40
+        e.close()
+    _say("512x512: waiting for the oracle (factor and 1 substep)")
+    orc.join()
+    assert orc.err is None, orc.err
+    xo = orc.out
+    rel = np.abs(x1 - xo).max() / np.abs(xo).max()
+    print("512x512 FP64 grid form vs oracle after 1 substep, relative", rel)
+    assert rel <= 1e-10
+
+
+def _oracle_positions(c, cfg, perm, nsub):
+    o = Oracle(c, cfg, perm, threads=4)
+    o.step(nsub)
+    return o.positions()
 
 
 def _scaled(c, s):
