@@ -1807,14 +1807,14 @@ constexpr int kLB = 8;  // rows per phase B task of a level sweep (the panels' h
 //    step st-1 is done (done[st-1], which by induction completes every earlier
 //    step; step 0 waits on prev / prevTarget when given).  Each lane subtracts its
 //    row's partials in run order; short chains finish with their inverse rows as
-//    in the one-workgroup sweep; a long chain's row goes to R.  Counts itself in
-//    doneA[st] and done[st].
+//    in the one-workgroup sweep; a long chain's row goes to R.  A wavefront
+//    counts its finished tasks of the step in doneA[st] and done[st] at once.
 //  * phase B: task j goes to wavefront (nA + j) mod GW, waits until every phase
 //    A task of the step is done, and applies 8 rows of a long chain's inverse to
 //    R: each lane reads 16 bytes of each row and of R's three components at a
 //    time (R through device-scope loads, as it was written by other
 //    workgroups), a row's sum is lane-strided partial sums then a butterfly, so
-//    the bits do not depend on the grid size.  Counts itself in done[st].
+//    the bits do not depend on the grid size.  Counted in done[st] per wavefront.
 // W and R cross workgroups as device-scope stores and loads, and a wavefront
 // counts itself only once its stores are performed.  Returns false when a wait
 // gave up.
@@ -1848,6 +1848,7 @@ __device__ bool levelSweep(const BlockSweepDev& sw, int m, Real* Wp, Real* Rp, P
     int* doneSt = done + (size_t)st * kFlowStride;
     int* doneASt = doneA + (size_t)st * kFlowStride;
     bool waited = false;
+    int mine = 0;  // tasks of this phase the wavefront finished
     for (int t = ta + gw; t < tb; t += GW) {
       const v4i mt = gmeta[(size_t)t * kWave + lane];
       const Real dv = kFwd ? Real(0) : gldinv[(size_t)t * kWave + lane];
@@ -1903,19 +1904,25 @@ __device__ bool levelSweep(const BlockSweepDev& sw, int m, Real* Wp, Real* Rp, P
           store(k, a0, a1, a2);
         }
       }
-      // the task's stores are performed before it counts as complete
-      storesPerformed();
-      if (lane == 0) {
-        __hip_atomic_fetch_add(doneASt, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_fetch_add(doneSt, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      }
+      ++mine;
       // the LDS slice is this wavefront's; the next task may overwrite it
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       __builtin_amdgcn_wave_barrier();
     }
+    // the wavefront's tasks of the step count once their stores are performed
+    // (every wait is for a step's or phase's total, so one count per wavefront
+    // and phase orders the same)
+    if (mine > 0) {
+      storesPerformed();
+      if (lane == 0) {
+        __hip_atomic_fetch_add(doneASt, mine, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_add(doneSt, mine, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
     int j = gw - nA % GW;
     if (j < 0) j += GW;
     bool waitedB = false;
+    int mineB = 0;
     for (; j < nB; j += GW) {
       const int* bt = sw.bTask + 4 * (size_t)(ja + j);
       const int xoff = __builtin_amdgcn_readfirstlane(bt[0]);
@@ -1975,8 +1982,11 @@ __device__ bool levelSweep(const BlockSweepDev& sw, int m, Real* Wp, Real* Rp, P
         agentStore(Wp + 4 * k + 2, y2);
         store(k, y0, y1, y2);
       }
+      ++mineB;
+    }
+    if (mineB > 0) {
       storesPerformed();
-      if (lane == 0) __hip_atomic_fetch_add(doneSt, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (lane == 0) __hip_atomic_fetch_add(doneSt, mineB, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
   }
   return true;
