@@ -30,9 +30,13 @@ namespace {
 
 constexpr int kWave = 64;
 
-// Phase timers (device wall clock, 100 MHz) read per sweep step and per phase.
-// Reading the clock waits on the scalar-memory counter, which the LDS traffic
-// shares, so the timers are compiled in only on request (PDX_PHASE_TIMERS=1).
+// Timers (device wall clock, 100 MHz).  The stage buckets of a substep (predict
+// and velocity, gather, forward sweep, backward sweep, criterion) are always
+// kept, as the reference keeps its stage times: thread 0 reads the clock a few
+// times per iteration, at points the workgroup has reached together.  The
+// per-step timers inside the sweeps read it twice per step, and reading the
+// clock waits on the scalar-memory counter, which the LDS traffic shares, so
+// they are compiled in only on request (PDX_PHASE_TIMERS=1).
 #ifndef PDX_PHASE_TIMERS
 #define PDX_PHASE_TIMERS 0
 #endif
@@ -43,6 +47,7 @@ __device__ inline long long tick() {
   return 0;
 #endif
 }
+__device__ inline long long tickStage() { return wall_clock64(); }
 constexpr double kMinLength = 1e-12;  // DistanceTerm.h:31
 
 __device__ inline double waveMax(double v) {
@@ -1152,7 +1157,7 @@ __device__ void substepBody(const InstDev& I, WP W, int nsub, int subBase, int f
   for (int sub = 0; sub < nsub; ++sub) {
     if (kRoll && (flags & kRollTraj)) applyPinTrajectory(I, subBase + sub);
     double* tape = (kRoll && (flags & kRollTape)) ? I.tape + (size_t)sub * (I.maxIter + 1) * n3 : nullptr;
-    const long long tp0 = tick();
+    const long long tp0 = tid == 0 ? tickStage() : 0;
     // predict (Integrator.cpp:447-463) and the displacement scale
     double lscale = 0.0;
     for (int i = tid; i < nv; i += bs) {
@@ -1169,19 +1174,19 @@ __device__ void substepBody(const InstDev& I, WP W, int nsub, int subBase, int f
     }
     double scale = blockMax(lscale, red);
     scale = fmax(scale, I.gScaleH2);
-    const long long tp1 = tick();
+    const long long tp1 = tid == 0 ? tickStage() : 0;
     gatherFirst<Real, kExact>(I, W);
     __syncthreads();
     if (tid == 0) {
       tacc[3] += tp1 - tp0;
-      tacc[0] += tick() - tp1;
+      tacc[0] += tickStage() - tp1;
     }
 
     double chebOmega = 1.0;
     int used = 0, conv = 0;
     double lastRes = -1.0;
     for (int k = 0; k < I.maxIter; ++k) {
-      if (tid == 0) tk[0] = tick();
+      if (tid == 0) tk[0] = tickStage();
       runForward<Real, kBlocked, kIdx16>(I, W, swt);
       if constexpr (kPairable) {
         if (paired) {
@@ -1209,7 +1214,7 @@ __device__ void substepBody(const InstDev& I, WP W, int nsub, int subBase, int f
           runSeparator<Real, kIdx16>(I, W, swt);
         }
       }
-      if (tid == 0) tk[1] = tick();
+      if (tid == 0) tk[1] = tickStage();
       runBackward<Real, kExact, kBlocked, kIdx16>(I, W, swt);
       if (kExact && kBlocked) __syncthreads();  // the sweep stored x; the gather reads it
       if (!kExact) {
@@ -1233,7 +1238,7 @@ __device__ void substepBody(const InstDev& I, WP W, int nsub, int subBase, int f
           if (paired && !pairSync(I, ++seq, true, pairOk, sameL2)) return;
         }
       }
-      if (tid == 0) tk[2] = tick();
+      if (tid == 0) tk[2] = tickStage();
       const bool more = (k + 1 < I.maxIter);
       double* tapeK = tape ? tape + (size_t)(k + 1) * n3 : nullptr;
       double ldiff = 0.0, lres = 0.0;
@@ -1319,7 +1324,7 @@ __device__ void substepBody(const InstDev& I, WP W, int nsub, int subBase, int f
       // criterion (Integrator.cpp:673-751); two slot sets alternate so one barrier suffices
       double* rs = red + 16 * ((k & 1) ^ 1);
       double diff = blockMax(ldiff, rs);
-      if (tid == 0) tk[3] = tick();
+      if (tid == 0) tk[3] = tickStage();
       double residual = -1.0;
       if (I.wantResidual) {
         __syncthreads();
@@ -1348,7 +1353,7 @@ __device__ void substepBody(const InstDev& I, WP W, int nsub, int subBase, int f
         tacc[1] += tk[1] - tk[0];
         tacc[2] += tk[2] - tk[1];
         tacc[0] += tk[3] - tk[2];
-        tacc[4] += tick() - tk[3];
+        tacc[4] += tickStage() - tk[3];
       }
       const double rel = (scale > 0.0) ? diff / scale : diff;
       const bool absHit = (I.absTol > 0.0) && (diff <= I.absTol);
@@ -1363,7 +1368,7 @@ __device__ void substepBody(const InstDev& I, WP W, int nsub, int subBase, int f
       }
     }
     // velocity + damping + re-pin (Integrator.cpp:774-791)
-    const long long tv0 = tick();
+    const long long tv0 = tid == 0 ? tickStage() : 0;
     for (int i = tid; i < nv; i += bs) {
       const int v = ownVertex(I, i);
       const bool pinnedV = I.pinned[v] != 0;
@@ -1377,7 +1382,7 @@ __device__ void substepBody(const InstDev& I, WP W, int nsub, int subBase, int f
       }
     }
     if (tid == 0 && I.pairRole != 2) {  // a pair's lead keeps the instance's statistics
-      tacc[3] += tick() - tv0;
+      tacc[3] += tickStage() - tv0;
       I.istats[0] = used;
       I.istats[1] = conv;
       I.istats[2] += conv;

@@ -470,7 +470,7 @@ extern "C" int pdx_bench_run(int argc, const char* const* argv, char* text, long
             st.phase_seconds[1] * 1e3, st.phase_seconds[2] * 1e3);
     out.add("  criterion          %10.3f  %5.1f%%\n", st.phase_seconds[4] * 1e3, pct(st.phase_seconds[4]));
   } else {
-    out.add("  (per-phase device timers are compiled in with PDX_PHASE_TIMERS=1)\n");
+    out.add("  (the grid and dense forms keep no stage buckets: their phases overlap across workgroups)\n");
   }
   out.add("  substep kernel     %10.3f  (%.4f ms per substep)\n", deviceMs, deviceMs / steps);
   out.add("  analysis+factor    %10.3f  (host, only when the stamp changes)\n",

@@ -209,3 +209,24 @@ def test_report_loads_obj_mesh(tmp_path):
     # a bad path falls back to the grid, as makeScene does
     text, s2 = report.run(["--mesh", str(tmp_path / "absent.obj"), "--grid", "6", "5", "--steps", "2"])
     assert s2["loaded_mesh"] == 0 and s2["n_vertices"] == 30 and "did not load" in text
+
+
+def test_stage_buckets_always_kept():
+    """The reference keeps stage times on every step (Scene.h:140-149); so does the
+    batch kernel: predict + velocity, gather, forward and backward sweeps and the
+    criterion, each positive, together within the substep kernel's time, and the
+    report prints the table."""
+    _say("stage buckets 40x40 exact FP64, 20 substeps")
+    c = scenes.grid_cloth(40, 40, 2300.0)
+    cfg = SimConfig.fixed_iterations(10, scenes.H600)
+    e = Engine("fp64", "exact")
+    e.add(c, cfg)
+    e.step(20)
+    ph = e.stats(0)["phase_seconds"][:5]
+    kernel_s = e.last_kernel_ms() * 1e-3
+    e.close()
+    print("stage seconds", ph, "kernel", kernel_s)
+    assert all(p > 0.0 for p in ph)
+    assert sum(ph) <= 1.05 * kernel_s and sum(ph) >= 0.5 * kernel_s
+    text, _ = report.run(["--grid", "20", "20", "--steps", "5"])
+    assert "global solve" in text and "project+scatter+residual gather" in text
