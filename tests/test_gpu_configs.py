@@ -1,5 +1,8 @@
 """The workloads at their full sizes on the GPU, checked against the FP64 paths:
 
+* config 1 (40 x 40, 600 substeps) and config 2 (100 x 100 with shear and bend,
+  300 substeps) in every FP64 form and the FP32-solve form against the CPU
+  oracle over the whole run;
 * the 512-instance mix, 200 substeps, through pdx_step_graph (the benchmark's
   path): the FP32-solve form in the nested-dissection order against the FP64
   batch in the reference's AMD order, every instance within 1e-5 m, and four
@@ -50,6 +53,83 @@ class _Bg(threading.Thread):
         if self.err is not None:
             raise self.err
         return self.out
+
+
+def _amd_perm(c, cfg):
+    e = Engine("fp64", "exact")
+    e.add(c, cfg)
+    perm = e.factor_arrays(0)["perm3"]
+    e.close()
+    return perm
+
+
+def _run(c, cfg, nsub, precision, solve, ordering=None, launch=None):
+    e = Engine(precision, solve)
+    e.add(c, cfg)
+    if ordering:
+        e.set_ordering(ordering)
+    if launch:
+        e.set_launch(*launch)
+    e.step(nsub)
+    x = e.state(0)[0]
+    st = e.stats(0)
+    e.close()
+    assert st["factorize_count"] == 1 and st["substeps"] == nsub
+    return x
+
+
+@pytest.mark.timeout(900)
+def test_config1_full_length():
+    """40 x 40, distance only, 10 iterations, 600 substeps, against the FP64 CPU
+    oracle (AMD order) over the whole run: the exact FP64 form bit for bit, the
+    dense and the sparse (nested dissection, one workgroup) FP64 forms within
+    1e-10 relative, the FP32-solve form within 1e-5 m."""
+    c, cfg, nsub = scenes.config1()
+    perm = _amd_perm(c, cfg)
+    orc = _Bg(lambda: _oracle_positions(c, cfg, perm, nsub))
+    _say("config 1: exact FP64,", nsub, "substeps")
+    xe = _run(c, cfg, nsub, "fp64", "exact")
+    _say("config 1: dense FP64")
+    xd = _run(c, cfg, nsub, "fp64", "dense")
+    _say("config 1: sparse FP64, ND")
+    xs = _run(c, cfg, nsub, "fp64", "fast", "nd")
+    _say("config 1: sparse mixed32, ND")
+    x32 = _run(c, cfg, nsub, "mixed32", "fast", "nd")
+    xo = orc.result()
+    scale = np.abs(xo).max()
+    assert np.array_equal(xe.view(np.uint64), xo.view(np.uint64))
+    for x in (xd, xs):
+        assert np.abs(x - xo).max() <= 1e-10 * scale
+    assert np.abs(x32 - xo).max() <= 1e-5
+
+
+@pytest.mark.timeout(900)
+def test_config2_full_length():
+    """100 x 100 with shear and bending, sigma 1e-3 m perturbation, 40 iterations,
+    300 substeps, against the FP64 CPU oracle over the whole run: the dense FP64
+    form and the grid form (level sweeps, nested dissection) in FP64 within 1e-10
+    relative, the FP32-solve grid form within 1e-5 m; the exact FP64 form bit for
+    bit over its first 2 substeps (1.5 s each on the GPU: the reference's own
+    dependent chain of subtractions)."""
+    c, cfg, nsub = scenes.config2()
+    perm = _amd_perm(c, cfg)
+    orc = _Bg(lambda: _oracle_positions(c, cfg, perm, nsub))
+    orc2 = _oracle_positions(c, cfg, perm, 2)
+    _say("config 2: exact FP64, 2 substeps")
+    xe = _run(c, cfg, 2, "fp64", "exact")
+    assert np.array_equal(xe.view(np.uint64), orc2.view(np.uint64))
+    _say("config 2: dense FP64,", nsub, "substeps")
+    xd = _run(c, cfg, nsub, "fp64", "dense")
+    _say("config 2: grid form FP64, ND")
+    xg = _run(c, cfg, nsub, "fp64", "fast", "nd", ("grid", 32))
+    _say("config 2: grid form mixed32, ND")
+    x32 = _run(c, cfg, nsub, "mixed32", "fast", "nd", ("grid", 32))
+    _say("config 2: waiting for the oracle")
+    xo = orc.result()
+    scale = np.abs(xo).max()
+    for x in (xd, xg):
+        assert np.abs(x - xo).max() <= 1e-10 * scale
+    assert np.abs(x32 - xo).max() <= 1e-5
 
 
 @pytest.mark.timeout(900)
