@@ -287,8 +287,10 @@ int pdx_get_gradients(pdx_ctx* ctx, int instance, double* g_x0, double* g_v0, do
 int pdx_rollout_timing(const pdx_ctx* ctx, double* forward_ms, double* adjoint_ms);
 
 /* Fast-mode schedule of one instance, 8 values per sweep (forward, then backward):
- * steps, tasks, chain tasks, rows in chains, runs, dense values, most tasks in
- * one step, most runs in one step. */
+ * steps, tasks (in the grid form's level sweeps: with the dense external tasks),
+ * chain tasks, rows in chains, runs, dense values (chain inverses, and in the
+ * level sweeps also inverse and external panels), most tasks in one step, most
+ * runs in one step. */
 int pdx_get_schedule_stats(pdx_ctx* ctx, int instance, int64_t* out16);
 
 /* Launch form (PDX_LAUNCH_*); grid_blocks = workgroups of the grid form, 0 = all

@@ -144,6 +144,8 @@ struct HostBlockSweep {
   std::vector<int> taskRun, stepB, bTask;
   std::vector<double> xpanel;
   int rTotal = 0;
+  std::vector<int> stepE, eTask, ucol;
+  std::vector<double> epanel;
 };
 
 // Block schedule of one sweep (fast mode).  Ranks are cut into contiguous
@@ -422,6 +424,10 @@ void chainInverse(const Factor& f, int s0, int nr, std::vector<double>& X) {
 //    multiple of 4 columns.
 constexpr int kWaveLanes = 64;     // lanes of a wavefront
 constexpr long long kLevelWaves = 1024;  // wavefronts a level step's runs are spread over
+// a long chain takes dense phase A tasks when its external entries fill at least
+// this share of rows x union columns (backward sweeps of nested-dissection
+// separators: about 1.0; forward: 0.2-0.3)
+constexpr double kDenseExternal = 0.6;
 
 void packLevels(const Factor& f, int cap, int waveRuns, HostBlockSweep& outF, HostBlockSweep& outB) {
   const int m = f.m;
@@ -486,16 +492,18 @@ void packLevels(const Factor& f, int cap, int waveRuns, HostBlockSweep& outF, Ho
     std::vector<std::vector<int>> byLevel((size_t)nlev);
     for (int b = 0; b < nb; ++b) byLevel[(size_t)blev[(size_t)b]].push_back(b);
     std::vector<int> rpos((size_t)nb, -1);
+    std::vector<std::vector<int>> unions((size_t)nb);
     out.stepTask.push_back(0);
     out.stepRun.push_back(0);
     out.stepDense.push_back(0);
     out.stepB.push_back(0);
+    out.stepE.push_back(0);
     struct Item {
       int block, len, runs, row;  // row >= 0: one row of a long chain
     };
     for (int l = 0; l < nlev; ++l) {
       std::vector<Item> items;
-      std::vector<int> longChains;
+      std::vector<int> longChains, denseChains;
       for (int b : byLevel[(size_t)l]) {
         const int s0 = bs[(size_t)b], len = bs[(size_t)b + 1] - s0;
         int runs = 0;
@@ -506,7 +514,24 @@ void packLevels(const Factor& f, int cap, int waveRuns, HostBlockSweep& outF, Ho
           longChains.push_back(b);
           rpos[(size_t)b] = out.rTotal;
           out.rTotal += round4(len);
-          for (int k = s0; k < s0 + len; ++k) items.push_back(Item{b, 1, rowRuns(k, b), k});
+          // the union of the rows' external columns; dense enough, the chain's
+          // phase A is dense tasks over it rather than per-row runs
+          std::vector<int> U;
+          long long nnz = 0;
+          for (int k = s0; k < s0 + len; ++k) {
+            int p0, p1;
+            extRange(k, b, p0, p1);
+            nnz += p1 - p0;
+            for (int p = p0; p < p1; ++p) U.push_back(fwd ? f.rc[(size_t)p] : f.ci[(size_t)p]);
+          }
+          std::sort(U.begin(), U.end());
+          U.erase(std::unique(U.begin(), U.end()), U.end());
+          if (U.size() >= (size_t)kWaveLanes && (double)nnz >= kDenseExternal * (double)len * (double)U.size()) {
+            denseChains.push_back(b);
+            unions[(size_t)b] = std::move(U);
+          } else {
+            for (int k = s0; k < s0 + len; ++k) items.push_back(Item{b, 1, rowRuns(k, b), k});
+          }
         }
       }
       std::stable_sort(items.begin(), items.end(), [](const Item& a, const Item& b) {
@@ -602,6 +627,33 @@ void packLevels(const Factor& f, int cap, int waveRuns, HostBlockSweep& outF, Ho
       }
       while ((out.dense.size() - (size_t)denseBase) % 4) out.dense.push_back(0.0);
       out.maxDense = std::max(out.maxDense, (int)out.dense.size() - denseBase);
+      // phase A's dense tasks: per 8 rows of a dense chain, its external entries
+      // over the union U, column-major (8 values per column, zero where a row has
+      // no entry)
+      for (int b : denseChains) {
+        const int s0 = bs[(size_t)b], len = bs[(size_t)b + 1] - s0;
+        const std::vector<int>& U = unions[(size_t)b];
+        const int c = (int)U.size();
+        const int uoff = (int)out.ucol.size();
+        out.ucol.insert(out.ucol.end(), U.begin(), U.end());
+        for (int r0 = 0; r0 < len; r0 += 8) {
+          const int nrows = std::min(8, len - r0);
+          const size_t eoff = out.epanel.size();
+          out.epanel.resize(eoff + (size_t)c * 8, 0.0);
+          for (int q = 0; q < nrows; ++q) {
+            const int k = s0 + r0 + q;
+            int p0, p1;
+            extRange(k, b, p0, p1);
+            for (int p = p0; p < p1; ++p) {
+              const int col = fwd ? f.rc[(size_t)p] : f.ci[(size_t)p];
+              const int j = (int)(std::lower_bound(U.begin(), U.end(), col) - U.begin());
+              out.epanel[eoff + (size_t)j * 8 + q] = fwd ? f.rx[(size_t)p] : f.cx[(size_t)p];
+            }
+          }
+          const int et[8] = {(int)eoff, c, uoff, (s0 + r0) | (nrows << 24), rpos[(size_t)b] + r0, 0, 0, 0};
+          out.eTask.insert(out.eTask.end(), et, et + 8);
+        }
+      }
       // phase B: the long chains' panels
       for (int b : longChains) {
         const int s0 = bs[(size_t)b], s = bs[(size_t)b + 1] - s0, spad = round4(s);
@@ -629,6 +681,7 @@ void packLevels(const Factor& f, int cap, int waveRuns, HostBlockSweep& outF, Ho
       out.stepRun.push_back((int)(out.runCol.size() / kRun));
       out.stepDense.push_back((int)out.dense.size());
       out.stepB.push_back((int)(out.bTask.size() / 4));
+      out.stepE.push_back((int)(out.eTask.size() / 8));
     }
     out.taskRun.push_back((int)(out.runCol.size() / kRun));
     out.nstep = nlev;
@@ -1033,7 +1086,7 @@ int upload(pdx_ctx* c) {
     size_t x, v, xbs, xhat, prev, mass, pinned, pinpos, brhs, ea, eb, rest, kappa, vstart, vedges, vedgesC,
         bstart, bst, sa, sb, sc, bk, vrank, rankv, dinv, W, ist, lst, dst, recE, rkE, recC, rkC, brR, brK;
     size_t f[9], b[9];
-    size_t sch[6][19];  // block schedules bf, bb, sf, sb, pf, pb
+    size_t sch[6][23];  // block schedules bf, bb, sf, sb, pf, pb
     size_t ownLead = 0, ownPart = 0, pairFlags = 0, pairRed = 0, xbuf = 0, flow = 0, flowPart = 0;
     int nOwnLead = 0, nOwnPart = 0;
   };
@@ -1149,6 +1202,10 @@ int upload(pdx_ctx* c) {
       bo[q][16] = ar.vec(h.stepB);
       bo[q][17] = ar.vec(h.bTask);
       bo[q][18] = fp32 ? ar.vec(castVec<float>(h.xpanel)) : ar.vec(h.xpanel);
+      bo[q][19] = ar.vec(h.stepE);
+      bo[q][20] = ar.vec(h.eTask);
+      bo[q][21] = ar.vec(h.ucol);
+      bo[q][22] = fp32 ? ar.vec(castVec<float>(h.epanel)) : ar.vec(h.epanel);
       if (c->idx16) {
         std::vector<uint16_t> c16(h.runCol.begin(), h.runCol.end());
         bo[q][9] = ar.vec(c16);
@@ -1379,6 +1436,10 @@ int upload(pdx_ctx* c) {
       bd->xpanel = B + bo[18];
       bd->rTotal = h.rTotal;
       bd->pad = 0;
+      bd->stepE = (const int*)(B + bo[19]);
+      bd->eTask = (const int*)(B + bo[20]);
+      bd->ucol = (const int*)(B + bo[21]);
+      bd->epanel = B + bo[22];
     };
     fillSweep(&d.bfwd, o.sch[0], I.bf);
     fillSweep(&d.bbwd, o.sch[1], I.bb);
@@ -1716,6 +1777,7 @@ bool replayLevels(const Instance& I, std::string& why, double& relErr, const std
     const size_t nrun = h.runCol.size() / kRun;
     const size_t ntask = h.taskInfo.size(), nb = h.bTask.size() / 4;
     if (h.stepTask.size() != (size_t)h.nstep + 1 || h.stepB.size() != (size_t)h.nstep + 1 ||
+        h.stepE.size() != (size_t)h.nstep + 1 || h.eTask.size() % 8 != 0 ||
         h.stepDense.size() != (size_t)h.nstep + 1 || h.taskRun.size() != ntask + 1 ||
         h.laneMeta.size() != ntask * 256 || h.laneDense.size() != ntask * 64 || h.rTotal % 4 != 0)
       return fault("tables of the wrong length");
@@ -1793,6 +1855,34 @@ bool replayLevels(const Instance& I, std::string& why, double& relErr, const std
             W[(size_t)kk[ln]] = y[ln];
             done.push_back(kk[ln]);
           }
+        }
+      }
+      // phase A's dense tasks: r = base - E W[U] for 8 rows of a dense chain
+      const int e0 = h.stepE[(size_t)st], e1 = h.stepE[(size_t)st + 1];
+      if (e0 < 0 || e1 < e0 || (size_t)e1 * 8 > h.eTask.size()) return fault("dense task range out of bounds");
+      for (int j = e0; j < e1; ++j) {
+        const int* et = h.eTask.data() + (size_t)j * 8;
+        const int eoff = et[0], c = et[1], uoff = et[2], k0 = et[3] & 0xffffff, nrows = et[3] >> 24, rp = et[4];
+        if (nrows < 1 || nrows > 8 || c < 1 || eoff < 0 || eoff % 8 || (size_t)eoff + (size_t)c * 8 > h.epanel.size() ||
+            uoff < 0 || (size_t)uoff + (size_t)c > h.ucol.size() || rp < 0 || rp + nrows > h.rTotal || k0 + nrows > m)
+          return fault("dense task out of bounds");
+        for (int q = 0; q < nrows; ++q) {
+          const int k = k0 + q;
+          if (state[(size_t)k] != need - 1) return fault("dense task on rank " + std::to_string(k) + " finished");
+          double x = W[(size_t)k];
+          if (!fwd) x = x * f.dinv[(size_t)k];
+          double acc = 0.0;
+          for (int u = 0; u < c; ++u) {
+            const double ev = h.epanel[(size_t)eoff + (size_t)u * 8 + q];
+            const int col = h.ucol[(size_t)uoff + u];
+            if (col < 0 || col >= m) return fault("union column out of range");
+            if (ev == 0.0) continue;
+            if (state[(size_t)col] != need) return fault("dense task reads rank " + std::to_string(col) + " before it is final");
+            acc += ev * W[(size_t)col];
+          }
+          if (rStep[(size_t)(rp + q)] >= 0) return fault("R position written twice");
+          R[(size_t)(rp + q)] = x - acc;
+          rStep[(size_t)(rp + q)] = st;
         }
       }
       for (int j = j0; j < j1; ++j) {
@@ -2644,14 +2734,14 @@ int pdx_get_schedule_stats(pdx_ctx* c, int i, int64_t* out) {
       maxTasks = std::max<int64_t>(maxTasks, h.stepTask[(size_t)st + 1] - h.stepTask[(size_t)st]);
     int64_t* o = out + 8 * q;
     o[0] = h.nstep;
-    o[1] = (int64_t)h.taskInfo.size();
+    o[1] = (int64_t)(h.taskInfo.size() + h.eTask.size() / 8);
     o[2] = chains;
     o[3] = chainRows;
     o[4] = (int64_t)h.runCol.size() / kRun;
     int64_t maxRuns = 0;
     for (int st = 0; st < h.nstep; ++st)
       maxRuns = std::max<int64_t>(maxRuns, h.stepRun[(size_t)st + 1] - h.stepRun[(size_t)st]);
-    o[5] = (int64_t)(h.dense.size() + h.xpanel.size());
+    o[5] = (int64_t)(h.dense.size() + h.xpanel.size() + h.epanel.size());
     o[6] = maxTasks;
     o[7] = maxRuns;
   }

@@ -80,6 +80,16 @@ struct BlockSweepDev {
   const void* xpanel;     // the long chains' inverses, panels of 8 rows (Real, zero padded)
   int rTotal;             // R values per component (each chain's range padded to 4)
   int pad;
+  // Dense external blocks: a long chain whose rows share most of their external
+  // columns (the backward sweep's supernodes) takes phase A as dense tasks, 8
+  // rows each: r = base - E W[U] over the chain's union U of external columns,
+  // eTask = {offset of the panel in epanel (8 values per column, column-major),
+  // columns, offset of U in ucol, first rank | rows << 24, R position of row 0,
+  // 0, 0, 0}; the step's dense tasks are [stepE[st], stepE[st+1]).
+  const int* stepE;       // nstep + 1
+  const int* eTask;       // 8 per dense task
+  const int* ucol;        // union columns (ranks)
+  const void* epanel;     // Real
 };
 
 // Exchange area of an instance split over two workgroups (a pair).  Each

@@ -1823,9 +1823,14 @@ constexpr int kLB = 8;  // rows per phase B task of a level sweep (the panels' h
 // W and R cross workgroups as device-scope stores and loads, and a wavefront
 // counts itself only once its stores are performed.  Returns false when a wait
 // gave up.
+// Tasks of level step st, both phases (the counts its done counter reaches per sweep).
+__device__ inline int levelStepTasks(const BlockSweepDev& sw, int st) {
+  return sw.stepTask[st + 1] - sw.stepTask[st] + sw.stepE[st + 1] - sw.stepE[st] + sw.stepB[st + 1] - sw.stepB[st];
+}
+
 template <typename Real, bool kFwd, bool kIdx16, typename Store>
-__device__ bool levelSweep(const BlockSweepDev& sw, int m, Real* Wp, Real* Rp, PDX_LDS Real* Pw, Store store, int* done,
-                           int* doneA, int rep, const int* prev, int prevTarget, int* giveUp) {
+__device__ bool levelSweep(const BlockSweepDev& sw, int m, const Real* dinv, Real* Wp, Real* Rp, PDX_LDS Real* Pw,
+                           Store store, int* done, int* doneA, int rep, const int* prev, int prevTarget, int* giveUp) {
   typedef typename Vec16<Real>::T VT;
   constexpr int V = Vec16<Real>::n;
   const int lane = threadIdx.x & (kWave - 1);
@@ -1836,6 +1841,8 @@ __device__ bool levelSweep(const BlockSweepDev& sw, int m, Real* Wp, Real* Rp, P
   const Real* rval = static_cast<const Real*>(sw.runVal);
   const PDX_GLOBAL Real* gdense = gview<Real>(sw.dense);
   const PDX_GLOBAL Real* gx = gview<Real>(sw.xpanel);
+  const PDX_GLOBAL Real* gep = gview<Real>(sw.epanel);
+  const PDX_GLOBAL int* gucol = gview<int>(sw.ucol);
   const PDX_GLOBAL v4i* gmeta = gview<v4i>(sw.laneMeta);
   const PDX_GLOBAL Real* gldinv = gview<Real>(sw.laneDinv);
   const int rT = sw.rTotal;
@@ -1845,10 +1852,10 @@ __device__ bool levelSweep(const BlockSweepDev& sw, int m, Real* Wp, Real* Rp, P
   for (int st = 0; st < sw.nstep; ++st) {
     const int ta = sw.stepTask[st], tb = sw.stepTask[st + 1];
     const int ja = sw.stepB[st], jb = sw.stepB[st + 1];
-    const int nA = tb - ta, nB = jb - ja;
+    const int ea = sw.stepE[st], eb = sw.stepE[st + 1];
+    const int nA = tb - ta, nB = jb - ja, nE = eb - ea;
     const int* prevOn = st > 0 ? done + (size_t)(st - 1) * kFlowStride : prev;
-    const int prevFor =
-        st > 0 ? rep * (sw.stepTask[st] - sw.stepTask[st - 1] + sw.stepB[st] - sw.stepB[st - 1]) : prevTarget;
+    const int prevFor = st > 0 ? rep * levelStepTasks(sw, st - 1) : prevTarget;
     const int dbase = sw.stepDense[st];
     int* doneSt = done + (size_t)st * kFlowStride;
     int* doneASt = doneA + (size_t)st * kFlowStride;
@@ -1914,6 +1921,74 @@ __device__ bool levelSweep(const BlockSweepDev& sw, int m, Real* Wp, Real* Rp, P
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       __builtin_amdgcn_wave_barrier();
     }
+    // dense tasks of phase A (a dense chain's 8 rows): r = base - E W[U], each
+    // lane reading the W values of its columns of U once and the 8 rows' entries
+    // of each (column-major panel), a row's sum lane-strided then a butterfly
+    {
+      int j = gw - nA % GW;
+      if (j < 0) j += GW;
+      for (; j < nE; j += GW) {
+        const int* et = sw.eTask + 8 * (size_t)(ea + j);
+        const int eoff = __builtin_amdgcn_readfirstlane(et[0]);
+        const int c = __builtin_amdgcn_readfirstlane(et[1]);
+        const int uoff = __builtin_amdgcn_readfirstlane(et[2]);
+        const int kr = __builtin_amdgcn_readfirstlane(et[3]);
+        const int rp = __builtin_amdgcn_readfirstlane(et[4]);
+        const int k0 = kr & 0xffffff, nrows = kr >> 24;
+        if (!waited) {
+          if (prevOn && !flowWait(prevOn, prevFor, giveUp)) return false;
+          waited = true;
+        }
+        Real acc[kLB][3];
+#pragma unroll
+        for (int q = 0; q < kLB; ++q) acc[q][0] = acc[q][1] = acc[q][2] = Real(0);
+        for (int jj = lane; jj < c; jj += kWave) {
+          const int col = gucol[uoff + jj];
+          Real w0, w1, w2;
+          loadW4<Real>(wsrc, col, w0, w1, w2);
+          const PDX_GLOBAL VT* ep = (const PDX_GLOBAL VT*)(gep + (size_t)eoff + (size_t)jj * kLB);
+          VT ev[kLB / V];
+#pragma unroll
+          for (int u = 0; u < kLB / V; ++u) ev[u] = ep[u];
+#pragma unroll
+          for (int q = 0; q < kLB; ++q) {
+            const Real e = ev[q / V][q % V];
+            acc[q][0] = fmaT(e, w0, acc[q][0]);
+            acc[q][1] = fmaT(e, w1, acc[q][1]);
+            acc[q][2] = fmaT(e, w2, acc[q][2]);
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < kLB; ++q)
+#pragma unroll
+          for (int d = 0; d < 3; ++d)
+#pragma unroll
+            for (int o = kWave / 2; o >= 1; o >>= 1) acc[q][d] += __shfl_xor(acc[q][d], o);
+        Real s0 = 0, s1 = 0, s2 = 0;
+#pragma unroll
+        for (int q = 0; q < kLB; ++q)
+          if (lane == q) {
+            s0 = acc[q][0];
+            s1 = acc[q][1];
+            s2 = acc[q][2];
+          }
+        if (lane < nrows) {
+          const int k = k0 + lane;
+          Real a0, a1, a2;
+          loadW4<Real>(wsrc, k, a0, a1, a2);
+          if (!kFwd) {
+            const Real dv = dinv[k];
+            a0 = a0 * dv;
+            a1 = a1 * dv;
+            a2 = a2 * dv;
+          }
+          agentStore(Rp + rp + lane, a0 - s0);
+          agentStore(Rp + rT + rp + lane, a1 - s1);
+          agentStore(Rp + 2 * rT + rp + lane, a2 - s2);
+        }
+        ++mine;
+      }
+    }
     // the wavefront's tasks of the step count once their stores are performed
     // (every wait is for a step's or phase's total, so one count per wavefront
     // and phase orders the same)
@@ -1924,7 +1999,7 @@ __device__ bool levelSweep(const BlockSweepDev& sw, int m, Real* Wp, Real* Rp, P
         __hip_atomic_fetch_add(doneSt, mine, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
     }
-    int j = gw - nA % GW;
+    int j = gw - (nA + nE) % GW;
     if (j < 0) j += GW;
     bool waitedB = false;
     int mineB = 0;
@@ -1940,7 +2015,7 @@ __device__ bool levelSweep(const BlockSweepDev& sw, int m, Real* Wp, Real* Rp, P
       for (int q = 0; q < kLB; ++q)  // rows past the panel repeat its last row (their sums are dropped)
         row[q] = (const PDX_GLOBAL VT*)(gx + xoff + (size_t)min(q, nrows - 1) * ld);
       if (!waitedB) {
-        if (!flowWait(doneASt, rep * nA, giveUp)) return false;
+        if (!flowWait(doneASt, rep * (nA + nE), giveUp)) return false;
         waitedB = true;
       }
       Real acc[kLB][3];
@@ -2100,7 +2175,7 @@ __global__ void __launch_bounds__(512) gridSubstepKernel(const InstDev* __restri
   if (!kDense && blockIdx.x == 0)
     for (int i = threadIdx.x; i < 2 * (nF + nB); i += blockDim.x) I.flow[(size_t)i * kFlowStride] = 0;
   int rep = 0;  // sweeps of this launch
-  const int lastF = nF > 0 ? I.bfwd.stepTask[nF] - I.bfwd.stepTask[nF - 1] + I.bfwd.stepB[nF] - I.bfwd.stepB[nF - 1] : 0;
+  const int lastF = nF > 0 ? levelStepTasks(I.bfwd, nF - 1) : 0;
   for (int sub = 0; sub < nsub; ++sub) {
     // predict (Integrator.cpp:447-463), displacement scale, first right-hand side
     double lscale = 0.0;
@@ -2186,9 +2261,9 @@ __global__ void __launch_bounds__(512) gridSubstepKernel(const InstDev* __restri
         int* doneF = I.flow;
         int* doneB = I.flow + (size_t)2 * nF * kFlowStride;
         Real* R = static_cast<Real*>(I.flowPart);
-        if (levelSweep<Real, true, kIdx16>(I.bfwd, I.m, (Real*)W, R, Pw, none, doneF, doneF + (size_t)nF * kFlowStride,
+        if (levelSweep<Real, true, kIdx16>(I.bfwd, I.m, static_cast<const Real*>(I.dinv), (Real*)W, R, Pw, none, doneF, doneF + (size_t)nF * kFlowStride,
                                            rep, nullptr, 0, I.pair.giveUp))
-          levelSweep<Real, false, kIdx16>(I.bbwd, I.m, (Real*)W, R, Pw, put, doneB, doneB + (size_t)nB * kFlowStride, rep,
+          levelSweep<Real, false, kIdx16>(I.bbwd, I.m, static_cast<const Real*>(I.dinv), (Real*)W, R, Pw, put, doneB, doneB + (size_t)nB * kFlowStride, rep,
                                            nF > 0 ? doneF + (size_t)(nF - 1) * kFlowStride : nullptr, rep * lastF,
                                            I.pair.giveUp);
         grid.sync();
@@ -2289,13 +2364,13 @@ __global__ void __launch_bounds__(512) gridSolveKernel(const InstDev* __restrict
     xout[3 * v + 1] = (double)a1;
     xout[3 * v + 2] = (double)a2;
   };
-  const int lastF = nF > 0 ? I.bfwd.stepTask[nF] - I.bfwd.stepTask[nF - 1] + I.bfwd.stepB[nF] - I.bfwd.stepB[nF - 1] : 0;
+  const int lastF = nF > 0 ? levelStepTasks(I.bfwd, nF - 1) : 0;
   int* doneF = I.flow;
   int* doneB = I.flow + (size_t)2 * nF * kFlowStride;
   Real* R = static_cast<Real*>(I.flowPart);
-  if (levelSweep<Real, true, kIdx16>(I.bfwd, I.m, (Real*)W, R, Pw, none, doneF, doneF + (size_t)nF * kFlowStride, 1,
+  if (levelSweep<Real, true, kIdx16>(I.bfwd, I.m, static_cast<const Real*>(I.dinv), (Real*)W, R, Pw, none, doneF, doneF + (size_t)nF * kFlowStride, 1,
                                      nullptr, 0, I.pair.giveUp))
-    levelSweep<Real, false, kIdx16>(I.bbwd, I.m, (Real*)W, R, Pw, put, doneB, doneB + (size_t)nB * kFlowStride, 1,
+    levelSweep<Real, false, kIdx16>(I.bbwd, I.m, static_cast<const Real*>(I.dinv), (Real*)W, R, Pw, put, doneB, doneB + (size_t)nB * kFlowStride, 1,
                                      nF > 0 ? doneF + (size_t)(nF - 1) * kFlowStride : nullptr, lastF, I.pair.giveUp);
 }
 
