@@ -126,12 +126,17 @@ typedef struct {
   int bwd_steps;
   double analyze_seconds;
   double factorize_seconds;
-  /* device time of the instance's workgroup per phase, summed over substeps (zero
-   * unless the library was built with PDX_PHASE_TIMERS=1; batch form only):
-   * gather (rhs + residual), forward sweep, backward sweep, predict + velocity,
-   * criterion reductions; fast mode, both sweeps: run streaming of thread 0, its wait at
-   * the step barrier, finalisation of wavefront 0, its wait at the second barrier */
+  /* device time of the instance's workgroup per phase, summed over substeps (batch
+   * form only): gather (rhs + residual), forward sweep, backward sweep, predict +
+   * velocity, criterion reductions (always kept, the reference's stage buckets);
+   * then, only in a library built with PDX_PHASE_TIMERS=1 (zero otherwise), fast
+   * mode, both sweeps: run streaming of thread 0, its wait at the step barrier,
+   * finalisation of wavefront 0, its wait at the second barrier */
   double phase_seconds[9];
+  /* PDX_PHASE_TIMERS=1 builds: start and end of the instance's workgroup in the
+   * last batch launch, and of its partner when split over two (device clock,
+   * seconds); zero otherwise */
+  double workgroup_seconds[4];
 } pdx_stats;
 
 /* --- context ---------------------------------------------------------------- */

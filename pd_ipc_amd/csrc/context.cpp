@@ -2248,6 +2248,7 @@ int pdx_get_stats(pdx_ctx* c, int i, pdx_stats* s) {
     if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->device) != hipSuccess || khz <= 0)
       khz = 100000;
     for (int q = 0; q < 9; ++q) s->phase_seconds[q] = (double)lst[2 + q] / (khz * 1e3);
+    for (int q = 0; q < 4; ++q) s->workgroup_seconds[q] = (double)lst[11 + q] / (khz * 1e3);
     HIPCHECK(c, hipMemcpy(dst, I.ddstats, 16, hipMemcpyDeviceToHost));
     s->iterations_used = ist[0];
     s->converged = ist[1];

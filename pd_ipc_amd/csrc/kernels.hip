@@ -1409,10 +1409,21 @@ __global__ void __launch_bounds__(512) substepKernel(const InstDev* __restrict__
                                                       int subBase, int flags) {
   if (!kRoll) flags = 0;
   const InstDev& I = insts[order[blockIdx.x]];
+#if PDX_PHASE_TIMERS
+  // the workgroup's start and end on the device clock (a pair's partner in slots 13, 14)
+  const long long t0 = tick();
+#endif
   if (kWL)
     substepBody<Real, kExact, kBlocked, kIdx16, kRoll>(I, ldsAt<Real>(32 * sizeof(double)), nsub, subBase, flags);
   else
     substepBody<Real, kExact, kBlocked, kIdx16, kRoll>(I, (PDX_GLOBAL Real*)I.Wg, nsub, subBase, flags);
+#if PDX_PHASE_TIMERS
+  if (threadIdx.x == 0) {
+    const int slot = I.pairRole == 2 ? 13 : 11;
+    I.lstats[slot] = t0;
+    I.lstats[slot + 1] = tick();
+  }
+#endif
 }
 
 // Reverse mode of `nsub` substeps (global indices subBase ..), one workgroup per

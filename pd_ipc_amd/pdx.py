@@ -58,7 +58,8 @@ class _Stats(C.Structure):
                 ("last_residual", C.c_double), ("factor_nnz", C.c_longlong),
                 ("fwd_levels", C.c_int), ("bwd_levels", C.c_int), ("fwd_steps", C.c_int),
                 ("bwd_steps", C.c_int), ("analyze_seconds", C.c_double),
-                ("factorize_seconds", C.c_double), ("phase_seconds", C.c_double * 9)]
+                ("factorize_seconds", C.c_double), ("phase_seconds", C.c_double * 9),
+                ("workgroup_seconds", C.c_double * 4)]
 
 
 class _Diag(C.Structure):
@@ -380,6 +381,7 @@ class Engine:
         self._check(self.lib.pdx_get_stats(self.h, i, C.byref(s)), "get_stats")
         out = {k: getattr(s, k) for k, _ in _Stats._fields_}
         out["phase_seconds"] = list(s.phase_seconds)
+        out["workgroup_seconds"] = list(s.workgroup_seconds)
         return out
 
     def diagnostics(self, i=0):
