@@ -1151,6 +1151,14 @@ __device__ void substepBody(const InstDev& I, WP W, int nsub, int subBase, int f
   }
 
   if (kBlocked) loadStepTables(I);  // visible after the first barrier below
+  if (paired && I.pairRole == 2 && tid < 3) {
+    // Runs are padded with entries of value 0 at column 0, and rank 0 is in the
+    // lead's half: the partner's W holds no value there until the first
+    // exchange, only what the LDS held before this workgroup (a NaN or infinity
+    // pattern there turns 0 * W[0] into NaN).  Zero makes every padded product
+    // +-0, which leaves a partial sum's bits as they are with any finite W[0].
+    W[tid] = (Real)0;
+  }
   long long tk[4] = {0, 0, 0, 0};
   long long tacc[5] = {0, 0, 0, 0, 0};
   SweepTimes swt;
