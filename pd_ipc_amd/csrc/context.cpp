@@ -1230,9 +1230,9 @@ int upload(pdx_ctx* c) {
     }
     // W in HBM: 3 Real per rank, 4 in the grid form's level sweeps
     o.W = c->ldsW ? 0 : ar.add(nullptr, (size_t)(c->launchMode == 1 && !dense ? 4 : 3) * I.fac.m * w);
-    // the grid form's step counters, one 256-byte line each (kFlowStride in the
-    // kernels), and R (zero, so its padding stays zero)
-    o.flow = ar.add(nullptr, c->launchMode == 1 ? (size_t)256 * 2 * std::max(1, I.bf.nstep + I.bb.nstep) : 256);
+    // the grid form's step counters, kFlowLines lines of 256 bytes each
+    // (kFlowStep in the kernels), and R (zero, so its padding stays zero)
+    o.flow = ar.add(nullptr, c->launchMode == 1 ? (size_t)256 * kFlowLines * 2 * std::max(1, I.bf.nstep + I.bb.nstep) : 256);
     o.flowPart = ar.add(nullptr, c->launchMode == 1 ? (size_t)3 * std::max(I.bf.rTotal, I.bb.rTotal) * w : 16);
     std::vector<int> ist(4, 0);
     std::vector<long long> lst(16, 0);

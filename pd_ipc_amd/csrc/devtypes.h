@@ -5,6 +5,11 @@
 
 namespace pdx {
 
+// Lines (256 bytes each) allocated per step counter of the grid form's level
+// sweeps: a launch uses 8 to kFlowLines of them by its workgroup count, each
+// workgroup adds to one, and the waits sum them.
+constexpr int kFlowLines = 64;
+
 // An edge's rest length and stiffness, loaded together.
 struct alignas(16) RestKappa {
   double rest, kap;
@@ -192,7 +197,8 @@ struct InstDev {
   // grid form: per step of the forward then the backward sweep, the tasks done
   // (both phases) and the phase A tasks done, cumulative over the sweeps of one
   // launch (zeroed at its start): [forward done | forward phase A | backward done
-  // | backward phase A], one 256-byte line per counter; and R, the long chains'
+  // | backward phase A], each counter kFlowLines lines of 256 bytes (summed by
+  // the waits); and R, the long chains'
   // rows between the phases (3 x rTotal Real, component-major, zero padding)
   int* flow;
   void* flowPart;

@@ -1774,7 +1774,39 @@ __device__ inline void putW4(WP W, int r, double a, double b, double c) {
 }
 
 constexpr long long kFlowSpin = 1LL << 26;  // polls before a flow wait gives up (~2 s at the back-off cap)
-constexpr int kFlowStride = 64;             // ints between two step counters (one 256-byte line each)
+constexpr int kFlowStride = 64;                     // ints between two counter lines (256 bytes)
+constexpr int kFlowStep = kFlowStride * kFlowLines;  // ints between two step counters
+static_assert(kFlowLines >= 8 && kFlowLines <= kWave && (kFlowLines & (kFlowLines - 1)) == 0,
+              "a step counter's lines are read one per lane");
+
+// Lines of a step counter in use: one per 4 workgroups of the grid, 8 to
+// kFlowLines (a power of two).  Atomics on one address serialise, so a wide grid
+// spreads them (512 x 512 mixed32 at 256 workgroups: 1.22 ms per iteration on 8
+// lines, 0.99 on 64), while a narrow one polls fewer lines.
+__device__ inline int flowLines() {
+  int p = 8;
+  while (p < kFlowLines && 4 * p < (int)gridDim.x) p *= 2;
+  return p;
+}
+
+// Adds v to a step counter, on the line of the calling workgroup (workgroup
+// index mod the lines in use).
+__device__ inline void flowAdd(int* counter, int v) {
+  __hip_atomic_fetch_add(counter + (int)(blockIdx.x & (flowLines() - 1)) * kFlowStride, v, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// A step counter's value: the sum of its lines (lane l loads line l).  Lines
+// only grow, so a sum that reaches a target stays there.
+__device__ inline int flowRead(const int* counter) {
+  const int lane = threadIdx.x & (kWave - 1);
+  int v = lane < flowLines()
+              ? __hip_atomic_load(counter + lane * kFlowStride, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+              : 0;
+#pragma unroll
+  for (int o = kFlowLines / 2; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+  return __builtin_amdgcn_readfirstlane(v);
+}
 
 // Waits (the whole wavefront) until *counter >= target.  Most of the grid's
 // wavefronts wait on some counter at any time, so the polls back off (s_sleep 1,
@@ -1785,8 +1817,7 @@ __device__ inline bool flowWait(const int* counter, int target, int* giveUp) {
   long long polls = 0;
   int nap = 1;
   for (;;) {
-    const int v = __builtin_amdgcn_readfirstlane(__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-    if (v >= target) return true;
+    if (flowRead(counter) >= target) return true;
     ++polls;
     if ((polls & 63) == 0 || polls > kFlowSpin) {
       const int g = __builtin_amdgcn_readfirstlane(__hip_atomic_load(giveUp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
@@ -1873,11 +1904,11 @@ __device__ bool levelSweep(const BlockSweepDev& sw, int m, const Real* dinv, Rea
     const int ja = sw.stepB[st], jb = sw.stepB[st + 1];
     const int ea = sw.stepE[st], eb = sw.stepE[st + 1];
     const int nA = tb - ta, nB = jb - ja, nE = eb - ea;
-    const int* prevOn = st > 0 ? done + (size_t)(st - 1) * kFlowStride : prev;
+    const int* prevOn = st > 0 ? done + (size_t)(st - 1) * kFlowStep : prev;
     const int prevFor = st > 0 ? rep * levelStepTasks(sw, st - 1) : prevTarget;
     const int dbase = sw.stepDense[st];
-    int* doneSt = done + (size_t)st * kFlowStride;
-    int* doneASt = doneA + (size_t)st * kFlowStride;
+    int* doneSt = done + (size_t)st * kFlowStep;
+    int* doneASt = doneA + (size_t)st * kFlowStep;
     bool waited = false;
     int mine = 0;  // tasks of this phase the wavefront finished
     for (int t = ta + gw; t < tb; t += GW) {
@@ -2014,8 +2045,8 @@ __device__ bool levelSweep(const BlockSweepDev& sw, int m, const Real* dinv, Rea
     if (mine > 0) {
       storesPerformed();
       if (lane == 0) {
-        __hip_atomic_fetch_add(doneASt, mine, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_fetch_add(doneSt, mine, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        flowAdd(doneASt, mine);
+        flowAdd(doneSt, mine);
       }
     }
     int j = gw - (nA + nE) % GW;
@@ -2085,7 +2116,7 @@ __device__ bool levelSweep(const BlockSweepDev& sw, int m, const Real* dinv, Rea
     }
     if (mineB > 0) {
       storesPerformed();
-      if (lane == 0) __hip_atomic_fetch_add(doneSt, mineB, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (lane == 0) flowAdd(doneSt, mineB);
     }
   }
   return true;
@@ -2192,7 +2223,7 @@ __global__ void __launch_bounds__(512) gridSubstepKernel(const InstDev* __restri
   // the first sweep by the grid barrier after the first right-hand side)
   const int nF = I.bfwd.nstep, nB = I.bbwd.nstep;
   if (!kDense && blockIdx.x == 0)
-    for (int i = threadIdx.x; i < 2 * (nF + nB); i += blockDim.x) I.flow[(size_t)i * kFlowStride] = 0;
+    for (int i = threadIdx.x; i < 2 * (nF + nB) * kFlowLines; i += blockDim.x) I.flow[(size_t)i * kFlowStride] = 0;
   int rep = 0;  // sweeps of this launch
   const int lastF = nF > 0 ? levelStepTasks(I.bfwd, nF - 1) : 0;
   for (int sub = 0; sub < nsub; ++sub) {
@@ -2278,12 +2309,12 @@ __global__ void __launch_bounds__(512) gridSubstepKernel(const InstDev* __restri
         // finish, and one grid barrier follows
         ++rep;
         int* doneF = I.flow;
-        int* doneB = I.flow + (size_t)2 * nF * kFlowStride;
+        int* doneB = I.flow + (size_t)2 * nF * kFlowStep;
         Real* R = static_cast<Real*>(I.flowPart);
-        if (levelSweep<Real, true, kIdx16>(I.bfwd, I.m, static_cast<const Real*>(I.dinv), (Real*)W, R, Pw, none, doneF, doneF + (size_t)nF * kFlowStride,
+        if (levelSweep<Real, true, kIdx16>(I.bfwd, I.m, static_cast<const Real*>(I.dinv), (Real*)W, R, Pw, none, doneF, doneF + (size_t)nF * kFlowStep,
                                            rep, nullptr, 0, I.pair.giveUp))
-          levelSweep<Real, false, kIdx16>(I.bbwd, I.m, static_cast<const Real*>(I.dinv), (Real*)W, R, Pw, put, doneB, doneB + (size_t)nB * kFlowStride, rep,
-                                           nF > 0 ? doneF + (size_t)(nF - 1) * kFlowStride : nullptr, rep * lastF,
+          levelSweep<Real, false, kIdx16>(I.bbwd, I.m, static_cast<const Real*>(I.dinv), (Real*)W, R, Pw, put, doneB, doneB + (size_t)nB * kFlowStep, rep,
+                                           nF > 0 ? doneF + (size_t)(nF - 1) * kFlowStep : nullptr, rep * lastF,
                                            I.pair.giveUp);
         grid.sync();
       }
@@ -2373,7 +2404,7 @@ __global__ void __launch_bounds__(512) gridSolveKernel(const InstDev* __restrict
   for (int v = gt; v < I.n; v += gs) putW4<Real>(W, I.vrank[v], b[3 * v], b[3 * v + 1], b[3 * v + 2]);
   const int nF = I.bfwd.nstep, nB = I.bbwd.nstep;
   if (blockIdx.x == 0)
-    for (int i = threadIdx.x; i < 2 * (nF + nB); i += blockDim.x) I.flow[(size_t)i * kFlowStride] = 0;
+    for (int i = threadIdx.x; i < 2 * (nF + nB) * kFlowLines; i += blockDim.x) I.flow[(size_t)i * kFlowStride] = 0;
   grid.sync();
   auto none = [](int, Real, Real, Real) {};
   const int* rankv = I.rankv;
@@ -2385,12 +2416,12 @@ __global__ void __launch_bounds__(512) gridSolveKernel(const InstDev* __restrict
   };
   const int lastF = nF > 0 ? levelStepTasks(I.bfwd, nF - 1) : 0;
   int* doneF = I.flow;
-  int* doneB = I.flow + (size_t)2 * nF * kFlowStride;
+  int* doneB = I.flow + (size_t)2 * nF * kFlowStep;
   Real* R = static_cast<Real*>(I.flowPart);
-  if (levelSweep<Real, true, kIdx16>(I.bfwd, I.m, static_cast<const Real*>(I.dinv), (Real*)W, R, Pw, none, doneF, doneF + (size_t)nF * kFlowStride, 1,
+  if (levelSweep<Real, true, kIdx16>(I.bfwd, I.m, static_cast<const Real*>(I.dinv), (Real*)W, R, Pw, none, doneF, doneF + (size_t)nF * kFlowStep, 1,
                                      nullptr, 0, I.pair.giveUp))
-    levelSweep<Real, false, kIdx16>(I.bbwd, I.m, static_cast<const Real*>(I.dinv), (Real*)W, R, Pw, put, doneB, doneB + (size_t)nB * kFlowStride, 1,
-                                     nF > 0 ? doneF + (size_t)(nF - 1) * kFlowStride : nullptr, lastF, I.pair.giveUp);
+    levelSweep<Real, false, kIdx16>(I.bbwd, I.m, static_cast<const Real*>(I.dinv), (Real*)W, R, Pw, put, doneB, doneB + (size_t)nB * kFlowStep, 1,
+                                     nF > 0 ? doneF + (size_t)(nF - 1) * kFlowStep : nullptr, lastF, I.pair.giveUp);
 }
 
 // Dense solve mode, x = Linv b: W = P b (rank space) ...
