@@ -441,9 +441,10 @@ def large_workload(args):
         e = Engine(prec, "fast")
         e.add(c, cfg)
         if args.no_tune:
-            # the tuner's choices on this cloth: grid form, nested dissection, 256 threads
+            # the tuner's choices on this cloth: grid form of 256 workgroups, nested
+            # dissection, 256 threads
             e.set_ordering("nd")
-            e.set_launch("grid", 128 if prec == "fp64" else 64)
+            e.set_launch("grid", 256)
             e.set_schedule(256, 32)
         t0 = time.perf_counter()
         e.factor()
