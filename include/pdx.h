@@ -126,9 +126,11 @@ typedef struct {
   int bwd_steps;
   double analyze_seconds;
   double factorize_seconds;
-  /* device time of the instance's workgroup per phase, summed over substeps (batch
-   * form only): gather (rhs + residual), forward sweep, backward sweep, predict +
-   * velocity, criterion reductions (always kept, the reference's stage buckets);
+  /* device time per phase, summed over substeps: gather (rhs + residual), forward
+   * sweep, backward sweep, predict + velocity, criterion reductions (always kept,
+   * the reference's stage buckets).  Batch form: the instance's workgroup; grid
+   * form and dense mode: workgroup 0 between the grid's barriers, with the whole
+   * solve in the forward bucket (its sweeps overlap across workgroups);
    * then, only in a library built with PDX_PHASE_TIMERS=1 (zero otherwise), fast
    * mode, both sweeps: run streaming of thread 0, its wait at the step barrier,
    * finalisation of wavefront 0, its wait at the second barrier */

@@ -2226,8 +2226,14 @@ __global__ void __launch_bounds__(512) gridSubstepKernel(const InstDev* __restri
     for (int i = threadIdx.x; i < 2 * (nF + nB) * kFlowLines; i += blockDim.x) I.flow[(size_t)i * kFlowStride] = 0;
   int rep = 0;  // sweeps of this launch
   const int lastF = nF > 0 ? levelStepTasks(I.bfwd, nF - 1) : 0;
+  // stage buckets (gather, solve, -, predict + velocity, criterion), read by
+  // thread 0 of workgroup 0 at points the grid's barriers align; the level
+  // sweeps overlap forward and backward across workgroups, so the solve is one
+  // bucket
+  long long tacc[5] = {0, 0, 0, 0, 0};
   for (int sub = 0; sub < nsub; ++sub) {
     // predict (Integrator.cpp:447-463), displacement scale, first right-hand side
+    const long long tp0 = lead ? tickStage() : 0;
     double lscale = 0.0;
     for (int v = gt; v < n; v += gs) {
       for (int d = 0; d < 3; ++d) {
@@ -2239,6 +2245,7 @@ __global__ void __launch_bounds__(512) gridSubstepKernel(const InstDev* __restri
         lscale = fmax(lscale, fabs(xh - p));
       }
     }
+    const long long tp1 = lead ? tickStage() : 0;
     for (int v = gt; v < n; v += gs) {
       const int r = I.vrank[v];
       if (kExact) {
@@ -2256,6 +2263,10 @@ __global__ void __launch_bounds__(512) gridSubstepKernel(const InstDev* __restri
     double scale = gridMax(grid, lscale, red, gslots, buf);
     buf ^= 1;
     scale = fmax(scale, I.gScaleH2);
+    if (lead) {
+      tacc[3] += tp1 - tp0;
+      tacc[0] += tickStage() - tp1;
+    }
 
     int used = 0, conv = 0;
     double lastRes = -1.0;
@@ -2283,6 +2294,7 @@ __global__ void __launch_bounds__(512) gridSubstepKernel(const InstDev* __restri
       }
     };
     for (int k = 0; k < I.maxIter; ++k) {
+      const long long ts0 = lead ? tickStage() : 0;
       if (kDense) {
         if (kExact) {
           gridDense<Real>(grid, I, W, put);
@@ -2318,6 +2330,7 @@ __global__ void __launch_bounds__(512) gridSubstepKernel(const InstDev* __restri
                                            I.pair.giveUp);
         grid.sync();
       }
+      const long long ts1 = lead ? tickStage() : 0;
       const bool more = (k + 1 < I.maxIter);
       double ldiff = 0.0, lres = 0.0;
       for (int v = gt; v < n; v += gs) {
@@ -2347,12 +2360,18 @@ __global__ void __launch_bounds__(512) gridSubstepKernel(const InstDev* __restri
           }
         }
       }
+      const long long ts2 = lead ? tickStage() : 0;
       double diff = 0.0, residual = -1.0;
       if (I.wantResidual)
         gridMax2(grid, ldiff, lres, red, gslots, buf, diff, residual);
       else
         diff = gridMax(grid, ldiff, red, gslots, buf);
       buf ^= 1;
+      if (lead) {
+        tacc[1] += ts1 - ts0;
+        tacc[0] += ts2 - ts1;
+        tacc[4] += tickStage() - ts2;
+      }
       const double rel = (scale > 0.0) ? diff / scale : diff;
       const bool absHit = (I.absTol > 0.0) && (diff <= I.absTol);
       const bool relHit = (I.relTol > 0.0) && (rel <= I.relTol);
@@ -2366,6 +2385,7 @@ __global__ void __launch_bounds__(512) gridSubstepKernel(const InstDev* __restri
       }
     }
     // velocity + damping + re-pin (Integrator.cpp:774-791)
+    const long long tv0 = lead ? tickStage() : 0;
     for (int v = gt; v < n; v += gs) {
       const bool pinnedV = I.pinned[v] != 0;
       for (int d = 0; d < 3; ++d) {
@@ -2378,6 +2398,7 @@ __global__ void __launch_bounds__(512) gridSubstepKernel(const InstDev* __restri
       }
     }
     if (lead) {
+      tacc[3] += tickStage() - tv0;
       I.istats[0] = used;
       I.istats[1] = conv;
       I.istats[2] += conv;
@@ -2387,6 +2408,8 @@ __global__ void __launch_bounds__(512) gridSubstepKernel(const InstDev* __restri
     }
     grid.sync();
   }
+  if (lead)
+    for (int q = 0; q < 5; ++q) I.lstats[2 + q] += tacc[q];
 }
 
 // Global solve x = L^{-1} b on one instance with the whole grid.

@@ -230,3 +230,26 @@ def test_stage_buckets_always_kept():
     assert sum(ph) <= 1.05 * kernel_s and sum(ph) >= 0.5 * kernel_s
     text, _ = report.run(["--grid", "20", "20", "--steps", "5"])
     assert "global solve" in text and "project+scatter+residual gather" in text
+
+
+@pytest.mark.parametrize("form", ["grid", "dense"])
+def test_stage_buckets_grid_and_dense(form):
+    """The grid form and the dense mode keep the stage buckets too (workgroup 0 at
+    the grid's barriers): gather, solve (one bucket: the sweeps overlap across
+    workgroups), predict + velocity and criterion, each positive, the backward
+    bucket empty, together within the kernel's time."""
+    _say("stage buckets 48x40 FP64", form, "10 substeps")
+    c = scenes.grid_cloth(48, 40, 2300.0, shear=460.0, bend=46.0)
+    cfg = SimConfig.fixed_iterations(10, scenes.H600)
+    e = Engine("fp64", "dense" if form == "dense" else "fast")
+    e.add(c, cfg)
+    if form == "grid":
+        e.set_ordering("nd")
+        e.set_launch("grid", 32)
+    e.step(10)
+    ph = e.stats(0)["phase_seconds"][:5]
+    kernel_s = e.last_kernel_ms() * 1e-3
+    e.close()
+    print("stage seconds", ph, "kernel", kernel_s)
+    assert ph[0] > 0.0 and ph[1] > 0.0 and ph[3] > 0.0 and ph[4] > 0.0 and ph[2] == 0.0
+    assert 0.5 * kernel_s <= sum(ph) <= 1.05 * kernel_s
