@@ -1851,6 +1851,7 @@ struct Vec16<float> {
 };
 
 constexpr int kLB = 8;  // rows per phase B task of a level sweep (the panels' height)
+constexpr int kEU = 4;  // union columns per lane per pass of a dense external task
 
 // A level sweep of the grid form (one step per level of whole elimination-tree
 // chains), without grid barriers.  Every step has two phases, each paced by
@@ -1992,21 +1993,34 @@ __device__ bool levelSweep(const BlockSweepDev& sw, int m, const Real* dinv, Rea
         Real acc[kLB][3];
 #pragma unroll
         for (int q = 0; q < kLB; ++q) acc[q][0] = acc[q][1] = acc[q][2] = Real(0);
-        for (int jj = lane; jj < c; jj += kWave) {
-          const int col = gucol[uoff + jj];
-          Real w0, w1, w2;
-          loadW4<Real>(wsrc, col, w0, w1, w2);
-          const PDX_GLOBAL VT* ep = (const PDX_GLOBAL VT*)(gep + (size_t)eoff + (size_t)jj * kLB);
-          VT ev[kLB / V];
+        // kEU columns per lane per pass: their column indices, then their W values
+        // and panel entries are all in flight before the sums, which take the
+        // columns in the lane's order (lane, lane + 64, ...) as one at a time would
+        for (int jb = lane; jb < c; jb += kEU * kWave) {
+          int col[kEU];
 #pragma unroll
-          for (int u = 0; u < kLB / V; ++u) ev[u] = ep[u];
+          for (int u = 0; u < kEU; ++u) col[u] = gucol[uoff + min(jb + u * kWave, c - 1)];
+          Real w[kEU][3];
+          VT ev[kEU][kLB / V];
 #pragma unroll
-          for (int q = 0; q < kLB; ++q) {
-            const Real e = ev[q / V][q % V];
-            acc[q][0] = fmaT(e, w0, acc[q][0]);
-            acc[q][1] = fmaT(e, w1, acc[q][1]);
-            acc[q][2] = fmaT(e, w2, acc[q][2]);
+          for (int u = 0; u < kEU; ++u) {
+            loadW4<Real>(wsrc, col[u], w[u][0], w[u][1], w[u][2]);
+            const PDX_GLOBAL VT* ep =
+                (const PDX_GLOBAL VT*)(gep + (size_t)eoff + (size_t)min(jb + u * kWave, c - 1) * kLB);
+#pragma unroll
+            for (int h = 0; h < kLB / V; ++h) ev[u][h] = ep[h];
           }
+#pragma unroll
+          for (int u = 0; u < kEU; ++u)
+            if (jb + u * kWave < c) {
+#pragma unroll
+              for (int q = 0; q < kLB; ++q) {
+                const Real e = ev[u][q / V][q % V];
+                acc[q][0] = fmaT(e, w[u][0], acc[q][0]);
+                acc[q][1] = fmaT(e, w[u][1], acc[q][1]);
+                acc[q][2] = fmaT(e, w[u][2], acc[q][2]);
+              }
+            }
         }
 #pragma unroll
         for (int q = 0; q < kLB; ++q)
