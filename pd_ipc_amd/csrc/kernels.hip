@@ -1850,6 +1850,16 @@ struct Vec16<float> {
   static constexpr int n = 4;
 };
 
+// 16 bytes of Real at p (16-byte aligned), in LDS or in global memory.
+template <typename Real>
+__device__ inline typename Vec16<Real>::T vec16At(const PDX_LDS Real* p) {
+  return *(const PDX_LDS typename Vec16<Real>::T*)p;
+}
+template <typename Real>
+__device__ inline typename Vec16<Real>::T vec16At(const PDX_GLOBAL Real* p) {
+  return *(const PDX_GLOBAL typename Vec16<Real>::T*)p;
+}
+
 constexpr int kLB = 8;  // rows per phase B task of a level sweep (the panels' height)
 constexpr int kEU = 4;  // union columns per lane per pass of a dense external task
 
@@ -2162,11 +2172,24 @@ __device__ PDX_INLINE void denseRows(const PDX_GLOBAL Real* L, int m, int ld, WP
       VT lv[kDR];
 #pragma unroll
       for (int q = 0; q < kDR; ++q) lv[q] = row[q][jv];
+      // the V columns' 3V values of W are 48 contiguous bytes from a 16-byte
+      // boundary: three 16-byte loads, or scalar ones at the end (padding
+      // columns: zero in L, outside W)
+      Real wv[3 * V];
+      if (jv * V + V <= m) {
+#pragma unroll
+        for (int h = 0; h < 3; ++h) {
+          const VT t = vec16At<Real>(W + 3 * V * jv + h * V);
+#pragma unroll
+          for (int u = 0; u < V; ++u) wv[h * V + u] = t[u];
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < 3 * V; ++i) wv[i] = 3 * V * jv + i < 3 * m ? W[3 * V * jv + i] : Real(0);
+      }
 #pragma unroll
       for (int u = 0; u < V; ++u) {
-        const int j = jv * V + u;
-        const bool in = j < m;  // padding columns: zero in L, outside W
-        const Real w0 = in ? W[3 * j] : Real(0), w1 = in ? W[3 * j + 1] : Real(0), w2 = in ? W[3 * j + 2] : Real(0);
+        const Real w0 = wv[3 * u], w1 = wv[3 * u + 1], w2 = wv[3 * u + 2];
 #pragma unroll
         for (int q = 0; q < kDR; ++q) {
           const Real l = lv[q][u];
