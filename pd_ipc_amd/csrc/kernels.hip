@@ -1996,30 +1996,37 @@ __device__ bool levelSweep(const BlockSweepDev& sw, int m, const Real* dinv, Rea
         const int kr = __builtin_amdgcn_readfirstlane(et[3]);
         const int rp = __builtin_amdgcn_readfirstlane(et[4]);
         const int k0 = kr & 0xffffff, nrows = kr >> 24;
+        Real acc[kLB][3];
+#pragma unroll
+        for (int q = 0; q < kLB; ++q) acc[q][0] = acc[q][1] = acc[q][2] = Real(0);
+        // kEU columns per lane per pass: their column indices and panel entries,
+        // then their W values, all in flight before the sums, which take the
+        // columns in the lane's order (lane, lane + 64, ...) as one at a time
+        // would.  The first pass's indices and entries do not depend on the
+        // step's wait and are issued before it (by every lane, clamped, so that
+        // the whole wavefront waits together).
+        int col[kEU];
+        VT ev[kEU][kLB / V];
+        auto loadPass = [&](int jb) {
+#pragma unroll
+          for (int u = 0; u < kEU; ++u) {
+            const int jj = min(jb + u * kWave, c - 1);
+            col[u] = gucol[uoff + jj];
+            const PDX_GLOBAL VT* ep = (const PDX_GLOBAL VT*)(gep + (size_t)eoff + (size_t)jj * kLB);
+#pragma unroll
+            for (int h = 0; h < kLB / V; ++h) ev[u][h] = ep[h];
+          }
+        };
+        loadPass(lane);
         if (!waited) {
           if (prevOn && !flowWait(prevOn, prevFor, giveUp)) return false;
           waited = true;
         }
-        Real acc[kLB][3];
-#pragma unroll
-        for (int q = 0; q < kLB; ++q) acc[q][0] = acc[q][1] = acc[q][2] = Real(0);
-        // kEU columns per lane per pass: their column indices, then their W values
-        // and panel entries are all in flight before the sums, which take the
-        // columns in the lane's order (lane, lane + 64, ...) as one at a time would
         for (int jb = lane; jb < c; jb += kEU * kWave) {
-          int col[kEU];
-#pragma unroll
-          for (int u = 0; u < kEU; ++u) col[u] = gucol[uoff + min(jb + u * kWave, c - 1)];
+          if (jb != lane) loadPass(jb);
           Real w[kEU][3];
-          VT ev[kEU][kLB / V];
 #pragma unroll
-          for (int u = 0; u < kEU; ++u) {
-            loadW4<Real>(wsrc, col[u], w[u][0], w[u][1], w[u][2]);
-            const PDX_GLOBAL VT* ep =
-                (const PDX_GLOBAL VT*)(gep + (size_t)eoff + (size_t)min(jb + u * kWave, c - 1) * kLB);
-#pragma unroll
-            for (int h = 0; h < kLB / V; ++h) ev[u][h] = ep[h];
-          }
+          for (int u = 0; u < kEU; ++u) loadW4<Real>(wsrc, col[u], w[u][0], w[u][1], w[u][2]);
 #pragma unroll
           for (int u = 0; u < kEU; ++u)
             if (jb + u * kWave < c) {
@@ -2088,6 +2095,11 @@ __device__ bool levelSweep(const BlockSweepDev& sw, int m, const Real* dinv, Rea
 #pragma unroll
       for (int q = 0; q < kLB; ++q)  // rows past the panel repeat its last row (their sums are dropped)
         row[q] = (const PDX_GLOBAL VT*)(gx + xoff + (size_t)min(q, nrows - 1) * ld);
+      // the panel's first vectors do not depend on the wait: issued before it
+      const int nv = ld / V;
+      VT lv[kLB];
+#pragma unroll
+      for (int q = 0; q < kLB; ++q) lv[q] = row[q][min(lane, nv - 1)];
       if (!waitedB) {
         if (!flowWait(doneASt, rep * (nA + nE), giveUp)) return false;
         waitedB = true;
@@ -2095,11 +2107,10 @@ __device__ bool levelSweep(const BlockSweepDev& sw, int m, const Real* dinv, Rea
       Real acc[kLB][3];
 #pragma unroll
       for (int q = 0; q < kLB; ++q) acc[q][0] = acc[q][1] = acc[q][2] = Real(0);
-      const int nv = ld / V;
       for (int jv = lane; jv < nv; jv += kWave) {
-        VT lv[kLB];
+        if (jv != lane)
 #pragma unroll
-        for (int q = 0; q < kLB; ++q) lv[q] = row[q][jv];
+          for (int q = 0; q < kLB; ++q) lv[q] = row[q][jv];
         const int off = (rp + jv * V) * (int)sizeof(Real);
         const VT w0 = __builtin_bit_cast(VT, __builtin_amdgcn_raw_buffer_load_b128(rsrc, off, 0, 16));
         const VT w1 = __builtin_bit_cast(VT, __builtin_amdgcn_raw_buffer_load_b128(rsrc, off + rT * (int)sizeof(Real), 0, 16));
