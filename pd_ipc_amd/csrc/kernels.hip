@@ -1927,6 +1927,20 @@ __device__ bool levelSweep(const BlockSweepDev& sw, int m, const Real* dinv, Rea
       const Real dv = kFwd ? Real(0) : gldinv[(size_t)t * kWave + lane];
       const int doff = gview<int>(sw.laneDense)[(size_t)t * kWave + lane];
       const int r0 = sw.taskRun[t], nr = sw.taskRun[t + 1] - r0;
+      const int k = mt.x;
+      const int info = mt.w;
+      const bool act = k >= 0;
+      const bool toR = (info >> 28) & 1;
+      // issued before the step's wait, as neither depends on it: (FP32) the
+      // lane's row of its short chain's inverse, into registers (FP64 reads it
+      // as it finishes: its registers would spill), and, in the forward sweep,
+      // the row's right-hand side (final since the barrier before the sweep; a
+      // backward row's value is the forward sweep's, which only the wait orders)
+      constexpr bool kRegRows = sizeof(Real) == 4;
+      Real xc[kRegRows ? kMaxChain : 1];
+      if constexpr (kRegRows) loadChainRow<Real>(gdense + dbase, mt, toR ? 0 : doff, xc);
+      Real a0 = 0, a1 = 0, a2 = 0;
+      if (kFwd && act) loadW4<Real>(wsrc, k, a0, a1, a2);
       for (int q0 = 0; q0 < nr; q0 += 2 * kWave) {
         RunRaw<Real, kIdx16> raw[2];
 #pragma unroll
@@ -1950,21 +1964,19 @@ __device__ bool levelSweep(const BlockSweepDev& sw, int m, const Real* dinv, Rea
       }
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       __builtin_amdgcn_wave_barrier();
-      const int k = mt.x;
-      const int info = mt.w;
-      const bool act = k >= 0;
-      const bool toR = (info >> 28) & 1;
-      Real a0 = 0, a1 = 0, a2 = 0;
       if (act) {
-        loadW4<Real>(wsrc, k, a0, a1, a2);
         if (!kFwd) {
+          loadW4<Real>(wsrc, k, a0, a1, a2);
           a0 = a0 * dv;
           a1 = a1 * dv;
           a2 = a2 * dv;
         }
         subtractPartials<Real>(Pw, mt.y, mt.z, a0, a1, a2);
       }
-      segmentFinish<Real>(gdense + dbase + (toR ? 0 : doff), info, lane, act, a0, a1, a2);
+      if constexpr (kRegRows)
+        segmentFinishR<Real>(xc, info, lane, a0, a1, a2);
+      else
+        segmentFinish<Real>(gdense + dbase + (toR ? 0 : doff), info, lane, act, a0, a1, a2);
       if (act) {
         if (toR) {
           agentStore(Rp + doff, a0);
