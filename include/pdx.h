@@ -309,8 +309,9 @@ int pdx_set_launch(pdx_ctx* ctx, int mode, int grid_blocks);
  * choice and keeps the fastest.  A context of one cloth in the fast mode first
  * probes the fill order (PDX_ORDER_AMD / PDX_ORDER_ND; a change re-analyses and
  * refactors, which factorize_count and analyze_count count) and the launch form
- * (PDX_LAUNCH_BATCH, or PDX_LAUNCH_GRID with 16, 32, ..., 256 workgroups);
- * pdx_get_launch reads the choice.  Then every context probes the launch shape
+ * (PDX_LAUNCH_BATCH at the current shape, or PDX_LAUNCH_GRID with 16, 32, ...,
+ * 256 workgroups of 256 or 512 threads; a probe within 1.5 x of the best so far
+ * runs twice and keeps the faster); pdx_get_launch reads the choice.  Then every context probes the launch shape
  * (threads per workgroup 256 / 512 x chain cap 8 / 16 / 24 / 32 in fast mode; in
  * the dense solve mode, 16, 32, 64, ... workgroups, reported in best_block with
  * best_cap 0); then, for a FP32-solve fast batch, the split threshold

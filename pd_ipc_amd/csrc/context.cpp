@@ -2909,45 +2909,66 @@ int pdx_autotune(pdx_ctx* c, int probe_substeps, int* best_block, int* best_cap,
   }
   const bool fast = c->mode == PDX_SOLVE_FAST;
   // A single cloth in the fast mode: the fill order (the current one first, then
-  // the other) and the launch form (one workgroup, or the grid with 16 .. 256
-  // workgroups) are probed at the current shape; the shape search below runs in
-  // the form chosen.  A change of order re-analyses and refactors the cloth
+  // the other) and the launch form (one workgroup at the current shape, or the
+  // grid with 16 .. 256 workgroups of 256 or 512 threads) are probed; the shape
+  // search below runs in the form chosen.  A change of order re-analyses and refactors the cloth
   // (factorize_count and analyze_count count it).
   if (fast && c->inst.size() == 1 && !c->rollMode && c->inst[0].cfg.chebRho <= 0.0) {
     double formMs = 1e300;
-    int bo = c->ordering, bm = c->launchMode, bgb = c->gridBlocks;
+    int bo = c->ordering, bm = c->launchMode, bgb = c->gridBlocks, bblk = c->block;
+    const int blk0 = c->block;
     const int orders[2] = {c->ordering, c->ordering == PDX_ORDER_AMD ? PDX_ORDER_ND : PDX_ORDER_AMD};
     for (int ord : orders) {
       // the state before the probes: a new analysis orders by these positions
       if (c->uploaded && (rc = restore())) return rc;
       if ((rc = pdx_set_ordering(c, ord))) return rc;
       for (int form : {PDX_LAUNCH_BATCH, PDX_LAUNCH_GRID}) {
-        for (int g = 16; g <= 256; g *= 2) {
-          const int gb = form == PDX_LAUNCH_GRID ? g : 0;
-          if (c->uploaded && (rc = restore())) return rc;
-          if ((rc = pdx_set_launch(c, form, gb))) return rc;
-          if ((rc = ensureReady(c))) {
-            if (rc == PDX_EUNSUPPORTED) break;  // this form does not fit the cloth
-            return rc;
+        // the grid form at both workgroup sizes (the best count depends on the
+        // size: 512 x 512 ties at 128 and 256 workgroups of 512 threads, and 256
+        // of 256 threads beat both); the batch form once, at the current shape
+        bool fits = true;
+        for (int blk : {256, 512}) {
+          const int useBlk = form == PDX_LAUNCH_BATCH ? blk0 : blk;
+          for (int g = 16; g <= 256 && fits; g *= 2) {
+            const int gb = form == PDX_LAUNCH_GRID ? g : 0;
+            if (c->uploaded && (rc = restore())) return rc;
+            if ((rc = pdx_set_launch(c, form, gb))) return rc;
+            if ((rc = pdx_set_schedule(c, useBlk, c->chainCap))) return rc;
+            if ((rc = ensureReady(c))) {
+              if (rc != PDX_EUNSUPPORTED) return rc;
+              fits = false;  // this form does not fit the cloth
+              break;
+            }
+            if (form == PDX_LAUNCH_GRID && gb > gridBlocksMax(c->block, c->lds, c->prec == PDX_PREC_MIXED32, c->idx16))
+              break;
+            // one launch per probe, and a second one for a probe within 1.5 x of
+            // the best so far, keeping the faster: the grid sizes near the best
+            // differ by less than one launch's spread on large cloths
+            double ms = 1e300;
+            for (int rep = 0; rep < 2; ++rep) {
+              if ((rc = restore())) return rc;
+              if ((rc = launch(c, probe_substeps))) return rc;
+              if ((rc = pdx_sync(c))) return rc;
+              ms = std::min(ms, (double)c->lastKernelMs);
+              if (ms > 1.5 * formMs) break;
+            }
+            if (ms < formMs) {
+              formMs = ms;
+              bo = ord;
+              bm = form;
+              bgb = gb;
+              bblk = useBlk;
+            }
+            if (form == PDX_LAUNCH_BATCH) break;
           }
-          if (form == PDX_LAUNCH_GRID && gb > gridBlocksMax(c->block, c->lds, c->prec == PDX_PREC_MIXED32, c->idx16))
-            break;
-          if ((rc = restore())) return rc;
-          if ((rc = launch(c, probe_substeps))) return rc;
-          if ((rc = pdx_sync(c))) return rc;
-          if (c->lastKernelMs < formMs) {
-            formMs = c->lastKernelMs;
-            bo = ord;
-            bm = form;
-            bgb = gb;
-          }
-          if (form == PDX_LAUNCH_BATCH) break;
+          if (form == PDX_LAUNCH_BATCH || !fits) break;
         }
       }
     }
     if ((rc = restore())) return rc;
     if ((rc = pdx_set_ordering(c, bo))) return rc;
     if ((rc = pdx_set_launch(c, bm, bgb))) return rc;
+    if ((rc = pdx_set_schedule(c, bblk, c->chainCap))) return rc;
     if ((rc = ensureReady(c))) return rc;
   }
   const int blocks[2] = {256, 512};
