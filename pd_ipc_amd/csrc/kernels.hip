@@ -727,6 +727,17 @@ __device__ inline bool stopTest(const InstDev& I, double scale, double diff, dou
   return (I.resTol > 0.0) ? resHit : (absHit || relHit);
 }
 
+// Runs are padded with entries of value 0 at column 0, and rank 0 is in the
+// lead's half: a partner's W holds no value there until the first exchange, only
+// what the LDS held before this workgroup (a NaN or infinity pattern there turns
+// 0 * W[0] into NaN).  The partner zeroes it before its first sweep (visible
+// after the next barrier): every padded product is then +-0, which leaves a
+// partial sum's bits as they are with any finite W[0].
+template <typename Real, typename WP>
+__device__ inline void clearPartnerPad(const InstDev& I, WP W) {
+  if (I.pairRole == 2 && threadIdx.x < 3) W[threadIdx.x] = (Real)0;
+}
+
 // Copies W[3 lo, 3 hi) between the workgroup's LDS and the pair's exchange buffer.
 template <typename Real, typename WP>
 __device__ inline void pairPut(const InstDev& I, WP W, int lo, int hi) {
@@ -1151,14 +1162,7 @@ __device__ void substepBody(const InstDev& I, WP W, int nsub, int subBase, int f
   }
 
   if (kBlocked) loadStepTables(I);  // visible after the first barrier below
-  if (paired && I.pairRole == 2 && tid < 3) {
-    // Runs are padded with entries of value 0 at column 0, and rank 0 is in the
-    // lead's half: the partner's W holds no value there until the first
-    // exchange, only what the LDS held before this workgroup (a NaN or infinity
-    // pattern there turns 0 * W[0] into NaN).  Zero makes every padded product
-    // +-0, which leaves a partial sum's bits as they are with any finite W[0].
-    W[tid] = (Real)0;
-  }
+  if (paired) clearPartnerPad<Real>(I, W);
   long long tk[4] = {0, 0, 0, 0};
   long long tacc[5] = {0, 0, 0, 0, 0};
   SweepTimes swt;
@@ -2553,7 +2557,10 @@ __global__ void __launch_bounds__(512) solveKernel(const InstDev* __restrict__ i
 
 template <typename Real, bool kBlocked, bool kIdx16, typename WP>
 __device__ void solveBody(const InstDev& I, WP W, const double* __restrict__ b, double* __restrict__ xout) {
-  if (kBlocked) loadStepTables(I);
+  if (kBlocked) {
+    loadStepTables(I);
+    clearPartnerPad<Real>(I, W);
+  }
   const int nv = ownCount(I);
   for (int i = threadIdx.x; i < nv; i += blockDim.x) {
     const int v = ownVertex(I, i);

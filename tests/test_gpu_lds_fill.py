@@ -1,7 +1,9 @@
 """No kernel reads LDS it has not written first.  A workgroup starts with whatever
 the previous workgroup on its CU left in LDS; every CU's LDS is filled with a NaN
 or infinity pattern (tests/liblds_fill.so) before each launch, and the results
-must be those of a run without the fill, bit for bit, in every launch form.
+must be those of a run without the fill, bit for bit: the substep in every
+launch form, the stand-alone solve, and the differentiable rollout with its
+adjoint.
 
 The split form is the case that needed it: runs are padded with value-0
 entries at column 0, rank 0 lies in the lead's half, and the partner's first
@@ -120,3 +122,63 @@ def test_split_partner_after_lds_fill_equals_one_workgroup():
         split.step(1)
     for i in range(len(cloths)):
         assert _same(one.state(i)[0], split.state(i)[0]), i
+
+
+@pytest.mark.parametrize("form", ["fp64-exact", "fp64-fast", "mixed32-fast-split", "fp64-dense", "fp64-grid"])
+def test_solve_after_lds_fill(form):
+    """The stand-alone global solve (pdx_solve3) after every CU's LDS was filled
+    with NaN: the bits of a solve without the fill."""
+    fill = _fill()
+    single = FORMS[form][1] == "dense" or FORMS[form][2] == "grid"
+    cloths = [_batch_cloths()[1]] if single else _batch_cloths()
+    cfg = SimConfig.fixed_iterations(12, scenes.H600)
+    clean = _engine(form, cloths, cfg)
+    filled = _engine(form, cloths, cfg)
+    rng = np.random.default_rng(7)
+    for i, c in enumerate(cloths):
+        b = rng.standard_normal(3 * c.n)
+        _say(form, "solve, instance", i)
+        xc = clean.solve(i, b)
+        assert fill(PATTERNS["nan"], 4096) == 0
+        xf = filled.solve(i, b)
+        assert np.all(np.isfinite(xf)), (form, i)
+        assert _same(xc, xf), (form, i)
+
+
+@pytest.mark.parametrize("prec,order", [("fp64", "amd"), ("mixed32", "nd")])
+def test_rollout_and_adjoint_after_lds_fill(prec, order):
+    """The differentiable rollout and its adjoint (rollout segments, reverse
+    kernel) after every CU's LDS was filled with NaN before each call: the
+    states and gradients of a run without the fill, bit for bit."""
+    fill = _fill()
+    cloths = _batch_cloths()[1:4]
+    cfg = SimConfig.fixed_iterations(6, scenes.H600)
+    nsub = 4
+    out = []
+    for use_fill in (False, True):
+        e = Engine(prec, "fast")
+        for i, c in enumerate(cloths):
+            e.add(c, cfg)
+        if order != "amd":
+            e.set_ordering(order)
+        for i, c in enumerate(cloths):
+            e.set_pin_trajectory(i, scenes.pin_trajectory(c, nsub, cfg.dt))
+        _say(prec, order, "rollout, fill", use_fill)
+        if use_fill:
+            assert fill(PATTERNS["nan"], 4096) == 0
+        e.rollout(nsub, 2)
+        xs = [e.state(i)[0] for i in range(len(cloths))]
+        for i, x in enumerate(xs):
+            e.set_adjoint_seed(i, 2.0 * x)
+        _say(prec, order, "adjoint, fill", use_fill)
+        if use_fill:
+            assert fill(PATTERNS["nan"], 4096) == 0
+        e.adjoint()
+        gs = [e.gradients(i) for i in range(len(cloths))]
+        e.close()
+        out.append((xs, gs))
+    for i in range(len(cloths)):
+        assert _same(out[0][0][i], out[1][0][i]), i
+        for k in out[0][1][i]:
+            assert np.all(np.isfinite(out[1][1][i][k])), (i, k)
+            assert _same(out[0][1][i][k], out[1][1][i][k]), (i, k)
